@@ -1,0 +1,178 @@
+// ddla/ddla.h -- C++ API of the MI355X-native LibDDLA Cholesky path.
+//
+// Drop-in for the reference's public templates on this path (same names,
+// argument meaning and error behaviour), for a 1x1 process grid on one GPU:
+//   handle lifecycle        include/ddla/ddla_handle_t.h:51-67
+//   memory helpers          include/ddla/ddla_handle_t.h:130-284
+//   descriptor              include/ddla/ddla_desc.h:10-135
+//   ppotrf / ppotrs / pposv include/ddla/ddla.h:1065, 1149, 1184
+//   gemmVbatched            include/ddla/ddla.h:1836
+//   potrf                   include/ddla/ddla.h:2160
+//   diag_logdet             include/ddla/ddla.h:2901
+//   check / size / run      src/Cholesky/cholesky_workspace.h:133-249
+// Differences: there is no MPI (ddlaSet takes the grid shape only and
+// accepts 1 x 1), and only the GPU backend exists (ddlaInit with the CPU
+// backend throws, as the reference does for a backend that is not built).
+#ifndef DDLA_MI355X_DDLA_H
+#define DDLA_MI355X_DDLA_H
+
+#include <algorithm>
+#include <complex>
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+
+namespace ddla {
+
+enum class DdlaBackend { CPU, GPU };
+
+enum class ddlaStatus_t {
+    DDLA_STATUS_SUCCESS = 0,
+    DDLA_STATUS_INVALID_HANDLE,
+    DDLA_STATUS_INVALID_VALUE,
+    DDLA_STATUS_ALLOC_FAILED,
+    DDLA_STATUS_BACKEND_UNAVAILABLE,
+    DDLA_STATUS_INTERNAL_ERROR
+};
+
+class DdlaStream;
+using DdlaHandle_t = DdlaStream*;
+
+// Lifecycle.  ddlaSet builds the 1 x 1 grid and the device resources (a
+// compute stream on the current device); nprows * npcols must be 1 (values
+// <= 0 select the default 1 x 1 grid).
+ddlaStatus_t ddlaInit(DdlaHandle_t& handle, DdlaBackend requested_backend);
+ddlaStatus_t ddlaSet(DdlaHandle_t handle, int nprows = 1, int npcols = 1, char major = 'R');
+ddlaStatus_t ddlaDestroy(DdlaHandle_t& handle);
+bool ddlaBackendAvailable(DdlaBackend backend);
+DdlaBackend ddlaGetBackend(const DdlaHandle_t& handle);
+
+// Memory and ordering on the handle's queue.
+ddlaStatus_t ddlaMalloc(const DdlaHandle_t& handle, void** ptr, std::size_t bytes);
+template <typename T>
+inline ddlaStatus_t ddlaMalloc(const DdlaHandle_t& handle, T** ptr, std::size_t bytes)
+{
+    return ddlaMalloc(handle, reinterpret_cast<void**>(ptr), bytes);
+}
+ddlaStatus_t ddlaFree(const DdlaHandle_t& handle, void* ptr);
+ddlaStatus_t ddlaMemcpy(const DdlaHandle_t& handle, void* dst, const void* src, std::size_t bytes);
+ddlaStatus_t ddlaMemset(const DdlaHandle_t& handle, void* ptr, int value, std::size_t bytes);
+ddlaStatus_t ddlaSynchronize(const DdlaHandle_t& handle);
+const char* ddlaGetErrorString(ddlaStatus_t status);
+// Replace the compute stream (the handle owns a non-null replacement; null
+// selects the default stream, which is never destroyed).
+ddlaStatus_t ddlaSetStream(const DdlaHandle_t& handle, void* stream);
+void* ddlaGetStream(const DdlaHandle_t& handle);
+
+int ddlaGetRank(const DdlaHandle_t& handle);
+int ddlaGetSize(const DdlaHandle_t& handle);
+void ddlaGetGridCoords(const DdlaHandle_t& handle, int& myprow, int& mypcol);
+void ddlaGetGridDims(const DdlaHandle_t& handle, int& nprows, int& npcols);
+
+inline void DDLA_CHECK(ddlaStatus_t status, const char* file = __builtin_FILE(), int line = __builtin_LINE())
+{
+    if (status != ddlaStatus_t::DDLA_STATUS_SUCCESS)
+        throw std::runtime_error(std::string("ddla error ") + std::to_string(static_cast<int>(status)) + " (" +
+                                 ddlaGetErrorString(status) + ") at " + file + ":" + std::to_string(line));
+}
+
+// ---------------------------------------------------------------------------
+// ScaLAPACK int[9] descriptor (include/ddla/ddla_desc.h).
+// ---------------------------------------------------------------------------
+constexpr int DDLA_DTYPE_ = 0;
+constexpr int DDLA_CTXT_ = 1;
+constexpr int DDLA_M_ = 2;
+constexpr int DDLA_N_ = 3;
+constexpr int DDLA_MB_ = 4;
+constexpr int DDLA_NB_ = 5;
+constexpr int DDLA_RSRC_ = 6;
+constexpr int DDLA_CSRC_ = 7;
+constexpr int DDLA_LLD_ = 8;
+constexpr int DDLA_DLEN_ = 9;
+constexpr int DDLA_BLOCK_CYCLIC_2D = 1;
+
+// Number of rows (columns) of an n-long block-cyclic dimension owned by
+// process iproc (the ScaLAPACK NUMROC count).
+inline int num_loc(int n, int nb, int iproc, int srcproc, int nprocs)
+{
+    const int mydist = (nprocs + iproc - srcproc) % nprocs;
+    const int nblocks = n / nb;
+    int count = (nblocks / nprocs) * nb;
+    const int extra = nblocks % nprocs;
+    if (mydist < extra) count += nb;
+    else if (mydist == extra) count += n % nb;
+    return count;
+}
+
+// DESCINIT semantics: lld is raised to max(1, LOCr(m)); lld < 0 derives it.
+ddlaStatus_t ddlaDescInit(int* desc, const DdlaHandle_t& handle, int m, int n, int mb, int nb, int irsrc, int icsrc,
+                          int lld = -1);
+
+// Validation of a descriptor against the handle's grid (src/ddla_desc.cpp:9-60).
+void check_desc(const int* desc, const DdlaHandle_t& handle);
+
+template <class T> struct real_type { using type = T; };
+template <class R> struct real_type<std::complex<R>> { using type = R; };
+
+// ---------------------------------------------------------------------------
+// Cholesky path.  float and double are instantiated.
+// ---------------------------------------------------------------------------
+template <typename T>
+void ppotrf(const DdlaHandle_t& handle, char uplo, int n, T* A, int ia, int ja, const int* descA, int& info);
+
+template <typename T>
+void ppotrs(const DdlaHandle_t& handle, char side, char uplo, char trans, int n, int nrhs, const T* A,
+            const int* descA, T* B, const int* descB);
+
+template <typename T>
+void pposv(const DdlaHandle_t& handle, char side, char uplo, char trans, int n, int nrhs, T* A, int ia, int ja,
+           const int* descA, T* B, int ib, int jb, const int* descB, int& info);
+
+// Local factorization with a device-resident info word; asynchronous.
+template <typename T>
+void potrf(const DdlaHandle_t& handle, char uplo, int n, T* A, int lda, int* info);
+
+template <typename T>
+void diag_logdet(const DdlaHandle_t& h, int n, const T* A, int lda, std::complex<typename real_type<T>::type>& result);
+
+template <typename T>
+void gemmVbatched(const DdlaHandle_t& handle, char transA, char transB, int* d_m, int* d_n, int* d_k, T alpha,
+                  const T* const* d_A_array, int* d_lda, const T* const* d_B_array, int* d_ldb, T beta,
+                  T** d_C_array, int* d_ldc, int batch_count);
+
+namespace cholesky_impl {
+// Three components per routine: <r>_check (validation), <r>_workspace_size
+// (backend and host bytes; no allocation) and <r>_run on caller scratch (no
+// validation, no allocation).
+void ppotrf_check(const DdlaHandle_t& h, char uplo, int n, const void* A, int ia, int ja, const int* descA);
+template <class T>
+void ppotrf_workspace_size(const DdlaHandle_t& h, char uplo, int n, T* A, const int* descA,
+                           std::size_t* backend_bytes, std::size_t* host_bytes);
+template <class T>
+void ppotrf_run(const DdlaHandle_t& h, char uplo, int n, T* A, int ia, int ja, const int* descA, int& info,
+                void* backend_work, std::size_t backend_bytes, void* host_work, std::size_t host_bytes);
+
+void ppotrs_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                  const int* descA, const void* B, const int* descB);
+template <class T>
+void ppotrs_workspace_size(const DdlaHandle_t& h, char side, int n, int nrhs, const int* descA, const int* descB,
+                           std::size_t* backend_bytes, std::size_t* host_bytes);
+template <class T>
+void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const T* A,
+                const int* descA, T* B, const int* descB, void* backend_work, std::size_t backend_bytes,
+                void* host_work, std::size_t host_bytes);
+
+void pposv_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A, int ia,
+                 int ja, const int* descA, const void* B, int ib, int jb, const int* descB);
+template <class T>
+void pposv_workspace_size(const DdlaHandle_t& h, char side, char uplo, int n, int nrhs, T* A, const int* descA,
+                          const int* descB, std::size_t* backend_bytes, std::size_t* host_bytes);
+template <class T>
+void pposv_run(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, T* A, int ia, int ja,
+               const int* descA, T* B, int ib, int jb, const int* descB, int& info, void* backend_work,
+               std::size_t backend_bytes, void* host_work, std::size_t host_bytes);
+} // namespace cholesky_impl
+
+} // namespace ddla
+
+#endif

@@ -1,0 +1,147 @@
+/*
+ * ddla_mi355x.h -- C ABI of the MI355X-native LibDDLA Cholesky path.
+ *
+ * The entry points keep the names, argument order and status convention of
+ * the reference's generated C ABI (fortran/ddla_c_api.cpp), so a Fortran /
+ * ctypes / cgo binding written for the reference binds this library
+ * unchanged.  Plain pointers and sizes only; matrices are device pointers.
+ *
+ * Status codes (fortran/ddla_c_api.cpp:43-51): 0 success, 1 invalid handle,
+ * 2 invalid value, 3 allocation failed, 4 backend unavailable, 5 internal
+ * (device) error, 6 a C++ exception escaped -- its what() text is returned by
+ * ddla_c_error_string(6) on the same thread (fortran/ddla_c_api.cpp:55-98).
+ *
+ * One process, one GPU, a 1 x 1 grid: ddla_c_set accepts nprows = npcols = 1
+ * (or <= 0 for the default) and ignores f_comm; only backend 1 (GPU) exists.
+ */
+#ifndef DDLA_MI355X_H
+#define DDLA_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DDLA_C_STATUS_SUCCESS = 0,
+    DDLA_C_STATUS_INVALID_HANDLE = 1,
+    DDLA_C_STATUS_INVALID_VALUE = 2,
+    DDLA_C_STATUS_ALLOC_FAILED = 3,
+    DDLA_C_STATUS_BACKEND_UNAVAILABLE = 4,
+    DDLA_C_STATUS_INTERNAL_ERROR = 5,
+    DDLA_C_STATUS_EXCEPTION = 6
+};
+
+/* ---- runtime: replaces fortran/ddla_c_api.cpp:91-271 ------------------- */
+const char* ddla_c_error_string(int status);                        /* :91-98   */
+int ddla_c_init_backend(void** handle, int backend);                /* :100-113 */
+int ddla_c_set(void* handle, int f_comm, int nprows, int npcols, char major); /* :115-130 */
+int ddla_c_destroy(void** handle);                                  /* :165-176 */
+int ddla_c_backend_available(int backend);                          /* :178-182 */
+int ddla_c_get_backend(void* handle);                               /* :184-188 */
+int ddla_c_grid_dims(void* handle, int* nprows, int* npcols);       /* :190-195 */
+int ddla_c_grid_coords(void* handle, int* myprow, int* mypcol);     /* :197-202 */
+int ddla_c_rank(void* handle);                                      /* :204-208 */
+int ddla_c_size(void* handle);                                      /* :210-214 */
+int ddla_c_descinit(int* desc, void* handle, int m, int n, int mb, int nb,
+                    int irsrc, int icsrc, int lld);                 /* :216-222 */
+int ddla_c_numloc(int n, int nb, int iproc, int srcproc, int nprocs); /* :224-227 */
+int ddla_c_malloc(void* handle, void** ptr, long long bytes);       /* :229-234 */
+int ddla_c_free(void* handle, void* ptr);                           /* :236-240 */
+int ddla_c_memcpy(void* handle, void* dst, const void* src, long long bytes); /* :242-247 */
+int ddla_c_memset(void* handle, void* ptr, int value, long long bytes); /* :260-265 */
+int ddla_c_synchronize(void* handle);                               /* :267-271 */
+/* ddlaSetStream / ddlaGetStream (include/ddla/ddla_handle_t.h:300, :337);
+ * no C entry in the reference. */
+int ddla_c_set_stream(void* handle, void* stream);
+void* ddla_c_get_stream(void* handle);
+
+/* ---- Cholesky drivers: replace fortran/ddla_c_api.cpp:1155-1335 ---------- */
+int ddla_s_ppotrf(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info); /* :1155 */
+int ddla_d_ppotrf(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info); /* :1164 */
+int ddla_s_ppotrs(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                  const int* descA, void* B, const int* descB);                                      /* :1263 */
+int ddla_d_ppotrs(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                  const int* descA, void* B, const int* descB);                                      /* :1272 */
+int ddla_s_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                 const int* descA, void* B, int ib, int jb, const int* descB, int* info);            /* :1299 */
+int ddla_d_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                 const int* descA, void* B, int ib, int jb, const int* descB, int* info);            /* :1308 */
+
+/* ---- three-phase API (no C entry in the reference): the C++ components
+ * cholesky_impl::{ppotrf,ppotrs,pposv}_{workspace_size,run} of
+ * src/Cholesky/cholesky_workspace.h:133-249.  The size query allocates
+ * nothing; a run validates nothing beyond the workspace size and allocates
+ * nothing (ppotrf/pposv run synchronize to read info; ppotrs run is
+ * asynchronous). -------------------------------------------------------- */
+int ddla_s_ppotrf_workspace_size(void* handle, char uplo, int n, const int* descA, long long* backend_bytes,
+                                 long long* host_bytes);
+int ddla_d_ppotrf_workspace_size(void* handle, char uplo, int n, const int* descA, long long* backend_bytes,
+                                 long long* host_bytes);
+int ddla_s_ppotrf_run(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info,
+                      void* backend_work, long long backend_bytes, void* host_work, long long host_bytes);
+int ddla_d_ppotrf_run(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info,
+                      void* backend_work, long long backend_bytes, void* host_work, long long host_bytes);
+int ddla_s_ppotrs_workspace_size(void* handle, char side, int n, int nrhs, const int* descA, const int* descB,
+                                 long long* backend_bytes, long long* host_bytes);
+int ddla_d_ppotrs_workspace_size(void* handle, char side, int n, int nrhs, const int* descA, const int* descB,
+                                 long long* backend_bytes, long long* host_bytes);
+int ddla_s_ppotrs_run(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                      const int* descA, void* B, const int* descB, void* backend_work, long long backend_bytes,
+                      void* host_work, long long host_bytes);
+int ddla_d_ppotrs_run(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                      const int* descA, void* B, const int* descB, void* backend_work, long long backend_bytes,
+                      void* host_work, long long host_bytes);
+int ddla_s_pposv_workspace_size(void* handle, char side, char uplo, int n, int nrhs, const int* descA,
+                                const int* descB, long long* backend_bytes, long long* host_bytes);
+int ddla_d_pposv_workspace_size(void* handle, char side, char uplo, int n, int nrhs, const int* descA,
+                                const int* descB, long long* backend_bytes, long long* host_bytes);
+int ddla_s_pposv_run(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                     const int* descA, void* B, int ib, int jb, const int* descB, int* info, void* backend_work,
+                     long long backend_bytes, void* host_work, long long host_bytes);
+int ddla_d_pposv_run(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                     const int* descA, void* B, int ib, int jb, const int* descB, int* info, void* backend_work,
+                     long long backend_bytes, void* host_work, long long host_bytes);
+
+/* ---- local potrf, device info (C++ ddla::potrf, include/ddla/ddla.h:2160) */
+int ddla_s_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
+int ddla_d_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
+
+/* ---- diag_logdet (C++ ddla::diag_logdet, include/ddla/ddla.h:2901; no C
+ * entry in the reference): result[0] = real part, result[1] = phase. ----- */
+int ddla_s_diag_logdet(void* handle, int n, const void* A, int lda, float* result);
+int ddla_d_diag_logdet(void* handle, int n, const void* A, int lda, double* result);
+
+/* ---- gemmVbatched (C++ ddla::gemmVbatched, include/ddla/ddla.h:1836; the
+ * reference binds no C entry, README.md:967-970).  All arrays are device
+ * resident; alpha / beta are host scalars passed by reference. ---------- */
+int ddla_s_gemm_vbatched(void* handle, char transA, char transB, int* d_m, int* d_n, int* d_k, const float* alpha,
+                         const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb, const float* beta,
+                         void** d_C, int* d_ldc, int batch_count);
+int ddla_d_gemm_vbatched(void* handle, char transA, char transB, int* d_m, int* d_n, int* d_k, const double* alpha,
+                         const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb, const double* beta,
+                         void** d_C, int* d_ldc, int batch_count);
+
+/* ---- harness helpers (no counterpart in the reference) ------------------
+ * Philox4x64-10 + Box-Muller normals of the project's seed convention
+ * (element e of stream s: counter {e/4, s, 0, 0}, key {seed, 0}), and the
+ * synthetic SPD matrix A = G G^T / n + delta I built from them (G from
+ * stream 0; `work` must hold n*n doubles).  Test / benchmark setup only. */
+int ddla_c_fill_normal(void* handle, unsigned long long seed, unsigned long long stream,
+                       unsigned long long first, long long count, double* d_out);
+int ddla_d_spd_generate(void* handle, int n, unsigned long long seed, double delta, double* A, int lda,
+                        double* work);
+
+/* ---- per-launch event profiler (no counterpart in the reference) --------
+ * enable = 1 starts recording an event pair around every kernel launch of
+ * the handle (and clears earlier records); ddla_c_profile_read synchronizes
+ * and returns, for kernel class cls, the number of launches, their summed
+ * duration in ms and their summed algorithmic flops and bytes.  Classes: 0
+ * trailing update, 1 panel trsm, 2 panel update, 3 potf2, 4 solve update, 5
+ * solve diagonal, 6 diagonal-block inverse, 7 other. */
+int ddla_c_profile(void* handle, int enable);
+int ddla_c_profile_read(void* handle, int cls, int* count, double* ms, double* flops, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

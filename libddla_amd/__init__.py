@@ -1,0 +1,39 @@
+"""libddla_amd -- MI355X-native LibDDLA Cholesky path.
+
+Host side of the drop-in: the reference's Python surface (Handle,
+DistributedMatrix, ppotrf / ppotrs / pposv, diag_logdet) over the C ABI of
+libddla_mi355x.so, whose kernels are hand-written HIP for gfx950.
+"""
+from ._capi import (
+    LIBRARY_PATH,
+    DdlaAllocFailedError,
+    DdlaBackendUnavailableError,
+    DdlaError,
+    DdlaExceptionError,
+    DdlaInternalError,
+    DdlaInvalidHandleError,
+    DdlaInvalidValueError,
+    check_status,
+    lib,
+)
+from .backend import Backend, Handle
+from .matrix import DistributedMatrix, descinit
+from .ops import (
+    diag_logdet,
+    ppotrf,
+    ppotrf_run,
+    ppotrf_workspace_size,
+    pposv,
+    pposv_run,
+    pposv_workspace_size,
+    ppotrs,
+    ppotrs_workspace_size,
+)
+
+__all__ = [
+    "LIBRARY_PATH", "lib", "check_status", "Backend", "Handle", "DistributedMatrix", "descinit",
+    "ppotrf", "ppotrs", "pposv", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
+    "pposv_workspace_size", "pposv_run", "diag_logdet", "DdlaError", "DdlaInvalidHandleError",
+    "DdlaInvalidValueError", "DdlaAllocFailedError", "DdlaBackendUnavailableError", "DdlaInternalError",
+    "DdlaExceptionError",
+]

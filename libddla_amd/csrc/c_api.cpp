@@ -1,0 +1,386 @@
+// C ABI over the C++ API (declarations and reference citations in
+// include/ddla_mi355x.h).  Same conventions as the reference's generated
+// shim, fortran/ddla_c_api.cpp:43-98: integer statuses, a C++ exception
+// becomes status 6 with its text kept per thread, the handle travels as void*.
+#include "internal.h"
+#include "kernels.h"
+
+#include <ddla_mi355x.h>
+
+#include <exception>
+#include <string>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void record_error(const char* what)
+{
+    try {
+        g_last_error = what ? what : "";
+    } catch (...) {
+        g_last_error.clear();
+    }
+}
+
+template <typename F>
+int guarded(F&& body)
+{
+    try {
+        return body();
+    } catch (const std::exception& e) {
+        record_error(e.what());
+        return DDLA_C_STATUS_EXCEPTION;
+    } catch (...) {
+        record_error("unknown C++ exception in a LibDDLA C API call");
+        return DDLA_C_STATUS_EXCEPTION;
+    }
+}
+
+// A computational call: a null handle is status 1; an unavailable backend 4.
+template <typename F>
+int run(void* handle, F&& body)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    return guarded([&]() -> int {
+        const ddla::DdlaHandle_t h = static_cast<ddla::DdlaHandle_t>(handle);
+        if (!ddla::ddlaBackendAvailable(ddla::ddlaGetBackend(h))) return DDLA_C_STATUS_BACKEND_UNAVAILABLE;
+        return body(h);
+    });
+}
+
+inline ddla::DdlaHandle_t H(void* p) { return static_cast<ddla::DdlaHandle_t>(p); }
+inline int S(ddla::ddlaStatus_t s) { return static_cast<int>(s); }
+
+} // namespace
+
+extern "C" {
+
+const char* ddla_c_error_string(int status)
+{
+    if (status == DDLA_C_STATUS_EXCEPTION && !g_last_error.empty()) return g_last_error.c_str();
+    if (status >= 0 && status <= 5) return ddla::ddlaGetErrorString(static_cast<ddla::ddlaStatus_t>(status));
+    return "unknown LibDDLA C API status";
+}
+
+int ddla_c_init_backend(void** handle, int backend)
+{
+    return guarded([&]() -> int {
+        if (handle == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+        if (backend < 0 || backend > 1) return DDLA_C_STATUS_INVALID_VALUE;
+        const ddla::DdlaBackend be = backend == 0 ? ddla::DdlaBackend::CPU : ddla::DdlaBackend::GPU;
+        if (!ddla::ddlaBackendAvailable(be)) return DDLA_C_STATUS_BACKEND_UNAVAILABLE;
+        ddla::DdlaHandle_t h = nullptr;
+        const ddla::ddlaStatus_t st = ddla::ddlaInit(h, be);
+        if (st != ddla::ddlaStatus_t::DDLA_STATUS_SUCCESS) return S(st);
+        *handle = h;
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_set(void* handle, int /*f_comm*/, int nprows, int npcols, char major)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    return guarded([&]() -> int { return S(ddla::ddlaSet(H(handle), nprows, npcols, major)); });
+}
+
+int ddla_c_destroy(void** handle)
+{
+    return guarded([&]() -> int {
+        if (handle == nullptr || *handle == nullptr) return DDLA_C_STATUS_SUCCESS;
+        ddla::DdlaHandle_t h = H(*handle);
+        const ddla::ddlaStatus_t st = ddla::ddlaDestroy(h);
+        *handle = h;
+        return S(st);
+    });
+}
+
+int ddla_c_backend_available(int backend)
+{
+    if (backend < 0 || backend > 1) return 0;
+    return ddla::ddlaBackendAvailable(backend == 0 ? ddla::DdlaBackend::CPU : ddla::DdlaBackend::GPU) ? 1 : 0;
+}
+
+int ddla_c_get_backend(void* handle)
+{
+    if (handle == nullptr) return -DDLA_C_STATUS_INVALID_HANDLE;
+    return static_cast<int>(ddla::ddlaGetBackend(H(handle)));
+}
+
+int ddla_c_grid_dims(void* handle, int* nprows, int* npcols)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (nprows == nullptr || npcols == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+    ddla::ddlaGetGridDims(H(handle), *nprows, *npcols);
+    return DDLA_C_STATUS_SUCCESS;
+}
+
+int ddla_c_grid_coords(void* handle, int* myprow, int* mypcol)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (myprow == nullptr || mypcol == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+    ddla::ddlaGetGridCoords(H(handle), *myprow, *mypcol);
+    return DDLA_C_STATUS_SUCCESS;
+}
+
+int ddla_c_rank(void* handle) { return handle == nullptr ? -1 : ddla::ddlaGetRank(H(handle)); }
+int ddla_c_size(void* handle) { return handle == nullptr ? -1 : ddla::ddlaGetSize(H(handle)); }
+
+int ddla_c_descinit(int* desc, void* handle, int m, int n, int mb, int nb, int irsrc, int icsrc, int lld)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    return S(ddla::ddlaDescInit(desc, H(handle), m, n, mb, nb, irsrc, icsrc, lld));
+}
+
+int ddla_c_numloc(int n, int nb, int iproc, int srcproc, int nprocs)
+{
+    return ddla::num_loc(n, nb, iproc, srcproc, nprocs);
+}
+
+int ddla_c_malloc(void* handle, void** ptr, long long bytes)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (bytes < 0) return DDLA_C_STATUS_INVALID_VALUE;
+    return S(ddla::ddlaMalloc(H(handle), ptr, static_cast<std::size_t>(bytes)));
+}
+
+int ddla_c_free(void* handle, void* ptr)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    return S(ddla::ddlaFree(H(handle), ptr));
+}
+
+int ddla_c_memcpy(void* handle, void* dst, const void* src, long long bytes)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (bytes < 0) return DDLA_C_STATUS_INVALID_VALUE;
+    return S(ddla::ddlaMemcpy(H(handle), dst, src, static_cast<std::size_t>(bytes)));
+}
+
+int ddla_c_memset(void* handle, void* ptr, int value, long long bytes)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (bytes < 0) return DDLA_C_STATUS_INVALID_VALUE;
+    return S(ddla::ddlaMemset(H(handle), ptr, value, static_cast<std::size_t>(bytes)));
+}
+
+int ddla_c_synchronize(void* handle)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    return S(ddla::ddlaSynchronize(H(handle)));
+}
+
+int ddla_c_set_stream(void* handle, void* stream)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    return guarded([&]() -> int { return S(ddla::ddlaSetStream(H(handle), stream)); });
+}
+
+void* ddla_c_get_stream(void* handle)
+{
+    if (handle == nullptr) return nullptr;
+    return ddla::ddlaGetStream(H(handle));
+}
+
+// ---- drivers ---------------------------------------------------------------
+#define DDLA_C_DRIVERS(P, T)                                                                                     \
+    int ddla_##P##_ppotrf(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info) \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (info == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                           \
+            ddla::ppotrf<T>(h, uplo, n, static_cast<T*>(A), ia, ja, descA, *info);                             \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_ppotrs(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,       \
+                          const int* descA, void* B, const int* descB)                                         \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::ppotrs<T>(h, side, uplo, trans, n, nrhs, static_cast<const T*>(A), descA, static_cast<T*>(B), \
+                            descB);                                                                            \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia,      \
+                         int ja, const int* descA, void* B, int ib, int jb, const int* descB, int* info)       \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (info == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                           \
+            ddla::pposv<T>(h, side, uplo, trans, n, nrhs, static_cast<T*>(A), ia, ja, descA,                   \
+                           static_cast<T*>(B), ib, jb, descB, *info);                                          \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_ppotrf_workspace_size(void* handle, char uplo, int n, const int* descA,                     \
+                                         long long* backend_bytes, long long* host_bytes)                      \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (!backend_bytes || !host_bytes) return DDLA_C_STATUS_INVALID_VALUE;                             \
+            std::size_t b = 0, hb = 0;                                                                         \
+            ddla::cholesky_impl::ppotrf_workspace_size<T>(h, uplo, n, static_cast<T*>(nullptr), descA, &b,     \
+                                                          &hb);                                                \
+            *backend_bytes = (long long)b;                                                                     \
+            *host_bytes = (long long)hb;                                                                       \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_ppotrf_run(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA,       \
+                              int* info, void* bw, long long bb, void* hw, long long hb)                       \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (info == nullptr || bb < 0 || hb < 0) return DDLA_C_STATUS_INVALID_VALUE;                       \
+            ddla::cholesky_impl::ppotrf_run<T>(h, uplo, n, static_cast<T*>(A), ia, ja, descA, *info, bw,       \
+                                               (std::size_t)bb, hw, (std::size_t)hb);                          \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_ppotrs_workspace_size(void* handle, char side, int n, int nrhs, const int* descA,           \
+                                         const int* descB, long long* backend_bytes, long long* host_bytes)    \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (!backend_bytes || !host_bytes) return DDLA_C_STATUS_INVALID_VALUE;                             \
+            std::size_t b = 0, hb = 0;                                                                         \
+            ddla::cholesky_impl::ppotrs_workspace_size<T>(h, side, n, nrhs, descA, descB, &b, &hb);            \
+            *backend_bytes = (long long)b;                                                                     \
+            *host_bytes = (long long)hb;                                                                       \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_ppotrs_run(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,  \
+                              const int* descA, void* B, const int* descB, void* bw, long long bb, void* hw,   \
+                              long long hb)                                                                    \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (bb < 0 || hb < 0) return DDLA_C_STATUS_INVALID_VALUE;                                          \
+            ddla::cholesky_impl::ppotrs_run<T>(h, side, uplo, trans, n, nrhs, static_cast<const T*>(A), descA, \
+                                               static_cast<T*>(B), descB, bw, (std::size_t)bb, hw,             \
+                                               (std::size_t)hb);                                               \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pposv_workspace_size(void* handle, char side, char uplo, int n, int nrhs, const int* descA, \
+                                        const int* descB, long long* backend_bytes, long long* host_bytes)     \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (!backend_bytes || !host_bytes) return DDLA_C_STATUS_INVALID_VALUE;                             \
+            std::size_t b = 0, hb = 0;                                                                         \
+            ddla::cholesky_impl::pposv_workspace_size<T>(h, side, uplo, n, nrhs, static_cast<T*>(nullptr),     \
+                                                         descA, descB, &b, &hb);                               \
+            *backend_bytes = (long long)b;                                                                     \
+            *host_bytes = (long long)hb;                                                                       \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pposv_run(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, \
+                             int ja, const int* descA, void* B, int ib, int jb, const int* descB, int* info,   \
+                             void* bw, long long bb, void* hw, long long hb)                                   \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (info == nullptr || bb < 0 || hb < 0) return DDLA_C_STATUS_INVALID_VALUE;                       \
+            ddla::cholesky_impl::pposv_run<T>(h, side, uplo, trans, n, nrhs, static_cast<T*>(A), ia, ja,       \
+                                              descA, static_cast<T*>(B), ib, jb, descB, *info, bw,             \
+                                              (std::size_t)bb, hw, (std::size_t)hb);                           \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info)                        \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::potrf<T>(h, uplo, n, static_cast<T*>(A), lda, d_info);                                       \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_diag_logdet(void* handle, int n, const void* A, int lda, T* result)                         \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (result == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                         \
+            std::complex<T> z;                                                                                 \
+            ddla::diag_logdet<T>(h, n, static_cast<const T*>(A), lda, z);                                      \
+            result[0] = z.real();                                                                              \
+            result[1] = z.imag();                                                                              \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_gemm_vbatched(void* handle, char ta, char tb, int* d_m, int* d_n, int* d_k, const T* alpha, \
+                                 const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb,       \
+                                 const T* beta, void** d_C, int* d_ldc, int batch_count)                       \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (alpha == nullptr || beta == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                       \
+            ddla::gemmVbatched<T>(h, ta, tb, d_m, d_n, d_k, *alpha, reinterpret_cast<const T* const*>(d_A),    \
+                                  d_lda, reinterpret_cast<const T* const*>(d_B), d_ldb, *beta,                 \
+                                  reinterpret_cast<T**>(d_C), d_ldc, batch_count);                             \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }
+
+DDLA_C_DRIVERS(s, float)
+DDLA_C_DRIVERS(d, double)
+#undef DDLA_C_DRIVERS
+
+// ---- harness helpers ---------------------------------------------------------
+int ddla_c_fill_normal(void* handle, unsigned long long seed, unsigned long long stream, unsigned long long first,
+                       long long count, double* d_out)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        ddla::require_gpu(h, "fill_normal");
+        if (count < 0 || (count > 0 && d_out == nullptr)) return DDLA_C_STATUS_INVALID_VALUE;
+        ddla::kern::fill_normal(h->stream, seed, stream, first, count, d_out);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_d_spd_generate(void* handle, int n, unsigned long long seed, double delta, double* A, int lda, double* work)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        ddla::require_gpu(h, "spd_generate");
+        if (n < 0 || lda < std::max(1, n) || (n > 0 && (A == nullptr || work == nullptr)))
+            return DDLA_C_STATUS_INVALID_VALUE;
+        if (n == 0) return DDLA_C_STATUS_SUCCESS;
+        ddla::kern::fill_normal(h->stream, seed, 0, 0, (long long)n * n, work);
+        ddla::kern::gemm<double>(h->stream, 'N', 'T', n, n, n, 1.0 / n, work, n, work, n, 0.0, A, lda);
+        ddla::kern::add_diagonal<double>(h->stream, n, A, lda, delta);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+// ---- profiler ------------------------------------------------------------------
+int ddla_c_profile(void* handle, int enable)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    return guarded([&]() -> int {
+        ddla::DdlaHandle_t h = H(handle);
+        if (h->stream != nullptr || h->initialized) DDLA_HIP(hipStreamSynchronize(h->stream));
+        h->release_events();
+        h->profiling = enable != 0;
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_profile_read(void* handle, int cls, int* count, double* ms, double* flops, double* bytes)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (count == nullptr || ms == nullptr || flops == nullptr || bytes == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+    return guarded([&]() -> int {
+        ddla::DdlaHandle_t h = H(handle);
+        DDLA_HIP(hipStreamSynchronize(h->stream));
+        int c = 0;
+        double t = 0, f = 0, b = 0;
+        for (const auto& r : h->records) {
+            if (r.cls != cls) continue;
+            float e = 0;
+            DDLA_HIP(hipEventElapsedTime(&e, r.start, r.stop));
+            ++c;
+            t += e;
+            f += r.flops;
+            b += r.bytes;
+        }
+        *count = c;
+        *ms = t;
+        *flops = f;
+        *bytes = b;
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,467 @@
+// Cholesky path on one GPU: ppotrf / ppotrs / pposv (check / workspace size /
+// run and the public drivers), local potrf, diag_logdet and gemmVbatched.
+//
+// Reference behaviour followed (1 x 1 grid):
+//   ppotrf   src/Cholesky/ppotrf.cpp:21-54, :326-349
+//   ppotrs   src/Cholesky/ppotrs.cpp:14-85 (trans mapping :44-46)
+//   pposv    src/Cholesky/pposv.cpp:15-63 (solve only if info == 0)
+//   sizes    src/Cholesky/cholesky_workspace.h:133-154, :207-249
+//   potrf    src/Cholesky/potrf.cpp:123-193
+//   logdet   src/diag_logdet.cpp:42-55, src/diag_logdet_impl.h:22-113
+//   vbatched src/BLAS/gemm_vbatched.cpp:74-160, :350-412
+// The arithmetic is this project's: the reference calls a vendor ?potrf /
+// ?trsm here; this file schedules the hand-written kernels of kernels.hip in
+// the blocked right-looking order of src/Cholesky/ppotrf.cpp:108-322.
+#include "internal.h"
+#include "kernels.h"
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+
+namespace ddla {
+
+namespace {
+
+using kern::kTriLower;
+using kern::kTriNone;
+using kern::kTriUpper;
+
+constexpr int kIB = 64;  // diagonal block of potf2 and of the solves
+
+// Outer panel width of the factorization: the depth of the trailing-update
+// GEMM.  Not the descriptor's NB -- at 1 x 1 the block size of the
+// distribution does not change the arithmetic (the reference hands the whole
+// matrix to one ?potrf call).
+template <class T> constexpr int outer_block() { return 512; }
+
+// The factor is addressed through an "L view": element (i, j), i >= j, of the
+// lower factor is A[i + j*lda] for uplo 'L' and A[j + i*lda] for uplo 'U'
+// (U = L^T).  The helpers below translate L-view operations into GEMM calls
+// on the stored layout.
+template <class T>
+struct LView {
+    T* A;
+    long long lda;
+    bool upper;
+    T* at(long long r, long long c) const { return A + r + c * lda; }  // stored (r, c)
+};
+
+// C_L(ci.., cj..) (M x N) := alpha * A_L(ai.., ak..) * B_L(bj.., bk..)^T + beta * C_L,
+// writing only the lower triangle of the global L view when mask is set.
+template <class T>
+void update_lower(DdlaStream* h, int cls, const LView<T>& v, int M, int N, int K, T alpha, long long ai,
+                  long long ak, long long bj, long long bk, T beta, long long ci, long long cj, bool mask,
+                  const int* abort, double flops)
+{
+    if (M <= 0 || N <= 0) return;
+    LaunchTimer timer(h, cls, flops);
+    if (!v.upper) {
+        kern::gemm<T>(h->stream, 'N', 'T', M, N, K, alpha, v.at(ai, ak), v.lda, v.at(bj, bk), v.lda, beta,
+                      v.at(ci, cj), v.lda, mask ? kTriLower : kTriNone, ci - cj, abort);
+    } else {
+        // Stored block at (cj, ci) is C_L^T = B_L * A_L^T.
+        kern::gemm<T>(h->stream, 'T', 'N', N, M, K, alpha, v.at(bk, bj), v.lda, v.at(ak, ai), v.lda, beta,
+                      v.at(cj, ci), v.lda, mask ? kTriUpper : kTriNone, cj - ci, abort);
+    }
+}
+
+// X_L(ri.., ci..) (M x w) := X_L * inv^T, inv the 64 x 64 inverse of the
+// diagonal block (in place: one workgroup covers the w columns).
+template <class T>
+void panel_trsm(DdlaStream* h, const LView<T>& v, int M, int w, long long ri, long long ci, const T* inv,
+                const int* abort)
+{
+    if (M <= 0) return;
+    LaunchTimer timer(h, kClassPanelTrsm, 2.0 * M * w * w);
+    if (!v.upper)
+        kern::gemm<T>(h->stream, 'N', 'T', M, w, w, T(1), v.at(ri, ci), v.lda, inv, kIB, T(0), v.at(ri, ci), v.lda,
+                      kTriNone, 0, abort);
+    else
+        kern::gemm<T>(h->stream, 'N', 'N', w, M, w, T(1), inv, kIB, v.at(ci, ri), v.lda, T(0), v.at(ci, ri), v.lda,
+                      kTriNone, 0, abort);
+}
+
+// Blocked right-looking Cholesky of the order-n L view; *info (device) must
+// be zero on entry and receives the 1-based index of the first failed pivot.
+template <class T>
+void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, T* inv)
+{
+    const LView<T> v{A, lda, upper};
+    const int NB = outer_block<T>();
+    for (int k = 0; k < n; k += NB) {
+        const int kb = std::min(NB, n - k);
+        for (int c = 0; c < kb; c += kIB) {
+            const int w = std::min(kIB, kb - c);
+            const long long r0 = (long long)k + c;
+            {
+                LaunchTimer timer(h, kClassPotf2, (double)w * w * w / 3.0);
+                kern::potf2_inv<T>(h->stream, upper, w, v.at(r0, r0), lda, inv, info, (int)r0);
+            }
+            const int below = n - (int)r0 - w;
+            if (below <= 0) continue;
+            panel_trsm<T>(h, v, below, w, r0 + w, r0, inv, info);
+            const int ncols = k + kb - (int)r0 - w;
+            if (ncols > 0)
+                update_lower<T>(h, kClassPanelUpdate, v, below, ncols, w, T(-1), r0 + w, r0, r0 + w, r0, T(1),
+                                r0 + w, r0 + w, true, info, 2.0 * below * ncols * w);
+        }
+        const int m = n - k - kb;
+        if (m > 0)
+            update_lower<T>(h, kClassTrailingUpdate, v, m, m, kb, T(-1), k + kb, k, k + kb, k, T(1), k + kb, k + kb,
+                            true, info, (double)m * (m + 1) * kb);
+    }
+}
+
+// Left solve A X = B with A = L L^T given by its L view; B is n x nrhs.
+template <class T>
+void solve_left(DdlaStream* h, bool upper, int n, int nrhs, const T* F, long long ldf, T* B, long long ldb,
+                const T* dinv)
+{
+    const LView<T> v{const_cast<T*>(F), ldf, upper};
+    const int nblk = (n + kIB - 1) / kIB;
+    // L Y = B
+    for (int jb = 0; jb < nblk; ++jb) {
+        const long long j0 = (long long)jb * kIB;
+        const int w = (int)std::min<long long>(kIB, n - j0);
+        const T* d = dinv + (long long)jb * kIB * kIB;
+        {
+            LaunchTimer timer(h, kClassSolveDiag, 2.0 * w * w * nrhs);
+            kern::gemm<T>(h->stream, 'N', 'N', w, nrhs, w, T(1), d, kIB, B + j0, ldb, T(0), B + j0, ldb);
+        }
+        const int below = n - (int)j0 - w;
+        if (below > 0) {
+            LaunchTimer timer(h, kClassSolveUpdate, 2.0 * below * w * nrhs, 8.0 * below * w);
+            if (!upper)
+                kern::gemm<T>(h->stream, 'N', 'N', below, nrhs, w, T(-1), v.at(j0 + w, j0), ldf, B + j0, ldb, T(1),
+                              B + j0 + w, ldb);
+            else
+                kern::gemm<T>(h->stream, 'T', 'N', below, nrhs, w, T(-1), v.at(j0, j0 + w), ldf, B + j0, ldb, T(1),
+                              B + j0 + w, ldb);
+        }
+    }
+    // L^T X = Y
+    for (int jb = nblk - 1; jb >= 0; --jb) {
+        const long long j0 = (long long)jb * kIB;
+        const int w = (int)std::min<long long>(kIB, n - j0);
+        const T* d = dinv + (long long)jb * kIB * kIB;
+        {
+            LaunchTimer timer(h, kClassSolveDiag, 2.0 * w * w * nrhs);
+            kern::gemm<T>(h->stream, 'T', 'N', w, nrhs, w, T(1), d, kIB, B + j0, ldb, T(0), B + j0, ldb);
+        }
+        if (j0 > 0) {
+            LaunchTimer timer(h, kClassSolveUpdate, 2.0 * j0 * w * nrhs, 8.0 * j0 * w);
+            if (!upper)
+                kern::gemm<T>(h->stream, 'T', 'N', (int)j0, nrhs, w, T(-1), v.at(j0, 0), ldf, B + j0, ldb, T(1), B,
+                              ldb);
+            else
+                kern::gemm<T>(h->stream, 'N', 'N', (int)j0, nrhs, w, T(-1), v.at(0, j0), ldf, B + j0, ldb, T(1), B,
+                              ldb);
+        }
+    }
+}
+
+std::size_t diag_inverse_elems(int n) { return (std::size_t)((n + kIB - 1) / kIB) * kIB * kIB; }
+
+void bad_workspace(const char* who)
+{
+    throw std::invalid_argument(std::string(who) + ": backend workspace smaller than the workspace query");
+}
+
+char upper_char(char c) { return (c >= 'a' && c <= 'z') ? char(c - 'a' + 'A') : c; }
+
+} // namespace
+
+namespace cholesky_impl {
+
+void ppotrf_check(const DdlaHandle_t& h, char uplo, int n, const void* A, int ia, int ja, const int* d)
+{
+    check_desc(d, h);
+    require_gpu(h, "ppotrf");
+    if ((uplo != 'L' && uplo != 'U') || n < 0 || n > d[DDLA_M_] || n > d[DDLA_N_] || d[DDLA_MB_] != d[DDLA_NB_] ||
+        ia != 1 || ja != 1)
+        throw std::invalid_argument("ppotrf: invalid triangle, dimensions, blocks or offsets");
+    if (n && !A) throw std::invalid_argument("ppotrf: null local matrix");
+}
+
+template <class T>
+void ppotrf_workspace_size(const DdlaHandle_t&, char, int n, T*, const int*, std::size_t* backend_bytes,
+                           std::size_t* host_bytes)
+{
+    *backend_bytes = *host_bytes = 0;
+    if (n <= 0) return;
+    // [ info word | inverse of the current 64 x 64 diagonal block ]
+    *backend_bytes = detail::padded_bytes<int>(1) + detail::padded_bytes<T>((std::size_t)kIB * kIB);
+}
+
+template <class T>
+void ppotrf_run(const DdlaHandle_t& h, char uplo, int n, T* A, int, int, const int* d, int& info, void* backend_work,
+                std::size_t backend_bytes, void*, std::size_t)
+{
+    if (n == 0) {
+        info = 0;
+        return;
+    }
+    std::size_t need = 0, host = 0;
+    ppotrf_workspace_size<T>(h, uplo, n, A, d, &need, &host);
+    if (backend_bytes < need || backend_work == nullptr) bad_workspace("ppotrf");
+    char* w = static_cast<char*>(backend_work);
+    int* d_info = detail::take<int>(w, 1);
+    T* inv = detail::take<T>(w, (std::size_t)kIB * kIB);
+    fill_int(h->stream, d_info, 0);
+    potrf_blocked<T>(h, uplo == 'U', n, A, d[DDLA_LLD_], d_info, inv);
+    DDLA_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    DDLA_HIP(hipStreamSynchronize(h->stream));
+}
+
+void ppotrs_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                  const int* da, const void* B, const int* db)
+{
+    check_desc(da, h);
+    check_desc(db, h);
+    require_gpu(h, "ppotrs");
+    if ((side != 'L' && side != 'R') || (uplo != 'L' && uplo != 'U') || (trans != 'N' && trans != 'C') || n < 0 ||
+        nrhs < 0 || n > da[DDLA_M_] || n > da[DDLA_N_] || da[DDLA_MB_] != da[DDLA_NB_])
+        throw std::invalid_argument("ppotrs: invalid flags, dimensions or factor blocks");
+    if (db[DDLA_MB_] != da[DDLA_MB_] || (side == 'L' && db[DDLA_RSRC_] != da[DDLA_RSRC_]) ||
+        (side == 'R' && (db[DDLA_NB_] != da[DDLA_NB_] || db[DDLA_CSRC_] != da[DDLA_CSRC_])))
+        throw std::invalid_argument("ppotrs: B must be distributed like A along the solved dimension");
+    const int rows = side == 'L' ? n : nrhs, cols = side == 'L' ? nrhs : n;
+    if (rows > db[DDLA_M_] || cols > db[DDLA_N_]) throw std::invalid_argument("ppotrs: invalid right-hand side");
+    if (n && !A) throw std::invalid_argument("ppotrs: null local factor");
+    if (rows && cols && !B) throw std::invalid_argument("ppotrs: null local right-hand side");
+}
+
+template <class T>
+void ppotrs_workspace_size(const DdlaHandle_t&, char side, int n, int nrhs, const int*, const int*,
+                           std::size_t* backend_bytes, std::size_t* host_bytes)
+{
+    *backend_bytes = *host_bytes = 0;
+    if (n == 0 || nrhs == 0) return;
+    // [ inverses of the 64 x 64 diagonal blocks | transposed B (side 'R') ]
+    *backend_bytes = detail::padded_bytes<T>(diag_inverse_elems(n));
+    if (side == 'R') *backend_bytes += detail::padded_bytes<T>((std::size_t)n * nrhs);
+}
+
+template <class T>
+void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char, int n, int nrhs, const T* A, const int* da, T* B,
+                const int* db, void* backend_work, std::size_t backend_bytes, void*, std::size_t)
+{
+    if (n == 0 || nrhs == 0) return;
+    std::size_t need = 0, host = 0;
+    ppotrs_workspace_size<T>(h, side, n, nrhs, da, db, &need, &host);
+    if (backend_bytes < need || backend_work == nullptr) bad_workspace("ppotrs");
+    const bool upper = uplo == 'U';
+    const long long lda = da[DDLA_LLD_], ldb = db[DDLA_LLD_];
+    char* w = static_cast<char*>(backend_work);
+    T* dinv = detail::take<T>(w, diag_inverse_elems(n));
+    {
+        LaunchTimer timer(h, kClassTrtri, (double)n * kIB * kIB / 3.0);
+        kern::trtri_diag_blocks<T>(h->stream, upper, n, A, lda, dinv);
+    }
+    // The factor is symmetric-positive-definite's: trans 'N' and 'C' solve the
+    // same system.  side 'R' (X A = B, B nrhs x n) is A X^T = B^T.
+    if (side == 'L') {
+        solve_left<T>(h, upper, n, nrhs, A, lda, B, ldb, dinv);
+    } else {
+        T* bt = detail::take<T>(w, (std::size_t)n * nrhs);
+        kern::transpose<T>(h->stream, nrhs, n, B, ldb, bt, n);
+        solve_left<T>(h, upper, n, nrhs, A, lda, bt, n, dinv);
+        kern::transpose<T>(h->stream, n, nrhs, bt, n, B, ldb);
+    }
+}
+
+void pposv_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A, int ia,
+                 int ja, const int* da, const void* B, int ib, int jb, const int* db)
+{
+    ppotrf_check(h, uplo, n, A, ia, ja, da);
+    ppotrs_check(h, side, uplo, trans, n, nrhs, A, da, B, db);
+    if (ib != 1 || jb != 1) throw std::invalid_argument("pposv: offsets must be one");
+}
+
+template <class T>
+void pposv_workspace_size(const DdlaHandle_t& h, char side, char uplo, int n, int nrhs, T* A, const int* da,
+                          const int* db, std::size_t* backend_bytes, std::size_t* host_bytes)
+{
+    std::size_t sb = 0, sh = 0;
+    ppotrf_workspace_size<T>(h, uplo, n, A, da, backend_bytes, host_bytes);
+    ppotrs_workspace_size<T>(h, side, n, nrhs, da, db, &sb, &sh);
+    *backend_bytes = std::max(*backend_bytes, sb);
+    *host_bytes = std::max(*host_bytes, sh);
+}
+
+template <class T>
+void pposv_run(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, T* A, int ia, int ja,
+               const int* da, T* B, int, int, const int* db, int& info, void* backend_work,
+               std::size_t backend_bytes, void* host_work, std::size_t host_bytes)
+{
+    if (n == 0) {
+        info = 0;
+        return;
+    }
+    ppotrf_run<T>(h, uplo, n, A, ia, ja, da, info, backend_work, backend_bytes, host_work, host_bytes);
+    if (info == 0)
+        ppotrs_run<T>(h, side, uplo, trans, n, nrhs, A, da, B, db, backend_work, backend_bytes, host_work,
+                      host_bytes);
+}
+
+} // namespace cholesky_impl
+
+// ---------------------------------------------------------------------------
+// Public drivers: check, size, scratch, run, synchronize.
+// ---------------------------------------------------------------------------
+template <class T>
+void ppotrf(const DdlaHandle_t& h, char uplo, int n, T* A, int ia, int ja, const int* descA, int& info)
+{
+    cholesky_impl::ppotrf_check(h, uplo, n, A, ia, ja, descA);
+    if (n == 0) {
+        info = 0;
+        return;
+    }
+    std::size_t bb = 0, hb = 0;
+    cholesky_impl::ppotrf_workspace_size<T>(h, uplo, n, A, descA, &bb, &hb);
+    void* work = h->ensure_scratch(bb);
+    cholesky_impl::ppotrf_run<T>(h, uplo, n, A, ia, ja, descA, info, work, bb, nullptr, 0);
+}
+
+template <class T>
+void ppotrs(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const T* A, const int* descA,
+            T* B, const int* descB)
+{
+    cholesky_impl::ppotrs_check(h, side, uplo, trans, n, nrhs, A, descA, B, descB);
+    if (n == 0 || nrhs == 0) return;
+    std::size_t bb = 0, hb = 0;
+    cholesky_impl::ppotrs_workspace_size<T>(h, side, n, nrhs, descA, descB, &bb, &hb);
+    void* work = h->ensure_scratch(bb);
+    cholesky_impl::ppotrs_run<T>(h, side, uplo, trans, n, nrhs, A, descA, B, descB, work, bb, nullptr, 0);
+    DDLA_CHECK(ddlaSynchronize(h));
+}
+
+template <class T>
+void pposv(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, T* A, int ia, int ja,
+           const int* descA, T* B, int ib, int jb, const int* descB, int& info)
+{
+    cholesky_impl::pposv_check(h, side, uplo, trans, n, nrhs, A, ia, ja, descA, B, ib, jb, descB);
+    if (n == 0) {
+        info = 0;
+        return;
+    }
+    std::size_t bb = 0, hb = 0;
+    cholesky_impl::pposv_workspace_size<T>(h, side, uplo, n, nrhs, A, descA, descB, &bb, &hb);
+    void* work = h->ensure_scratch(bb);
+    cholesky_impl::pposv_run<T>(h, side, uplo, trans, n, nrhs, A, ia, ja, descA, B, ib, jb, descB, info, work, bb,
+                                nullptr, 0);
+    DDLA_CHECK(ddlaSynchronize(h));
+}
+
+template <class T>
+void potrf(const DdlaHandle_t& h, char uplo, int n, T* A, int lda, int* info)
+{
+    require_gpu(h, "potrf");
+    const char u = upper_char(uplo);
+    if (u != 'L' && u != 'U') throw std::invalid_argument("potrf: uplo must be 'U' or 'L'");
+    if (n < 0) throw std::runtime_error("potrf: negative matrix size");
+    if (info == nullptr) throw std::runtime_error("potrf: null info");
+    if (n == 0) {
+        fill_int(h->stream, info, 0);
+        return;
+    }
+    if (A == nullptr) throw std::runtime_error("potrf: null A");
+    if (lda < std::max(1, n)) throw std::runtime_error("potrf: lda too small");
+    T* inv = static_cast<T*>(h->ensure_scratch(detail::padded_bytes<T>((std::size_t)kIB * kIB)));
+    fill_int(h->stream, info, 0);
+    potrf_blocked<T>(h, u == 'U', n, A, lda, info, inv);
+}
+
+template <class T>
+void diag_logdet(const DdlaHandle_t& h, int n, const T* a, int lda, std::complex<typename real_type<T>::type>& result)
+{
+    using R = typename real_type<T>::type;
+    require_gpu(h, "diag_logdet");
+    if (n < 0 || lda < std::max(1, n) || (n > 0 && !a))
+        throw std::invalid_argument("diag_logdet: invalid order, leading dimension or pointer");
+    if (n > 0) {
+        const std::uint64_t last = std::uint64_t(n - 1) * std::uint64_t(lda) + n - 1;
+        if (last > std::uint64_t(std::numeric_limits<std::ptrdiff_t>::max()) / sizeof(T))
+            throw std::overflow_error("diag_logdet: matrix address span is not representable");
+    }
+    if (n == 0) {
+        result = {R(0), R(0)};
+        return;
+    }
+    const std::size_t part = detail::padded_bytes<char>(kern::logdet_work_bytes());
+    char* w = static_cast<char*>(h->ensure_scratch(part + detail::padded_bytes<kern::LogdetResult>(1)));
+    auto* d_out = reinterpret_cast<kern::LogdetResult*>(w + part);
+    kern::diag_logdet<T>(h->stream, n, a, lda, w, d_out);
+    kern::LogdetResult out{};
+    DDLA_HIP(hipMemcpyAsync(&out, d_out, sizeof(out), hipMemcpyDeviceToHost, h->stream));
+    DDLA_HIP(hipStreamSynchronize(h->stream));
+    if (out.flags & 1) throw std::domain_error("diag_logdet: nonfinite diagonal");
+    if (out.flags & 2) {
+        result = {-std::numeric_limits<R>::infinity(), R(0)};
+        return;
+    }
+    const double pi = std::acos(-1.0);
+    double phase = std::remainder(((out.flags >> 2) & 1) * pi, 2 * pi);
+    if (phase <= -pi) phase = pi;
+    result = {static_cast<R>(out.magnitude), static_cast<R>(phase)};
+}
+
+template <class T>
+void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* d_m, int* d_n, int* d_k, T alpha,
+                  const T* const* d_A, int* d_lda, const T* const* d_B, int* d_ldb, T beta, T** d_C, int* d_ldc,
+                  int batch)
+{
+    if (h == nullptr || !h->initialized || h->backend != DdlaBackend::GPU)
+        throw std::invalid_argument("gemmVbatched: invalid DDLA handle");
+    if (batch < 0) throw std::invalid_argument("gemmVbatched: negative batch count");
+    if (batch == 0) return;
+    if (d_A == nullptr || d_B == nullptr || d_C == nullptr)
+        throw std::invalid_argument("gemmVbatched: null matrix pointer array");
+    auto valid = [](char op) {
+        return op == 'N' || op == 'n' || op == 'T' || op == 't' || op == 'C' || op == 'c';
+    };
+    if (!valid(transA) || !valid(transB)) throw std::invalid_argument("gemmVbatched: invalid transpose operation");
+    if (!d_m || !d_n || !d_k || !d_lda || !d_ldb || !d_ldc)
+        throw std::invalid_argument("gemmVbatched: null dimension array");
+    int* summary = static_cast<int*>(h->ensure_scratch(detail::padded_bytes<int>(4)));
+    const bool na = transA == 'N' || transA == 'n', nb = transB == 'N' || transB == 'n';
+    kern::vbatched_validate(h->stream, na, nb, d_m, d_n, d_k, d_lda, d_ldb, d_ldc, batch, summary);
+    int host[4] = {0, 0, 0, 0};
+    DDLA_HIP(hipMemcpyAsync(host, summary, sizeof(host), hipMemcpyDeviceToHost, h->stream));
+    DDLA_HIP(hipStreamSynchronize(h->stream));
+    if (host[3] != 0)
+        throw std::invalid_argument("gemmVbatched: invalid dimensions at batch " + std::to_string(host[3] - 1));
+    kern::gemm_vbatched<T>(h->stream, transA, transB, d_m, d_n, d_k, alpha, d_A, d_lda, d_B, d_ldb, beta, d_C,
+                           d_ldc, batch, host[0], host[1]);
+}
+
+#define DDLA_INSTANTIATE(T)                                                                                    \
+    template void ppotrf<T>(const DdlaHandle_t&, char, int, T*, int, int, const int*, int&);                  \
+    template void ppotrs<T>(const DdlaHandle_t&, char, char, char, int, int, const T*, const int*, T*,         \
+                            const int*);                                                                       \
+    template void pposv<T>(const DdlaHandle_t&, char, char, char, int, int, T*, int, int, const int*, T*, int, \
+                           int, const int*, int&);                                                             \
+    template void potrf<T>(const DdlaHandle_t&, char, int, T*, int, int*);                                     \
+    template void diag_logdet<T>(const DdlaHandle_t&, int, const T*, int, std::complex<real_type<T>::type>&);  \
+    template void gemmVbatched<T>(const DdlaHandle_t&, char, char, int*, int*, int*, T, const T* const*, int*, \
+                                  const T* const*, int*, T, T**, int*, int);                                   \
+    template void cholesky_impl::ppotrf_workspace_size<T>(const DdlaHandle_t&, char, int, T*, const int*,      \
+                                                          std::size_t*, std::size_t*);                         \
+    template void cholesky_impl::ppotrf_run<T>(const DdlaHandle_t&, char, int, T*, int, int, const int*, int&, \
+                                               void*, std::size_t, void*, std::size_t);                        \
+    template void cholesky_impl::ppotrs_workspace_size<T>(const DdlaHandle_t&, char, int, int, const int*,     \
+                                                          const int*, std::size_t*, std::size_t*);             \
+    template void cholesky_impl::ppotrs_run<T>(const DdlaHandle_t&, char, char, char, int, int, const T*,      \
+                                               const int*, T*, const int*, void*, std::size_t, void*,          \
+                                               std::size_t);                                                   \
+    template void cholesky_impl::pposv_workspace_size<T>(const DdlaHandle_t&, char, char, int, int, T*,        \
+                                                         const int*, const int*, std::size_t*, std::size_t*);  \
+    template void cholesky_impl::pposv_run<T>(const DdlaHandle_t&, char, char, char, int, int, T*, int, int,   \
+                                              const int*, T*, int, int, const int*, int&, void*, std::size_t,  \
+                                              void*, std::size_t);
+DDLA_INSTANTIATE(float)
+DDLA_INSTANTIATE(double)
+#undef DDLA_INSTANTIATE
+
+} // namespace ddla

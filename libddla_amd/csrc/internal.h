@@ -1,0 +1,102 @@
+// Private runtime state of a handle (the reference's DdlaStream,
+// src/ddla_stream.h:18-128, reduced to one GPU and a 1 x 1 grid).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <ddla/ddla.h>
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ddla {
+
+#define DDLA_HIP(call)                                                                                         \
+    do {                                                                                                       \
+        const hipError_t ddla_e_ = (call);                                                                     \
+        if (ddla_e_ != hipSuccess)                                                                             \
+            throw std::runtime_error(std::string(#call) + " failed: " + hipGetErrorString(ddla_e_));           \
+    } while (0)
+
+// Kernel classes timed by the optional per-launch event profiler.
+enum KernelClass {
+    kClassTrailingUpdate = 0,  // syrk-shaped trailing update of potrf (MFMA)
+    kClassPanelTrsm = 1,       // panel triangular solve (MFMA with the inverted diagonal block)
+    kClassPanelUpdate = 2,     // update of the remaining panel columns (MFMA)
+    kClassPotf2 = 3,           // diagonal 64 x 64 factorization + inverse
+    kClassSolveUpdate = 4,     // potrs off-diagonal updates (MFMA)
+    kClassSolveDiag = 5,       // potrs diagonal-block applications
+    kClassTrtri = 6,           // inversion of the diagonal blocks for potrs
+    kClassOther = 7,
+    kClassCount = 8
+};
+
+class DdlaStream {
+public:
+    DdlaBackend backend = DdlaBackend::GPU;
+    bool initialized = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+    int nprows_ = 0, npcols_ = 0, myprow_ = 0, mypcol_ = 0, nprocs = 0;
+    char major = 'R';
+
+    // Cached device scratch for the public entry points (grown on demand).
+    void* scratch = nullptr;
+    std::size_t scratch_bytes = 0;
+    void* ensure_scratch(std::size_t bytes);
+
+    // Per-launch event profiler (off by default).
+    struct Record {
+        int cls;
+        hipEvent_t start, stop;
+        double flops;
+        double bytes;
+    };
+    bool profiling = false;
+    std::vector<Record> records;
+    std::vector<hipEvent_t> event_pool;
+    std::size_t events_used = 0;
+    hipEvent_t next_event();
+    void release_events();
+};
+
+// Brackets one kernel launch with events when the handle profiles.
+class LaunchTimer {
+public:
+    LaunchTimer(DdlaStream* h, int cls, double flops, double bytes = 0);
+    ~LaunchTimer();
+    LaunchTimer(const LaunchTimer&) = delete;
+    LaunchTimer& operator=(const LaunchTimer&) = delete;
+
+private:
+    DdlaStream* h_;
+    int cls_;
+    double flops_, bytes_;
+    hipEvent_t start_ = nullptr;
+};
+
+void require_gpu(const DdlaHandle_t& h, const char* who);
+
+namespace detail {
+// Bytes take<T>(cursor, count) consumes: count elements padded to 256 bytes
+// (src/plan_util.h:28-43).
+template <typename T>
+constexpr std::size_t padded_bytes(std::size_t count)
+{
+    return (count * sizeof(T) + 255) / 256 * 256;
+}
+template <typename T>
+inline T* take(char*& cursor, std::size_t count)
+{
+    T* p = reinterpret_cast<T*>(cursor);
+    cursor += padded_bytes<T>(count);
+    return p;
+}
+} // namespace detail
+
+// Launch helpers shared by the Cholesky schedules.
+void fill_int(hipStream_t s, int* p, int value);
+
+} // namespace ddla

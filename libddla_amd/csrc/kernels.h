@@ -1,0 +1,71 @@
+// Launchers of the hand-written CDNA4 (gfx950) kernels.  Host code calls only
+// these; every launcher enqueues on the given stream and returns immediately.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ddla {
+namespace kern {
+
+// Triangle mask applied by the GEMM epilogue, in the coordinates of the
+// matrix C that is written: element (i, j) of C is written only if
+//   kTriNone, or kTriLower and i + diag >= j, or kTriUpper and i + diag <= j.
+enum Tri { kTriNone = 0, kTriLower = 1, kTriUpper = 2 };
+
+// C := alpha * op(A) * op(B) + beta * C, op(X) = X or X^T ('N' / 'T' / 'C').
+// alpha == 0 or k == 0 only scales C; beta == 0 never reads C.  If abort is
+// non-null and *abort != 0 when a workgroup starts, the workgroup does nothing
+// (the factorization stops at the first failed pivot).  In-place use with C
+// aliasing op(B) is safe when m <= 64 (one workgroup row covers all of m),
+// and with C aliasing op(A) when n <= 64.
+template <class T>
+void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, long long lda,
+          const T* B, long long ldb, T beta, T* C, long long ldc, Tri tri = kTriNone, long long diag = 0,
+          const int* abort = nullptr);
+
+// Variable-size batched GEMM over device arrays of sizes, leading dimensions
+// and pointers; max_m / max_n size the grid.
+template <class T>
+void gemm_vbatched(hipStream_t s, char ta, char tb, const int* m, const int* n, const int* k, T alpha,
+                   const T* const* A, const int* lda, const T* const* B, const int* ldb, T beta, T* const* C,
+                   const int* ldc, int batch, int max_m, int max_n);
+
+// Validation + max reduction of vbatched dimensions: summary[0..2] = max m, n,
+// k; summary[3] = 1 + first batch index with a negative size or a too-small
+// leading dimension (0 if none).  summary must be zeroed beforehand.
+void vbatched_validate(hipStream_t s, int na, int nb, const int* m, const int* n, const int* k, const int* lda,
+                       const int* ldb, const int* ldc, int batch, int* summary);
+
+// One workgroup factors the order-w (w <= 64) diagonal block whose first
+// element is A (L view: element (i, j) at A[i + j*lda], or A[j + i*lda] when
+// upper), writes the factor back and the inverse of the factor into inv
+// (64 x 64, column-major, zero above the diagonal and identity beyond w).
+// On a non-positive or NaN pivot p (0-based) it sets *info = info_base + p + 1
+// and returns.  Does nothing if *info != 0 on entry.
+template <class T>
+void potf2_inv(hipStream_t s, bool upper, int w, T* A, long long lda, T* inv, int* info, int info_base);
+
+// Inverts every 64 x 64 diagonal block of the order-n factor (L view as for
+// potf2_inv) into inv[b * 4096 ...], identity-padded for a ragged last block.
+template <class T>
+void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv);
+
+// diag_logdet reduction over A[i * (lda + 1)], i < n: work must hold
+// logdet_work_bytes(); out (device) receives {sum log|d|, flags} where flags
+// = 1 nonfinite | 2 zero | 4 odd number of negative entries.
+struct LogdetResult { double magnitude; int flags; int pad; };
+size_t logdet_work_bytes();
+template <class T>
+void diag_logdet(hipStream_t s, int n, const T* A, long long lda, void* work, LogdetResult* out);
+
+// Setup helpers (not on the timed path).
+void fill_normal(hipStream_t s, uint64_t seed, uint64_t stream, uint64_t first, long long count, double* out);
+template <class T>
+void add_diagonal(hipStream_t s, int n, T* A, long long lda, T delta);
+template <class T>
+void transpose(hipStream_t s, int m, int n, const T* in, long long ldi, T* out, long long ldo);
+template <class D, class S>
+void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, long long ldo);
+
+} // namespace kern
+} // namespace ddla

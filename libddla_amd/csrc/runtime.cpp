@@ -1,0 +1,311 @@
+// Handle lifecycle, memory helpers, descriptor and the launch profiler.
+// Follows the reference's runtime for a single GPU and a 1 x 1 grid:
+// src/ddla_handle_t.cpp:94-200 (init / set / destroy), :322-770 (memory),
+// src/ddla_desc.cpp:9-60 (check_desc), include/ddla/ddla_desc.h:102-135.
+#include "internal.h"
+
+#include <cstring>
+
+namespace ddla {
+
+namespace {
+thread_local std::string g_status_text;
+
+bool is_device_pointer(const void* p)
+{
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+__global__ void fill_int_kernel(int* p, int value) { *p = value; }
+} // namespace
+
+void fill_int(hipStream_t s, int* p, int value)
+{
+    fill_int_kernel<<<1, 1, 0, s>>>(p, value);
+    DDLA_HIP(hipGetLastError());
+}
+
+void require_gpu(const DdlaHandle_t& h, const char* who)
+{
+    if (h == nullptr) throw std::runtime_error(std::string(who) + ": null handle");
+    if (!h->initialized) throw std::runtime_error(std::string(who) + ": handle has no process grid (call ddlaSet)");
+    if (h->backend != DdlaBackend::GPU) throw std::runtime_error(std::string(who) + ": GPU handle required");
+}
+
+void* DdlaStream::ensure_scratch(std::size_t bytes)
+{
+    if (bytes <= scratch_bytes) return scratch;
+    if (scratch != nullptr) {
+        DDLA_HIP(hipStreamSynchronize(stream));
+        DDLA_HIP(hipFree(scratch));
+        scratch = nullptr;
+        scratch_bytes = 0;
+    }
+    DDLA_HIP(hipMalloc(&scratch, bytes));
+    scratch_bytes = bytes;
+    return scratch;
+}
+
+hipEvent_t DdlaStream::next_event()
+{
+    if (events_used == event_pool.size()) {
+        hipEvent_t e;
+        DDLA_HIP(hipEventCreate(&e));
+        event_pool.push_back(e);
+    }
+    return event_pool[events_used++];
+}
+
+void DdlaStream::release_events()
+{
+    records.clear();
+    events_used = 0;
+}
+
+LaunchTimer::LaunchTimer(DdlaStream* h, int cls, double flops, double bytes)
+    : h_(h), cls_(cls), flops_(flops), bytes_(bytes)
+{
+    if (h_ != nullptr && h_->profiling) {
+        start_ = h_->next_event();
+        DDLA_HIP(hipEventRecord(start_, h_->stream));
+    }
+}
+
+LaunchTimer::~LaunchTimer()
+{
+    if (start_ == nullptr) return;
+    hipEvent_t stop = h_->next_event();
+    if (hipEventRecord(stop, h_->stream) == hipSuccess)
+        h_->records.push_back({cls_, start_, stop, flops_, bytes_});
+}
+
+bool ddlaBackendAvailable(DdlaBackend backend) { return backend == DdlaBackend::GPU; }
+
+ddlaStatus_t ddlaInit(DdlaHandle_t& handle, DdlaBackend requested_backend)
+{
+    if (!ddlaBackendAvailable(requested_backend))
+        throw std::runtime_error("ddlaInit: requested backend is not available in this build (GPU only)");
+    handle = new DdlaStream;
+    handle->backend = requested_backend;
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaSet(DdlaHandle_t handle, int nprows, int npcols, char major)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (nprows <= 0 || npcols <= 0) nprows = npcols = 1;
+    if (nprows * npcols != 1)
+        throw std::invalid_argument("ddlaSet: this build runs one process on one GPU; the grid must be 1 x 1");
+    if (major != 'R' && major != 'C' && major != 'r' && major != 'c')
+        return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return ddlaStatus_t::DDLA_STATUS_BACKEND_UNAVAILABLE;
+    }
+    if (!handle->initialized) {
+        DDLA_HIP(hipGetDevice(&handle->device));
+        DDLA_HIP(hipStreamCreate(&handle->stream));
+        handle->owns_stream = true;
+    }
+    handle->nprows_ = handle->npcols_ = handle->nprocs = 1;
+    handle->myprow_ = handle->mypcol_ = 0;
+    handle->major = (major == 'c' || major == 'C') ? 'C' : 'R';
+    handle->initialized = true;
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaDestroy(DdlaHandle_t& handle)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    if (handle->initialized) {
+        (void)hipStreamSynchronize(handle->stream);
+        if (handle->scratch != nullptr) (void)hipFree(handle->scratch);
+        for (hipEvent_t e : handle->event_pool) (void)hipEventDestroy(e);
+        if (handle->owns_stream && handle->stream != nullptr) (void)hipStreamDestroy(handle->stream);
+    }
+    delete handle;
+    handle = nullptr;
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+DdlaBackend ddlaGetBackend(const DdlaHandle_t& handle)
+{
+    if (handle == nullptr) throw std::runtime_error("ddlaGetBackend: null handle");
+    return handle->backend;
+}
+
+static ddlaStatus_t internal_error(hipError_t e)
+{
+    g_status_text = hipGetErrorString(e);
+    (void)hipGetLastError();
+    return ddlaStatus_t::DDLA_STATUS_INTERNAL_ERROR;
+}
+
+ddlaStatus_t ddlaMalloc(const DdlaHandle_t& handle, void** ptr, std::size_t bytes)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (ptr == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    if (bytes == 0) {
+        *ptr = nullptr;
+        return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    }
+    const hipError_t e = hipMalloc(ptr, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        return ddlaStatus_t::DDLA_STATUS_ALLOC_FAILED;
+    }
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaFree(const DdlaHandle_t& handle, void* ptr)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (ptr == nullptr) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    if (handle->stream != nullptr) (void)hipStreamSynchronize(handle->stream);
+    const hipError_t e = hipFree(ptr);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaMemcpy(const DdlaHandle_t& handle, void* dst, const void* src, std::size_t bytes)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (bytes == 0) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    if (dst == nullptr || src == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const bool dd = is_device_pointer(dst), sd = is_device_pointer(src);
+    if (!dd && !sd) {
+        std::memcpy(dst, src, bytes);
+        return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    }
+    const hipMemcpyKind kind = sd ? (dd ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost) : hipMemcpyHostToDevice;
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, handle->stream);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaMemset(const DdlaHandle_t& handle, void* ptr, int value, std::size_t bytes)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (bytes == 0) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    if (ptr == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const hipError_t e = hipMemsetAsync(ptr, value, bytes, handle->stream);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaSynchronize(const DdlaHandle_t& handle)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    const hipError_t e = hipStreamSynchronize(handle->stream);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+const char* ddlaGetErrorString(ddlaStatus_t status)
+{
+    const char* name = "DDLA_STATUS_UNKNOWN";
+    switch (status) {
+    case ddlaStatus_t::DDLA_STATUS_SUCCESS: name = "DDLA_STATUS_SUCCESS"; break;
+    case ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE: name = "DDLA_STATUS_INVALID_HANDLE"; break;
+    case ddlaStatus_t::DDLA_STATUS_INVALID_VALUE: name = "DDLA_STATUS_INVALID_VALUE"; break;
+    case ddlaStatus_t::DDLA_STATUS_ALLOC_FAILED: name = "DDLA_STATUS_ALLOC_FAILED"; break;
+    case ddlaStatus_t::DDLA_STATUS_BACKEND_UNAVAILABLE: name = "DDLA_STATUS_BACKEND_UNAVAILABLE"; break;
+    case ddlaStatus_t::DDLA_STATUS_INTERNAL_ERROR: name = "DDLA_STATUS_INTERNAL_ERROR"; break;
+    }
+    thread_local std::string text;
+    text = name;
+    if (status == ddlaStatus_t::DDLA_STATUS_INTERNAL_ERROR && !g_status_text.empty()) text += ": " + g_status_text;
+    return text.c_str();
+}
+
+ddlaStatus_t ddlaSetStream(const DdlaHandle_t& handle, void* stream)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (!handle->initialized) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    hipStream_t next = static_cast<hipStream_t>(stream);
+    if (next == handle->stream) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    DDLA_HIP(hipStreamSynchronize(handle->stream));
+    if (handle->owns_stream && handle->stream != nullptr) DDLA_HIP(hipStreamDestroy(handle->stream));
+    handle->stream = next;
+    handle->owns_stream = next != nullptr;
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+void* ddlaGetStream(const DdlaHandle_t& handle)
+{
+    if (handle == nullptr) throw std::runtime_error("ddlaGetStream: null handle");
+    return static_cast<void*>(handle->stream);
+}
+
+int ddlaGetRank(const DdlaHandle_t& handle) { return handle == nullptr ? -1 : 0; }
+int ddlaGetSize(const DdlaHandle_t& handle) { return handle == nullptr ? -1 : (handle->initialized ? 1 : 0); }
+
+void ddlaGetGridCoords(const DdlaHandle_t& handle, int& myprow, int& mypcol)
+{
+    if (handle == nullptr) throw std::runtime_error("ddlaGetGridCoords: null handle");
+    myprow = handle->myprow_;
+    mypcol = handle->mypcol_;
+}
+
+void ddlaGetGridDims(const DdlaHandle_t& handle, int& nprows, int& npcols)
+{
+    if (handle == nullptr) throw std::runtime_error("ddlaGetGridDims: null handle");
+    nprows = handle->nprows_;
+    npcols = handle->npcols_;
+}
+
+ddlaStatus_t ddlaDescInit(int* desc, const DdlaHandle_t& handle, int m, int n, int mb, int nb, int irsrc, int icsrc,
+                          int lld)
+{
+    if (desc == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    if (handle == nullptr || !handle->initialized || handle->nprows_ <= 0 || handle->npcols_ <= 0)
+        return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (m < 0 || n < 0 || mb <= 0 || nb <= 0) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    if (irsrc < 0 || irsrc >= handle->nprows_ || icsrc < 0 || icsrc >= handle->npcols_)
+        return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const int lld_min = std::max(1, num_loc(m, mb, handle->myprow_, irsrc, handle->nprows_));
+    desc[DDLA_DTYPE_] = DDLA_BLOCK_CYCLIC_2D;
+    desc[DDLA_CTXT_] = 0;
+    desc[DDLA_M_] = m;
+    desc[DDLA_N_] = n;
+    desc[DDLA_MB_] = mb;
+    desc[DDLA_NB_] = nb;
+    desc[DDLA_RSRC_] = irsrc;
+    desc[DDLA_CSRC_] = icsrc;
+    desc[DDLA_LLD_] = std::max(lld, lld_min);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+void check_desc(const int* desc, const DdlaHandle_t& handle)
+{
+    if (desc == nullptr) throw std::invalid_argument("ddla: null ScaLAPACK descriptor");
+    if (handle == nullptr) throw std::invalid_argument("ddla: null handle for ScaLAPACK descriptor");
+    if (desc[DDLA_DTYPE_] != DDLA_BLOCK_CYCLIC_2D)
+        throw std::invalid_argument("ddla: unsupported descriptor type " + std::to_string(desc[DDLA_DTYPE_]) +
+                                    "; only dense block-cyclic (DTYPE_ == 1) is supported");
+    if (!handle->initialized || handle->nprows_ <= 0 || handle->npcols_ <= 0)
+        throw std::invalid_argument("ddla: handle carries no process grid");
+    if (desc[DDLA_M_] < 0 || desc[DDLA_N_] < 0)
+        throw std::invalid_argument("ddla: descriptor M/N must be non-negative, got M=" +
+                                    std::to_string(desc[DDLA_M_]) + " N=" + std::to_string(desc[DDLA_N_]));
+    if (desc[DDLA_MB_] <= 0 || desc[DDLA_NB_] <= 0)
+        throw std::invalid_argument("ddla: descriptor MB/NB must be positive, got MB=" +
+                                    std::to_string(desc[DDLA_MB_]) + " NB=" + std::to_string(desc[DDLA_NB_]));
+    if (desc[DDLA_RSRC_] < 0 || desc[DDLA_RSRC_] >= handle->nprows_ || desc[DDLA_CSRC_] < 0 ||
+        desc[DDLA_CSRC_] >= handle->npcols_)
+        throw std::invalid_argument("ddla: descriptor RSRC/CSRC outside the process grid");
+    const int required =
+        std::max(1, num_loc(desc[DDLA_M_], desc[DDLA_MB_], handle->myprow_, desc[DDLA_RSRC_], handle->nprows_));
+    if (desc[DDLA_LLD_] < required)
+        throw std::invalid_argument("ddla: ScaLAPACK descriptor LLD_A=" + std::to_string(desc[DDLA_LLD_]) +
+                                    " is smaller than the required max(1, LOCr(M_A))=" + std::to_string(required));
+}
+
+} // namespace ddla

@@ -1,0 +1,98 @@
+"""Cholesky-path routines over DistributedMatrix (mirrors python/ddla/ops.py:113-143).
+
+Every call goes through the C ABI of libddla_mi355x.so and runs the
+hand-written gfx950 kernels; there is no host fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from ._capi import char, check_status, lib
+
+
+def _fn(mat, name):
+    return getattr(lib, f"ddla_{mat.type_char}_{name}")
+
+
+def ppotrf(A, uplo: str = "L"):
+    """Cholesky factorization in place; returns (A, info)."""
+    info = ctypes.c_int(0)
+    st = _fn(A, "ppotrf")(A.handle.ptr, char(uplo), A.n, ctypes.c_void_p(A.data_ptr), 1, 1, A.desc_ptr,
+                          ctypes.byref(info))
+    check_status(st, f"ddla_{A.type_char}_ppotrf")
+    return A, info.value
+
+
+def ppotrs(A, B, uplo: str = "L", side: str = "L", trans: str = "N"):
+    """Solve with the factor from ppotrf: A X = B (side L) or X A = B (side R), B overwritten."""
+    n = A.n
+    nrhs = B.n if char(side) == b"L" else B.m
+    st = _fn(A, "ppotrs")(A.handle.ptr, char(side), char(uplo), char(trans), n, nrhs, ctypes.c_void_p(A.data_ptr),
+                          A.desc_ptr, ctypes.c_void_p(B.data_ptr), B.desc_ptr)
+    check_status(st, f"ddla_{A.type_char}_ppotrs")
+    return B
+
+
+def pposv(A, B, uplo: str = "L", side: str = "L", trans: str = "N"):
+    """Factor and solve (the reference's pposv driver); B is untouched when info != 0."""
+    info = ctypes.c_int(0)
+    n = A.n
+    nrhs = B.n if char(side) == b"L" else B.m
+    st = _fn(A, "pposv")(A.handle.ptr, char(side), char(uplo), char(trans), n, nrhs, ctypes.c_void_p(A.data_ptr),
+                         1, 1, A.desc_ptr, ctypes.c_void_p(B.data_ptr), 1, 1, B.desc_ptr, ctypes.byref(info))
+    check_status(st, f"ddla_{A.type_char}_pposv")
+    return B, info.value
+
+
+def ppotrf_workspace_size(A, uplo: str = "L"):
+    bb, hb = ctypes.c_longlong(), ctypes.c_longlong()
+    check_status(_fn(A, "ppotrf_workspace_size")(A.handle.ptr, char(uplo), A.n, A.desc_ptr, ctypes.byref(bb),
+                                                 ctypes.byref(hb)), "ppotrf_workspace_size")
+    return bb.value, hb.value
+
+
+def ppotrf_run(A, uplo, backend_work: int, backend_bytes: int, host_work: int = 0, host_bytes: int = 0):
+    info = ctypes.c_int(0)
+    st = _fn(A, "ppotrf_run")(A.handle.ptr, char(uplo), A.n, ctypes.c_void_p(A.data_ptr), 1, 1, A.desc_ptr,
+                              ctypes.byref(info), ctypes.c_void_p(backend_work), backend_bytes,
+                              ctypes.c_void_p(host_work), host_bytes)
+    check_status(st, f"ddla_{A.type_char}_ppotrf_run")
+    return info.value
+
+
+def ppotrs_workspace_size(A, B, side: str = "L"):
+    bb, hb = ctypes.c_longlong(), ctypes.c_longlong()
+    nrhs = B.n if char(side) == b"L" else B.m
+    check_status(_fn(A, "ppotrs_workspace_size")(A.handle.ptr, char(side), A.n, nrhs, A.desc_ptr, B.desc_ptr,
+                                                 ctypes.byref(bb), ctypes.byref(hb)), "ppotrs_workspace_size")
+    return bb.value, hb.value
+
+
+def pposv_workspace_size(A, B, uplo: str = "L", side: str = "L"):
+    bb, hb = ctypes.c_longlong(), ctypes.c_longlong()
+    nrhs = B.n if char(side) == b"L" else B.m
+    check_status(_fn(A, "pposv_workspace_size")(A.handle.ptr, char(side), char(uplo), A.n, nrhs, A.desc_ptr,
+                                                B.desc_ptr, ctypes.byref(bb), ctypes.byref(hb)),
+                 "pposv_workspace_size")
+    return bb.value, hb.value
+
+
+def pposv_run(A, B, uplo, backend_work: int, backend_bytes: int, side: str = "L", trans: str = "N"):
+    info = ctypes.c_int(0)
+    nrhs = B.n if char(side) == b"L" else B.m
+    st = _fn(A, "pposv_run")(A.handle.ptr, char(side), char(uplo), char(trans), A.n, nrhs,
+                             ctypes.c_void_p(A.data_ptr), 1, 1, A.desc_ptr, ctypes.c_void_p(B.data_ptr), 1, 1,
+                             B.desc_ptr, ctypes.byref(info), ctypes.c_void_p(backend_work), backend_bytes,
+                             ctypes.c_void_p(0), 0)
+    check_status(st, f"ddla_{A.type_char}_pposv_run")
+    return info.value
+
+
+def diag_logdet(A, n: int = None):
+    """Principal log of the product of the diagonal: complex (sum log|d|, phase)."""
+    n = A.m if n is None else n
+    res = np.zeros(2, dtype=A.dtype)
+    st = _fn(A, "diag_logdet")(A.handle.ptr, n, ctypes.c_void_p(A.data_ptr), A.lld,
+                               ctypes.c_void_p(res.ctypes.data))
+    check_status(st, f"ddla_{A.type_char}_diag_logdet")
+    return complex(float(res[0]), float(res[1]))

@@ -1,0 +1,253 @@
+"""Pins the CPU oracle (oracle/ddla_oracle.c) against facts that do not come from it.
+
+Each oracle function is checked against at least one of: library output
+(numpy's Philox, scipy's LAPACK fixtures), the reference's closed-form test
+matrices, brute force on tiny inputs, or invariants; and the planted-mistake
+tests show that a plausible error in a factor or a solution fails these checks.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lapack_cases.npz"))
+M64 = (1 << 64) - 1
+
+
+def numpy_philox_block(ctr, key):
+    """numpy's Philox increments its 256-bit counter before generating a block."""
+    c = sum(int(w) << (64 * i) for i, w in enumerate(ctr))
+    c = (c - 1) % (1 << 256)
+    counter = [(c >> (64 * i)) & M64 for i in range(4)]
+    bg = np.random.Philox(key=np.array(key, dtype=np.uint64), counter=np.array(counter, dtype=np.uint64))
+    return [int(x) for x in bg.random_raw(4)]
+
+
+@pytest.mark.parametrize("key", [(0, 0), (7, 11), (M64, 5)])
+@pytest.mark.parametrize("ctr", [(0, 0, 0, 0), (1, 0, 0, 0), (5, 9, 0, 0), (M64, 3, 7, 2)])
+def test_philox_matches_numpy(ctr, key):
+    assert orc.philox(ctr, key) == numpy_philox_block(ctr, key)
+
+
+def test_fill_normal_is_box_muller_of_pinned_words():
+    seed, stream = 3, 1
+    vals = orc.fill_normal(seed, stream, 5, 40)
+    for t, v in enumerate(vals):
+        e = 5 + t
+        w = numpy_philox_block((e >> 2, stream, 0, 0), (seed, 0))
+        word, pair = e & 3, (e & 3) >> 1
+        u1 = ((w[2 * pair] >> 11) + 1) * 2.0 ** -53
+        u2 = (w[2 * pair + 1] >> 11) * 2.0 ** -53
+        r = math.sqrt(-2.0 * math.log(u1))
+        z = r * (math.sin(2 * math.pi * u2) if word & 1 else math.cos(2 * math.pi * u2))
+        assert v == pytest.approx(z, rel=1e-15, abs=1e-300)
+
+
+def test_fill_normal_moments():
+    x = orc.fill_normal(0, 0, 0, 400_000)
+    assert abs(x.mean()) < 0.006
+    assert abs(x.var() - 1) < 0.01
+    assert abs((x ** 4).mean() - 3) < 0.05
+
+
+def _tri(uplo, a):
+    return np.tril(a) if uplo == "L" else np.triu(a)
+
+
+def fixture_keys():
+    return sorted(k[len("potrf_in_"):] for k in GOLD.files if k.startswith("potrf_in_"))
+
+
+@pytest.mark.parametrize("key", fixture_keys())
+def test_potrf_matches_lapack_fixture(key):
+    tag, uplo, n, pad = key.split("_")
+    n = int(n)
+    a = GOLD[f"potrf_in_{key}"]
+    f, info = orc.potrf(uplo, a)
+    assert info == 0
+    want = GOLD[f"potrf_out_{key}"]
+    got = _tri(uplo, f[:n, :n])
+    tol = 1e-13 if tag == "d" else 2e-6
+    assert np.abs(got - want).max() <= tol * max(1.0, np.abs(want).max())
+    # the other triangle and the padding are neither read nor written: they stay NaN
+    other = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+    assert np.isnan(f[:n, :n][other]).all()
+    assert np.isnan(f[n:, :]).all()
+
+
+@pytest.mark.parametrize("key", fixture_keys())
+def test_potrs_matches_lapack_fixture(key):
+    tag, uplo, n, pad = key.split("_")
+    factor = GOLD[f"potrf_out_{key}"]
+    b = GOLD[f"potrs_b_{key}"]
+    x = orc.potrs("L", uplo, factor, b)
+    want = GOLD[f"potrs_x_{key}"]
+    tol = 1e-12 if tag == "d" else 1e-4
+    assert np.abs(x - want).max() <= tol * max(1.0, np.abs(want).max())
+
+
+def analytic_factor(n):
+    """Real part of factor(i, j) in tests/test_ppotrf_grids.cpp:17-20."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    l = 0.003 * (1 + (i + j) % 5)
+    l = np.where(i == j, 2 + 0.01 * i, l)
+    return np.tril(l)
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 38, 129])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrf_recovers_analytic_factor(n, uplo):
+    l = analytic_factor(n)
+    a = l @ l.T
+    f, info = orc.potrf(uplo, a)
+    assert info == 0
+    got = np.tril(f) if uplo == "L" else np.triu(f).T
+    assert (np.abs(got - l) / (1 + np.abs(l))).max() < 3e-12
+
+
+def test_potrf_n1_known_answer():
+    f, info = orc.potrf("L", np.array([[4.0]]))
+    assert info == 0 and f[0, 0] == 2.0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrf_final_negative_pivot_reports_n(dtype, uplo):
+    a = np.eye(9, dtype=dtype)
+    a[-1, -1] = -1
+    _, info = orc.potrf(uplo, a)
+    assert info == 9
+
+
+def test_potrf_nan_pivot_reports_its_index():
+    a = np.eye(5)
+    a[2, 2] = np.nan
+    _, info = orc.potrf("L", a)
+    assert info == 3
+
+
+def test_potrf_reconstruction_and_planted_mistakes():
+    rng = np.random.RandomState(1)
+    n = 60
+    m = rng.standard_normal((n, n))
+    a = m @ m.T / n + np.eye(n)
+    f, info = orc.potrf("L", a)
+    l = np.tril(f)
+    tol = 200 * np.finfo(float).eps * n * n  # tests/test_backend_lapack.cpp:313-334
+    assert np.abs(a - l @ l.T).max() <= tol
+    # a dropped term of the update, a transposed factor and a wrong sign are all caught
+    bad = l.copy()
+    bad[30, 10] += 1e-9
+    assert np.abs(a - bad @ bad.T).max() > tol
+    assert np.abs(a - l.T @ l).max() > tol
+    bad = l.copy()
+    bad[40, 3] = -bad[40, 3]
+    assert np.abs(a - bad @ bad.T).max() > tol
+
+
+def hpd_real(n):
+    """Real part of hpd_value (tests/api_grid_test_common.h:490-500)."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    lo, hi = np.minimum(i, j), np.maximum(i, j)
+    a = 0.01 * ((lo + 2 * hi) % 5 - 2)
+    return np.where(i == j, 5.0 + 0.2 * n + 0.05 * i, a)
+
+
+def x_real(n, nrhs):
+    """Real part of x_value (tests/api_grid_test_common.h:509-512)."""
+    i, j = np.meshgrid(np.arange(n), np.arange(nrhs), indexing="ij")
+    return 0.2 + 0.03 * i - 0.02 * j
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_potrs_closed_form_solution(uplo, side):
+    n, nrhs = 40, 5
+    a = hpd_real(n)
+    f, info = orc.potrf(uplo, a)
+    assert info == 0
+    x_true = x_real(n, nrhs) if side == "L" else x_real(n, nrhs).T
+    b = a @ x_true if side == "L" else x_true @ a
+    x = orc.potrs(side, uplo, f, b)
+    assert np.abs(x - x_true).max() < 5e-9
+    # planted mistake: solving with the transposed factor misses the closed form
+    wrong = orc.potrs(side, uplo, f.T.copy(), b)
+    assert np.abs(wrong - x_true).max() > 1e-6
+
+
+def logdet_reference(d):
+    """Long-double sum of logs and phase (tests/test_diag_logdet.cpp:21-30)."""
+    mag = np.longdouble(0)
+    negatives = 0
+    for x in d:
+        if x == 0:
+            return complex(-np.inf, 0)
+        mag += np.log(np.longdouble(abs(x)))
+        negatives += x < 0
+    # for real entries the summed phase atan2(0, x) is exactly pi per negative entry
+    return complex(float(mag), math.pi if negatives % 2 else 0.0)
+
+
+def phase_close(a, b, tol):
+    return abs(math.remainder(a - b, 2 * math.pi)) < tol
+
+
+@pytest.mark.parametrize("d", [[2, 3, 5], [-2, 3, 5], [-2, -3, 5], [0, -3, 5],
+                               [np.finfo(float).max, np.finfo(float).tiny * 2 ** -52],
+                               [1 + i % 3 for i in range(11)], [-1.0] * 1025, []])
+def test_logdet_cases(d):
+    n = len(d)
+    a = np.full((n + 3, max(n, 1)), np.nan)
+    for i, x in enumerate(d):
+        a[i, i] = x
+    mag, flags = orc.diag_logdet(a, n)
+    got = orc.finish_logdet(mag, flags)
+    want = logdet_reference(d)
+    if math.isinf(want.real):
+        assert math.isinf(got.real) and got.real < 0 and got.imag == 0
+    else:
+        assert abs(got.real - want.real) < 3e-12 * (1 + abs(want.real))
+        assert phase_close(got.imag, want.imag, 3e-12)
+
+
+@pytest.mark.parametrize("bad", [np.inf, np.nan, -np.inf])
+def test_logdet_nonfinite_flag(bad):
+    a = np.diag([1.0, bad])
+    mag, flags = orc.diag_logdet(a)
+    assert flags & 1
+    with pytest.raises(ValueError):
+        orc.finish_logdet(mag, flags)
+
+
+def closed_form(count, shape):
+    """value(i) = ((i % 13) - 6) / 7 of tests/test_gemm_vbatched.cpp:26-43, column-major fill."""
+    v = ((np.arange(count) % 13) - 6) / 7.0
+    return v.reshape(shape[::-1]).T.copy()
+
+
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_gemm_against_numpy(ta, tb):
+    m, n, k = 7, 5, 9
+    a = closed_form(m * k, (m, k) if ta == "N" else (k, m))
+    b = closed_form(k * n, (k, n) if tb == "N" else (n, k))
+    c = closed_form(m * n, (m, n)) + 0.25
+    opa = a if ta == "N" else a.T
+    opb = b if tb == "N" else b.T
+    got = orc.gemm(ta, tb, m, n, k, 1.5, a, b, -0.5, c)
+    assert np.abs(got - (1.5 * opa @ opb - 0.5 * c)).max() < 1e-13
+
+
+def test_gemm_special_cases():
+    a = np.full((3, 2), np.nan)
+    b = np.full((2, 4), np.nan)
+    c = np.ones((3, 4))
+    assert (orc.gemm("N", "N", 3, 4, 2, 0.0, a, b, 2.0, c) == 2.0).all()  # alpha 0: A, B unread
+    assert (orc.gemm("N", "N", 3, 4, 0, 1.0, a, b, 3.0, c) == 3.0).all()  # k 0: scale only
+    a = np.ones((3, 2))
+    b = np.ones((2, 4))
+    c = np.full((3, 4), np.nan)
+    assert (orc.gemm("N", "N", 3, 4, 2, 1.0, a, b, 0.0, c) == 2.0).all()  # beta 0: C unread
