@@ -1,0 +1,360 @@
+// MFMA GEMM tile for gfx950 (device code, included by kernels.hip).
+//
+// C := alpha * op(A) * op(B) + beta * C on 64-wide wavefronts:
+//   * a 256-thread workgroup (4 wavefronts, 2 x 2) owns a BM x BN tile of C,
+//     each wavefront a (BM/2) x (BN/2) block of 16 x 16 MFMA tiles
+//     (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32);
+//   * op(A) and op(B) stream through two alternating LDS slabs, BK deep; the
+//     global loads of slab t+1 are in flight while slab t is multiplied;
+//   * interior tiles with 16-byte aligned operands load whole 16-byte vectors
+//     along the contiguous dimension without bounds checks;
+//   * the 1-D grid is remapped so that each of the 8 XCDs (workgroup b runs on
+//     XCD b % 8) sweeps a contiguous run of tiles in column groups, keeping the
+//     operand rows its concurrent tiles share resident in that XCD's L2; a
+//     lower-triangular C (syrk) is enumerated directly, with no empty
+//     workgroups.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ddla {
+namespace kern {
+namespace gemm_detail {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+
+// Operand layout of both 16x16x4 instructions: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16].  The accumulator layouts differ (checked on gfx950).
+template <class T> struct Mfma;
+template <> struct Mfma<double> {
+    using acc_t = d4_t;
+    using vec_t = d2_t;  // 16 bytes
+    static constexpr int kVec = 2;
+    __device__ static acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // lane l, accumulator r holds D[l/16 + 4r][l%16]
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+    using acc_t = f4_t;
+    using vec_t = f4_t;  // 16 bytes
+    static constexpr int kVec = 4;
+    __device__ static acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // lane l, accumulator r holds D[4(l/16) + r][l%16]
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+enum { kTriNoneD = 0, kTriLowerD = 1, kTriUpperD = 2 };
+
+template <class T>
+struct GemmProblem {
+    int m, n, k;
+    T alpha, beta;
+    const T* A;
+    long long lda;
+    const T* B;
+    long long ldb;
+    T* C;
+    long long ldc;
+    int tri;
+    long long diag;
+    int vec_ab;  // A, B and their leading dimensions allow 16-byte loads along the contiguous dimension
+};
+
+// Tile schedule of one launch (see map_tile).
+struct TileOrder {
+    int ntm, ntn;    // tiles along m and n
+    int total;       // tiles to compute
+    int per_xcd;     // ceil(total / 8)
+    int group;       // tile columns per group
+    int lower_enum;  // enumerate only the lower triangle (ntm == ntn, square tiles, diag 0)
+};
+
+// Workgroup b -> tile (ti, tj).  Tile t = (b % 8) * per_xcd + b / 8, so XCD x
+// owns tiles [x * per_xcd, (x + 1) * per_xcd).  Tiles are ordered in groups of
+// `group` tile columns; within a group by tile row, then column.
+__device__ inline bool map_tile(const TileOrder& o, int b, int& ti, int& tj)
+{
+    const int t = (b & 7) * o.per_xcd + (b >> 3);
+    if (t >= o.total) return false;
+    const int G = o.group;
+    if (!o.lower_enum) {
+        const int per_group = G * o.ntm;
+        const int g = t / per_group;
+        const int j0 = g * G;
+        const int geff = min(G, o.ntn - j0);
+        const int r = t - g * per_group;
+        ti = r / geff;
+        tj = j0 + r % geff;
+        return true;
+    }
+    // Lower triangle, groups of G tile columns.  Group g (full) holds
+    // G (nt - gG) - G (G - 1) / 2 tiles; S(g) = tiles before group g.
+    const long long nt = o.ntm;
+    auto S = [&](long long g) { return g * (G * nt - (long long)G * (G - 1) / 2) - (long long)G * G * g * (g - 1) / 2; };
+    const int nfull = (int)(nt / G);
+    int g;
+    if (t >= S(nfull)) {
+        g = nfull;
+    } else {
+        int lo = 0, hi = nfull - 1;  // S(lo) <= t < S(hi + 1)
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S(mid) <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        g = lo;
+    }
+    const int j0 = g * G;
+    const int geff = min((long long)G, nt - j0);
+    const int o2 = t - (int)S(g);
+    const int tri = geff * (geff - 1) / 2;
+    if (o2 < tri) {
+        int r = (int)((sqrtf(8.0f * o2 + 1.0f) - 1.0f) * 0.5f);
+        while (r * (r + 1) / 2 > o2) --r;
+        while ((r + 1) * (r + 2) / 2 <= o2) ++r;
+        ti = j0 + r;
+        tj = j0 + (o2 - r * (r + 1) / 2);
+    } else {
+        const int q = o2 - tri;
+        ti = j0 + geff - 1 + q / geff;
+        tj = j0 + q % geff;
+    }
+    return true;
+}
+
+constexpr int kPad = 16;  // LDS row padding: the 4 k-rows of a fragment load fall on distinct banks
+
+template <int BM, int BN, int BK>
+constexpr int lds_elems()
+{
+    return 2 * BK * (BM + kPad) + 2 * BK * (BN + kPad);
+}
+
+// One BM x BN tile of C at (m0, n0).  FAST: the whole tile and every k slab
+// are in bounds and the operands allow 16-byte vector loads (decided once per
+// tile, so the k loop carries no branches between its loads).
+template <class T, int BM, int BN, int BK, bool TA, bool TB, bool FAST>
+__device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, int n0, T* smem)
+{
+    using Op = Mfma<T>;
+    using acc_t = typename Op::acc_t;
+    using vec_t = typename Op::vec_t;
+    constexpr int VW = Op::kVec;
+    constexpr int LA = BM + kPad, LB = BN + kPad;
+    constexpr int TM = BM / 32, TN = BN / 32;
+    constexpr int APT = BM * BK / 256, BPT = BN * BK / 256;  // scalars per thread per slab
+    constexpr int AVT = APT / VW, BVT = BPT / VW;            // vectors per thread per slab
+    static_assert(APT * 256 == BM * BK && BPT * 256 == BN * BK, "tile must split over 256 threads");
+    static_assert(AVT * VW == APT && BVT * VW == BPT, "slab must split into 16-byte vectors");
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    T* As = smem;
+    T* Bs = smem + 2 * BK * LA;
+
+    acc_t acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+    const int nkt = (p.alpha == T(0)) ? 0 : (p.k + BK - 1) / BK;
+
+    // Vector v of a slab: TA == false (op(A) = A, contiguous along m):
+    // (i, kk) = ((v % (BM/VW)) * VW, v / (BM/VW)); TA == true (contiguous along
+    // k): (i, kk) = (v / (BK/VW), (v % (BK/VW)) * VW).  Same for op(B).
+    T ra[AVT][VW], rb[BVT][VW];
+
+    auto a_coord = [&](int e, int& i, int& kk) {
+        const int v = tid + 256 * e;
+        if (!TA) { i = (v % (BM / VW)) * VW; kk = v / (BM / VW); }
+        else { kk = (v % (BK / VW)) * VW; i = v / (BK / VW); }
+    };
+    auto b_coord = [&](int e, int& j, int& kk) {
+        const int v = tid + 256 * e;
+        if (TB) { j = (v % (BN / VW)) * VW; kk = v / (BN / VW); }
+        else { kk = (v % (BK / VW)) * VW; j = v / (BK / VW); }
+    };
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < AVT; ++e) {
+            int i, kk;
+            a_coord(e, i, kk);
+            const long long gi = m0 + i, gk = k0 + kk;
+            const T* src = TA ? p.A + gk + gi * p.lda : p.A + gi + gk * p.lda;
+            if (FAST) {
+                const vec_t v = *reinterpret_cast<const vec_t*>(src);
+#pragma unroll
+                for (int q = 0; q < VW; ++q) ra[e][q] = v[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    const long long ii = TA ? gi : gi + q, kq = TA ? gk + q : gk;
+                    ra[e][q] = (ii < p.m && kq < p.k) ? (TA ? p.A[kq + ii * p.lda] : p.A[ii + kq * p.lda]) : T(0);
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < BVT; ++e) {
+            int j, kk;
+            b_coord(e, j, kk);
+            const long long gj = n0 + j, gk = k0 + kk;
+            const T* src = TB ? p.B + gj + gk * p.ldb : p.B + gk + gj * p.ldb;
+            if (FAST) {
+                const vec_t v = *reinterpret_cast<const vec_t*>(src);
+#pragma unroll
+                for (int q = 0; q < VW; ++q) rb[e][q] = v[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    const long long jj = TB ? gj + q : gj, kq = TB ? gk : gk + q;
+                    rb[e][q] = (jj < p.n && kq < p.k) ? (TB ? p.B[jj + kq * p.ldb] : p.B[kq + jj * p.ldb]) : T(0);
+                }
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        T* as = As + buf * BK * LA;
+        T* bs = Bs + buf * BK * LB;
+#pragma unroll
+        for (int e = 0; e < AVT; ++e) {
+            int i, kk;
+            a_coord(e, i, kk);
+            if (!TA) {
+                vec_t v;
+#pragma unroll
+                for (int q = 0; q < VW; ++q) v[q] = ra[e][q];
+                *reinterpret_cast<vec_t*>(as + kk * LA + i) = v;
+            } else {
+#pragma unroll
+                for (int q = 0; q < VW; ++q) as[(kk + q) * LA + i] = ra[e][q];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < BVT; ++e) {
+            int j, kk;
+            b_coord(e, j, kk);
+            if (TB) {
+                vec_t v;
+#pragma unroll
+                for (int q = 0; q < VW; ++q) v[q] = rb[e][q];
+                *reinterpret_cast<vec_t*>(bs + kk * LB + j) = v;
+            } else {
+#pragma unroll
+                for (int q = 0; q < VW; ++q) bs[(kk + q) * LB + j] = rb[e][q];
+            }
+        }
+    };
+
+    if (nkt > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nkt) load((kt + 1) * BK);
+        const T* as = As + buf * BK * LA + wm * (BM / 2) + (lane & 15);
+        const T* bs = Bs + buf * BK * LB + wn * (BN / 2) + (lane & 15);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            T a[TM], b[TN];
+#pragma unroll
+            for (int t = 0; t < TM; ++t) a[t] = as[kr * LA + t * 16];
+#pragma unroll
+            for (int t = 0; t < TN; ++t) b[t] = bs[kr * LB + t * 16];
+#pragma unroll
+            for (int x = 0; x < TM; ++x)
+#pragma unroll
+                for (int y = 0; y < TN; ++y) acc[x][y] = Op::mma(a[x], b[y], acc[x][y]);
+        }
+        if (kt + 1 < nkt) store(buf ^ 1);
+        __syncthreads();
+    }
+
+    const bool full = FAST || (m0 + BM <= p.m && n0 + BN <= p.n);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = m0 + wm * (BM / 2) + x * 16 + Op::row(lane, r);
+                const int j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
+                if (!full && (i >= p.m || j >= p.n)) continue;
+                if (p.tri == kTriLowerD && (long long)i + p.diag < j) continue;
+                if (p.tri == kTriUpperD && (long long)i + p.diag > j) continue;
+                T* c = p.C + i + (long long)j * p.ldc;
+                T v = p.alpha * acc[x][y][r];
+                if (p.beta != T(0)) v += p.beta * *c;
+                *c = v;
+            }
+}
+
+template <class T, int BM, int BN, int BK, bool TA, bool TB>
+__device__ __forceinline__ void gemm_tile(const GemmProblem<T>& p, int m0, int n0, T* smem)
+{
+    if (p.tri == kTriLowerD && (long long)(m0 + BM - 1) + p.diag < n0) return;
+    if (p.tri == kTriUpperD && (long long)m0 + p.diag > n0 + BN - 1) return;
+    const bool fast = p.vec_ab && m0 + BM <= p.m && n0 + BN <= p.n && p.k % BK == 0;
+    if (fast) gemm_tile_body<T, BM, BN, BK, TA, TB, true>(p, m0, n0, smem);
+    else gemm_tile_body<T, BM, BN, BK, TA, TB, false>(p, m0, n0, smem);
+}
+
+template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
+__global__ __launch_bounds__(256, WAVES_PER_EU) void gemm_kernel(GemmProblem<T> p, TileOrder o, const int* abort)
+{
+    if (abort != nullptr && *abort != 0) return;
+    int ti, tj;
+    if (!map_tile(o, blockIdx.x, ti, tj)) return;
+    __shared__ T smem[lds_elems<BM, BN, BK>()];
+    gemm_tile<T, BM, BN, BK, TA, TB>(p, ti * BM, tj * BN, smem);
+}
+
+template <class T>
+struct VBatch {
+    const int *m, *n, *k;
+    T alpha, beta;
+    const T* const* A;
+    const int* lda;
+    const T* const* B;
+    const int* ldb;
+    T* const* C;
+    const int* ldc;
+};
+
+template <class T, int BM, int BN, int BK, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_vbatched_kernel(VBatch<T> v, int offset)
+{
+    const int b = offset + blockIdx.z;
+    GemmProblem<T> p;
+    p.m = v.m[b];
+    p.n = v.n[b];
+    p.k = v.k[b];
+    if ((int)blockIdx.x * BM >= p.m || (int)blockIdx.y * BN >= p.n) return;
+    p.alpha = v.alpha;
+    p.beta = v.beta;
+    p.A = v.A[b];
+    p.lda = v.lda[b];
+    p.B = v.B[b];
+    p.ldb = v.ldb[b];
+    p.C = v.C[b];
+    p.ldc = v.ldc[b];
+    p.tri = kTriNoneD;
+    p.diag = 0;
+    p.vec_ab = 0;
+    __shared__ T smem[lds_elems<BM, BN, BK>()];
+    gemm_tile<T, BM, BN, BK, TA, TB>(p, blockIdx.x * BM, blockIdx.y * BN, smem);
+}
+
+} // namespace gemm_detail
+} // namespace kern
+} // namespace ddla

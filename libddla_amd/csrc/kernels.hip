@@ -6,8 +6,10 @@
 // LDS tiles.  The diagonal-block factorization and the triangular inverses run
 // in one workgroup on LDS-resident 64 x 64 blocks.
 #include "kernels.h"
+#include "gemm_tile.h"
 
 #include <cfloat>
+#include <cstdint>
 #include <climits>
 #include <stdexcept>
 #include <string>
@@ -23,218 +25,7 @@ inline void check_launch(const char* what)
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-typedef double d4_t __attribute__((ext_vector_type(4)));
-typedef float f4_t __attribute__((ext_vector_type(4)));
-
-// Operand layout of both 16x16x4 instructions: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16].  The accumulator layout differs between the two (checked
-// on gfx950): see row().
-template <class T> struct Mfma;
-template <> struct Mfma<double> {
-    using acc_t = d4_t;
-    __device__ static acc_t mma(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // lane l, accumulator r holds D[l/16 + 4r][l%16]
-    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
-template <> struct Mfma<float> {
-    using acc_t = f4_t;
-    __device__ static acc_t mma(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    // lane l, accumulator r holds D[4(l/16) + r][l%16]
-    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-
-template <class T>
-struct GemmProblem {
-    int m, n, k;
-    T alpha, beta;
-    const T* A;
-    long long lda;
-    const T* B;
-    long long ldb;
-    T* C;
-    long long ldc;
-    int tri;
-    long long diag;
-};
-
-constexpr int kPad = 16;  // LDS row padding (elements): keeps the 4 k-rows of a fragment load on distinct banks
-
-template <int BM, int BN, int BK>
-constexpr int gemm_lds_elems()
-{
-    return 2 * BK * (BM + kPad) + 2 * BK * (BN + kPad);
-}
-
-// One BM x BN tile of C at (m0, n0).  256 threads = 4 wavefronts in a 2 x 2
-// arrangement; each wavefront owns a (BM/2) x (BN/2) sub-tile made of
-// (BM/32) x (BN/32) MFMA tiles.  The k loop streams BK-deep slabs of op(A) and
-// op(B) through registers into two alternating LDS buffers (one barrier per
-// slab): the global loads of slab t+1 are in flight while slab t is multiplied.
-template <class T, int BM, int BN, int BK, bool TA, bool TB>
-__device__ __forceinline__ void gemm_tile(const GemmProblem<T>& p, int m0, int n0, T* smem)
-{
-    constexpr int LA = BM + kPad, LB = BN + kPad;
-    constexpr int TM = BM / 32, TN = BN / 32;
-    constexpr int APT = BM * BK / 256, BPT = BN * BK / 256;
-    static_assert(APT * 256 == BM * BK && BPT * 256 == BN * BK, "tile must split over 256 threads");
-    using Op = Mfma<T>;
-    using acc_t = typename Op::acc_t;
-
-    if (p.tri == kTriLower && (long long)(m0 + BM - 1) + p.diag < n0) return;
-    if (p.tri == kTriUpper && (long long)m0 + p.diag > n0 + BN - 1) return;
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    T* As = smem;
-    T* Bs = smem + 2 * BK * LA;
-
-    acc_t acc[TM][TN];
-#pragma unroll
-    for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-
-    const int nkt = (p.alpha == T(0)) ? 0 : (p.k + BK - 1) / BK;
-    T ra[APT], rb[BPT];
-
-    auto a_index = [&](int e, int& i, int& kk) {
-        const int lin = tid + 256 * e;
-        if (!TA) { i = lin % BM; kk = lin / BM; } else { kk = lin % BK; i = lin / BK; }
-    };
-    auto b_index = [&](int e, int& j, int& kk) {
-        const int lin = tid + 256 * e;
-        if (!TB) { kk = lin % BK; j = lin / BK; } else { j = lin % BN; kk = lin / BN; }
-    };
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int e = 0; e < APT; ++e) {
-            int i, kk;
-            a_index(e, i, kk);
-            const int gi = m0 + i, gk = k0 + kk;
-            ra[e] = (gi < p.m && gk < p.k)
-                        ? (TA ? p.A[gk + (long long)gi * p.lda] : p.A[gi + (long long)gk * p.lda])
-                        : T(0);
-        }
-#pragma unroll
-        for (int e = 0; e < BPT; ++e) {
-            int j, kk;
-            b_index(e, j, kk);
-            const int gj = n0 + j, gk = k0 + kk;
-            rb[e] = (gj < p.n && gk < p.k)
-                        ? (TB ? p.B[gj + (long long)gk * p.ldb] : p.B[gk + (long long)gj * p.ldb])
-                        : T(0);
-        }
-    };
-    auto store = [&](int buf) {
-        T* as = As + buf * BK * LA;
-        T* bs = Bs + buf * BK * LB;
-#pragma unroll
-        for (int e = 0; e < APT; ++e) {
-            int i, kk;
-            a_index(e, i, kk);
-            as[kk * LA + i] = ra[e];
-        }
-#pragma unroll
-        for (int e = 0; e < BPT; ++e) {
-            int j, kk;
-            b_index(e, j, kk);
-            bs[kk * LB + j] = rb[e];
-        }
-    };
-
-    if (nkt > 0) {
-        load(0);
-        store(0);
-    }
-    __syncthreads();
-    for (int kt = 0; kt < nkt; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nkt) load((kt + 1) * BK);
-        const T* as = As + buf * BK * LA + wm * (BM / 2) + (lane & 15);
-        const T* bs = Bs + buf * BK * LB + wn * (BN / 2) + (lane & 15);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            T a[TM], b[TN];
-#pragma unroll
-            for (int t = 0; t < TM; ++t) a[t] = as[kr * LA + t * 16];
-#pragma unroll
-            for (int t = 0; t < TN; ++t) b[t] = bs[kr * LB + t * 16];
-#pragma unroll
-            for (int x = 0; x < TM; ++x)
-#pragma unroll
-                for (int y = 0; y < TN; ++y) acc[x][y] = Op::mma(a[x], b[y], acc[x][y]);
-        }
-        if (kt + 1 < nkt) store(buf ^ 1);
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = m0 + wm * (BM / 2) + x * 16 + Op::row(lane, r);
-                const int j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
-                if (i >= p.m || j >= p.n) continue;
-                if (p.tri == kTriLower && (long long)i + p.diag < j) continue;
-                if (p.tri == kTriUpper && (long long)i + p.diag > j) continue;
-                T* c = p.C + i + (long long)j * p.ldc;
-                T v = p.alpha * acc[x][y][r];
-                if (p.beta != T(0)) v += p.beta * *c;
-                *c = v;
-            }
-}
-
-template <class T, int BM, int BN, int BK, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmProblem<T> p, const int* abort)
-{
-    if (abort != nullptr && *abort != 0) return;
-    __shared__ T smem[gemm_lds_elems<BM, BN, BK>()];
-    gemm_tile<T, BM, BN, BK, TA, TB>(p, blockIdx.x * BM, blockIdx.y * BN, smem);
-}
-
-template <class T>
-struct VBatch {
-    const int *m, *n, *k;
-    T alpha, beta;
-    const T* const* A;
-    const int* lda;
-    const T* const* B;
-    const int* ldb;
-    T* const* C;
-    const int* ldc;
-};
-
-template <class T, int BM, int BN, int BK, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_vbatched_kernel(VBatch<T> v, int offset)
-{
-    const int b = offset + blockIdx.z;
-    GemmProblem<T> p;
-    p.m = v.m[b];
-    p.n = v.n[b];
-    p.k = v.k[b];
-    if ((int)blockIdx.x * BM >= p.m || (int)blockIdx.y * BN >= p.n) return;
-    p.alpha = v.alpha;
-    p.beta = v.beta;
-    p.A = v.A[b];
-    p.lda = v.lda[b];
-    p.B = v.B[b];
-    p.ldb = v.ldb[b];
-    p.C = v.C[b];
-    p.ldc = v.ldc[b];
-    p.tri = kTriNone;
-    p.diag = 0;
-    __shared__ T smem[gemm_lds_elems<BM, BN, BK>()];
-    gemm_tile<T, BM, BN, BK, TA, TB>(p, blockIdx.x * BM, blockIdx.y * BN, smem);
-}
+using namespace gemm_detail;
 
 __global__ __launch_bounds__(256) void vbatched_validate_kernel(int na, int nb, const int* m, const int* n,
                                                                 const int* k, const int* lda, const int* ldb,
@@ -276,46 +67,105 @@ __global__ __launch_bounds__(256) void vbatched_validate_kernel(int na, int nb, 
 // 64 x 64 diagonal blocks: factorization and triangular inverse in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kIB = 64;
-constexpr int kLd = kIB + 1;
 
-// Column c of X = L^-1 by forward substitution, one thread per column.
-template <class T>
-__device__ void lower_inverse_lds(const T* Ls, T* Xs)
+// One 64 x 64 diagonal block in one 256-thread workgroup.  Thread t owns row
+// i = t / 4 and the 16 columns c = t % 4 + 4q (q < 16) of both the factor L
+// and of X = L^-1, in registers.  Before step j, column j of L and row j of X
+// are published in LDS (two alternating buffers, one barrier per step); at
+// step j every thread reads the published values it needs into registers and
+// applies, with FACTOR, the right-looking Cholesky step (pivot sqrt(L(j,j)),
+// column scaling, rank-1 update of the trailing lower triangle) and, in both
+// cases, the matching step of the forward elimination L X = I (row j of X
+// divided by L(j,j), then subtracted from the rows below); the owners of
+// column j+1 and of row j+1 then publish them.  Rows and columns at or beyond
+// the block order w carry the identity and are neither factored nor written
+// back.  Returns 0, or j + 1 for the first pivot that is not positive (or is
+// NaN), in which case X is not written.
+template <class T, bool FACTOR>
+__device__ int factor_invert_block(bool upper, int w, T* A, long long lda, T* inv, T (*col)[kIB], T (*xrow)[kIB])
 {
-    const int c = threadIdx.x;
-    if (c < kIB) {
-        for (int i = 0; i < c; ++i) Xs[c * kLd + i] = T(0);
-        Xs[c * kLd + c] = T(1) / Ls[c + c * kLd];
-        for (int i = c + 1; i < kIB; ++i) {
-            T s = T(0);
-            for (int q = c; q < i; ++q) s += Ls[i + q * kLd] * Xs[c * kLd + q];
-            Xs[c * kLd + i] = -s / Ls[i + i * kLd];
-        }
-    }
-}
-
-template <class T>
-__device__ void load_lower_block(bool upper, int w, const T* A, long long lda, T* Ls)
-{
-    for (int idx = threadIdx.x; idx < kIB * kIB; idx += blockDim.x) {
-        const int i = idx & (kIB - 1), j = idx >> 6;
+    const int tid = threadIdx.x, i = tid >> 2, c0 = tid & 3;
+    T Lr[16], Xr[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = c0 + 4 * q;
         T v = T(0);
-        if (i < w && j < w) {
-            if (i >= j) v = upper ? A[j + (long long)i * lda] : A[i + (long long)j * lda];
-        } else if (i == j) {
+        if (i < w && c < w) {
+            if (c <= i) v = upper ? A[c + (long long)i * lda] : A[i + (long long)c * lda];
+        } else if (i == c) {
             v = T(1);
         }
-        Ls[i + j * kLd] = v;
+        Lr[q] = v;
+        Xr[q] = (i == c) ? T(1) : T(0);
     }
-}
-
-template <class T>
-__device__ void store_inverse(const T* Xs, T* inv)
-{
-    for (int idx = threadIdx.x; idx < kIB * kIB; idx += blockDim.x) {
-        const int i = idx & (kIB - 1), j = idx >> 6;
-        inv[idx] = Xs[j * kLd + i];  // inv[i + 64 j] = X(i, j)
+    // publish column 0 and row 0
+    if (c0 == 0) col[0][i] = Lr[0];
+    if (i == 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xrow[0][c0 + 4 * q] = Xr[q];
     }
+    __syncthreads();
+    int fail = 0;
+    for (int j = 0; j < w; ++j) {
+        const int cur = j & 1, nxt = cur ^ 1;
+        T cv[16], xv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            cv[q] = col[cur][c0 + 4 * q];
+            xv[q] = xrow[cur][c0 + 4 * q];
+        }
+        const T d = col[cur][j];
+        const T ci = col[cur][i];
+        if (FACTOR && !(d > T(0))) {
+            fail = j + 1;
+            break;
+        }
+        const T ljj = FACTOR ? T(sqrt(d)) : d;
+        const T rinv = T(1) / ljj;
+        const T li = (i > j) ? (FACTOR ? ci * rinv : ci) : T(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = c0 + 4 * q;
+            if (FACTOR) {
+                const T upd = Lr[q] - li * (cv[q] * rinv);
+                const T lo = (c == j) ? li : ((c > j && c <= i) ? upd : Lr[q]);
+                const T dg = (c == j) ? ljj : Lr[q];
+                Lr[q] = (i > j) ? lo : ((i == j) ? dg : Lr[q]);
+            }
+            const T xu = Xr[q] - li * (xv[q] * rinv);
+            const T xs = Xr[q] * rinv;
+            Xr[q] = (c <= j) ? ((i > j) ? xu : ((i == j) ? xs : Xr[q])) : Xr[q];
+        }
+        // publish column j + 1 of L and row j + 1 of X for the next step
+        const int j1 = j + 1;
+        if (c0 == (j1 & 3) && i >= j1) {
+            T v = T(0);
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (q == (j1 >> 2)) v = Lr[q];
+            col[nxt][i] = v;
+        }
+        if (i == j1) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) xrow[nxt][c0 + 4 * q] = Xr[q];
+        }
+        __syncthreads();
+    }
+    if (FACTOR) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = c0 + 4 * q;
+            if (i < w && c < w && c <= i) {
+                if (upper) A[c + (long long)i * lda] = Lr[q];
+                else A[i + (long long)c * lda] = Lr[q];
+            }
+        }
+    }
+    if (fail == 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) inv[i + 64 * (c0 + 4 * q)] = Xr[q];
+    }
+    return fail;
 }
 
 template <class T>
@@ -323,59 +173,19 @@ __global__ __launch_bounds__(256) void potf2_inv_kernel(bool upper, int w, T* A,
                                                         int info_base)
 {
     if (*info != 0) return;
-    __shared__ T Ls[kIB * kLd];
-    __shared__ T Xs[kIB * kLd];
-    __shared__ int fail;
-    const int tid = threadIdx.x;
-    load_lower_block(upper, w, A, lda, Ls);
-    if (tid == 0) fail = 0;
-    __syncthreads();
-    for (int j = 0; j < w; ++j) {
-        if (tid == 0) {
-            const T d = Ls[j + j * kLd];
-            if (!(d > T(0))) fail = j + 1;
-            else Ls[j + j * kLd] = sqrt(d);
-        }
-        __syncthreads();
-        if (fail != 0) break;
-        const T rinv = T(1) / Ls[j + j * kLd];
-        for (int i = j + 1 + tid; i < w; i += 256) Ls[i + j * kLd] *= rinv;
-        __syncthreads();
-        const int r = w - j - 1;
-        for (int idx = tid; idx < r * r; idx += 256) {
-            const int c = j + 1 + idx / r, i = j + 1 + idx % r;
-            if (i >= c) Ls[i + c * kLd] -= Ls[i + j * kLd] * Ls[c + j * kLd];
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < kIB * kIB; idx += 256) {
-        const int i = idx & (kIB - 1), j = idx >> 6;
-        if (i < w && j < w && i >= j) {
-            if (upper) A[j + (long long)i * lda] = Ls[i + j * kLd];
-            else A[i + (long long)j * lda] = Ls[i + j * kLd];
-        }
-    }
-    if (fail != 0) {
-        if (tid == 0) *info = info_base + fail;
-        return;
-    }
-    lower_inverse_lds(Ls, Xs);
-    __syncthreads();
-    store_inverse(Xs, inv);
+    __shared__ T col[2][kIB], xrow[2][kIB];
+    const int fail = factor_invert_block<T, true>(upper, w, A, lda, inv, col, xrow);
+    if (fail != 0 && threadIdx.x == 0) *info = info_base + fail;
 }
 
 template <class T>
 __global__ __launch_bounds__(256) void trtri_kernel(bool upper, int n, const T* A, long long lda, T* inv)
 {
-    __shared__ T Ls[kIB * kLd];
-    __shared__ T Xs[kIB * kLd];
+    __shared__ T col[2][kIB], xrow[2][kIB];
     const long long j0 = (long long)blockIdx.x * kIB;
     const int w = (int)min((long long)kIB, n - j0);
-    load_lower_block(upper, w, A + j0 + j0 * lda, lda, Ls);
-    __syncthreads();
-    lower_inverse_lds(Ls, Xs);
-    __syncthreads();
-    store_inverse(Xs, inv + (long long)blockIdx.x * kIB * kIB);
+    factor_invert_block<T, false>(upper, w, const_cast<T*>(A) + j0 + j0 * lda, lda,
+                                  inv + (long long)blockIdx.x * kIB * kIB, col, xrow);
 }
 
 // ---------------------------------------------------------------------------
@@ -512,16 +322,24 @@ int grid_for(long long work, int per_block, int cap = 65536)
 
 bool is_n(char op) { return op == 'N' || op == 'n'; }
 
-template <class T, int BM, int BN, int BK>
+// Builds the tile schedule of one launch (see map_tile) and launches it.
+template <class T, int BM, int BN, int BK, int WPE>
 void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, const int* abort)
 {
-    const long long gx = (p.m + BM - 1) / BM, gy = (p.n + BN - 1) / BN;
-    if (gy > 65535 || gx > INT_MAX) throw std::invalid_argument("gemm: problem too large for one launch");
-    const dim3 grid((unsigned)gx, (unsigned)gy), block(256);
-    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false><<<grid, block, 0, s>>>(p, abort);
-    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true><<<grid, block, 0, s>>>(p, abort);
-    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false><<<grid, block, 0, s>>>(p, abort);
-    else gemm_kernel<T, BM, BN, BK, true, true><<<grid, block, 0, s>>>(p, abort);
+    TileOrder o;
+    o.ntm = (p.m + BM - 1) / BM;
+    o.ntn = (p.n + BN - 1) / BN;
+    o.lower_enum = (p.tri == kTriLower && p.diag == 0 && p.m == p.n && BM == BN) ? 1 : 0;
+    const long long total = o.lower_enum ? (long long)o.ntm * (o.ntm + 1) / 2 : (long long)o.ntm * o.ntn;
+    if (total > (long long)INT_MAX / 2) throw std::invalid_argument("gemm: problem too large for one launch");
+    o.total = (int)total;
+    o.group = 8;
+    o.per_xcd = (o.total + 7) / 8;
+    const dim3 grid((unsigned)(o.per_xcd * 8)), block(256);
+    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    else gemm_kernel<T, BM, BN, BK, true, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
     check_launch("gemm");
 }
 
@@ -548,10 +366,12 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
           long long ldb, T beta, T* C, long long ldc, Tri tri, long long diag, const int* abort)
 {
     if (m <= 0 || n <= 0) return;
-    GemmProblem<T> p{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, (int)tri, diag};
+    constexpr int VW = 16 / sizeof(T);
+    const int vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % VW == 0 && ldb % VW == 0;
+    GemmProblem<T> p{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, (int)tri, diag, vec};
     const bool a_t = !is_n(ta), b_t = !is_n(tb);
-    if (m >= 128 && n >= 128) launch_gemm_cfg<T, 128, 128, 16>(s, a_t, b_t, p, abort);
-    else launch_gemm_cfg<T, 64, 64, 16>(s, a_t, b_t, p, abort);
+    if (m >= 128 && n >= 128) launch_gemm_cfg<T, 128, 128, 16, 2>(s, a_t, b_t, p, abort);
+    else launch_gemm_cfg<T, 64, 64, 16, 2>(s, a_t, b_t, p, abort);
 }
 
 template <class T>
