@@ -50,115 +50,107 @@ struct LView {
 // C_L(ci.., cj..) (M x N) := alpha * A_L(ai.., ak..) * B_L(bj.., bk..)^T + beta * C_L,
 // writing only the lower triangle of the global L view when mask is set.
 template <class T>
-void update_lower(DdlaStream* h, int cls, const LView<T>& v, int M, int N, int K, T alpha, long long ai,
-                  long long ak, long long bj, long long bk, T beta, long long ci, long long cj, bool mask,
-                  const int* abort, double flops)
+void update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& v, int M, int N, int K, T alpha,
+                  long long ai, long long ak, long long bj, long long bk, T beta, long long ci, long long cj,
+                  bool mask, const int* abort, double flops)
 {
     if (M <= 0 || N <= 0) return;
-    LaunchTimer timer(h, cls, flops);
+    LaunchTimer timer(h, cls, flops, 0, st);
     if (!v.upper) {
-        kern::gemm<T>(h->stream, 'N', 'T', M, N, K, alpha, v.at(ai, ak), v.lda, v.at(bj, bk), v.lda, beta,
-                      v.at(ci, cj), v.lda, mask ? kTriLower : kTriNone, ci - cj, abort);
+        kern::gemm<T>(st, 'N', 'T', M, N, K, alpha, v.at(ai, ak), v.lda, v.at(bj, bk), v.lda, beta, v.at(ci, cj),
+                      v.lda, mask ? kTriLower : kTriNone, ci - cj, abort);
     } else {
         // Stored block at (cj, ci) is C_L^T = B_L * A_L^T.
-        kern::gemm<T>(h->stream, 'T', 'N', N, M, K, alpha, v.at(bk, bj), v.lda, v.at(ak, ai), v.lda, beta,
-                      v.at(cj, ci), v.lda, mask ? kTriUpper : kTriNone, cj - ci, abort);
+        kern::gemm<T>(st, 'T', 'N', N, M, K, alpha, v.at(bk, bj), v.lda, v.at(ak, ai), v.lda, beta, v.at(cj, ci),
+                      v.lda, mask ? kTriUpper : kTriNone, cj - ci, abort);
     }
 }
 
 // X_L(ri.., ci..) (M x w) := X_L * inv^T, inv the 64 x 64 inverse of the
 // diagonal block (in place: one workgroup covers the w columns).
 template <class T>
-void panel_trsm(DdlaStream* h, const LView<T>& v, int M, int w, long long ri, long long ci, const T* inv,
-                const int* abort)
+void panel_trsm(DdlaStream* h, hipStream_t st, const LView<T>& v, int M, int w, long long ri, long long ci,
+                const T* inv, const int* abort)
 {
     if (M <= 0) return;
-    LaunchTimer timer(h, kClassPanelTrsm, 2.0 * M * w * w);
+    LaunchTimer timer(h, kClassPanelTrsm, 2.0 * M * w * w, 0, st);
     if (!v.upper)
-        kern::gemm<T>(h->stream, 'N', 'T', M, w, w, T(1), v.at(ri, ci), v.lda, inv, kIB, T(0), v.at(ri, ci), v.lda,
+        kern::gemm<T>(st, 'N', 'T', M, w, w, T(1), v.at(ri, ci), v.lda, inv, kIB, T(0), v.at(ri, ci), v.lda,
                       kTriNone, 0, abort);
     else
-        kern::gemm<T>(h->stream, 'N', 'N', w, M, w, T(1), inv, kIB, v.at(ci, ri), v.lda, T(0), v.at(ci, ri), v.lda,
+        kern::gemm<T>(st, 'N', 'N', w, M, w, T(1), inv, kIB, v.at(ci, ri), v.lda, T(0), v.at(ci, ri), v.lda,
                       kTriNone, 0, abort);
 }
 
-// Blocked right-looking Cholesky of the order-n L view; *info (device) must
-// be zero on entry and receives the 1-based index of the first failed pivot.
+// Factors the panel of columns [k, k + kb) of the order-n L view (its
+// diagonal block and everything below it), assuming every earlier trailing
+// update has been applied to these columns: for each 64-column block, potf2 +
+// inverse, the triangular solve of the rows below, and the update of the
+// panel's remaining columns.
+template <class T>
+void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k, int kb, int* info, T* inv)
+{
+    for (int c = 0; c < kb; c += kIB) {
+        const int w = std::min(kIB, kb - c);
+        const long long r0 = (long long)k + c;
+        {
+            LaunchTimer timer(h, kClassPotf2, (double)w * w * w / 3.0, 0, st);
+            kern::potf2_inv<T>(st, v.upper, w, v.at(r0, r0), v.lda, inv, info, (int)r0);
+        }
+        const int below = n - (int)r0 - w;
+        if (below <= 0) continue;
+        panel_trsm<T>(h, st, v, below, w, r0 + w, r0, inv, info);
+        const int ncols = k + kb - (int)r0 - w;
+        if (ncols > 0)
+            update_lower<T>(h, st, kClassPanelUpdate, v, below, ncols, w, T(-1), r0 + w, r0, r0 + w, r0, T(1), r0 + w,
+                            r0 + w, true, info, 2.0 * below * ncols * w);
+    }
+}
+
+// Blocked right-looking Cholesky of the order-n L view with look-ahead; *info
+// (device) must be zero on entry and receives the 1-based index of the first
+// failed pivot.  After panel k is factored, the trailing update is split: the
+// columns of panel k+1 are updated first on the handle's stream, panel k+1 is
+// then factored on the side stream while the rest of the trailing matrix is
+// updated on the handle's stream (the two touch disjoint columns).  Returns
+// with all work ordered on the handle's stream.
 template <class T>
 void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, T* inv)
 {
     const LView<T> v{A, lda, upper};
     const int NB = outer_block<T>();
+    hipStream_t main = h->stream, side = h->side;
+    factor_panel<T>(h, main, v, n, 0, std::min(NB, n), info, inv);
     for (int k = 0; k < n; k += NB) {
         const int kb = std::min(NB, n - k);
-        for (int c = 0; c < kb; c += kIB) {
-            const int w = std::min(kIB, kb - c);
-            const long long r0 = (long long)k + c;
-            {
-                LaunchTimer timer(h, kClassPotf2, (double)w * w * w / 3.0);
-                kern::potf2_inv<T>(h->stream, upper, w, v.at(r0, r0), lda, inv, info, (int)r0);
-            }
-            const int below = n - (int)r0 - w;
-            if (below <= 0) continue;
-            panel_trsm<T>(h, v, below, w, r0 + w, r0, inv, info);
-            const int ncols = k + kb - (int)r0 - w;
-            if (ncols > 0)
-                update_lower<T>(h, kClassPanelUpdate, v, below, ncols, w, T(-1), r0 + w, r0, r0 + w, r0, T(1),
-                                r0 + w, r0 + w, true, info, 2.0 * below * ncols * w);
-        }
         const int m = n - k - kb;
-        if (m > 0)
-            update_lower<T>(h, kClassTrailingUpdate, v, m, m, kb, T(-1), k + kb, k, k + kb, k, T(1), k + kb, k + kb,
-                            true, info, (double)m * (m + 1) * kb);
+        if (m <= 0) break;
+        const int kb2 = std::min(NB, m);  // width of the next panel
+        const long long r = (long long)k + kb;
+        // next panel's columns: rows r.., columns [r, r + kb2)
+        update_lower<T>(h, main, kClassTrailingUpdate, v, m, kb2, kb, T(-1), r, k, r, k, T(1), r, r, true, info,
+                        (double)kb2 * (kb2 + 1) / 2 * 2 * kb + 2.0 * (m - kb2) * kb2 * kb);
+        DDLA_HIP(hipEventRecord(h->ev_update, main));
+        DDLA_HIP(hipStreamWaitEvent(side, h->ev_update, 0));
+        factor_panel<T>(h, side, v, n, (int)r, kb2, info, inv);
+        DDLA_HIP(hipEventRecord(h->ev_panel, side));
+        // the rest of the trailing matrix: rows and columns [r + kb2, n)
+        const int mr = m - kb2;
+        if (mr > 0)
+            update_lower<T>(h, main, kClassTrailingUpdate, v, mr, mr, kb, T(-1), r + kb2, k, r + kb2, k, T(1),
+                            r + kb2, r + kb2, true, info, (double)mr * (mr + 1) * kb);
+        DDLA_HIP(hipStreamWaitEvent(main, h->ev_panel, 0));
     }
 }
 
-// Left solve A X = B with A = L L^T given by its L view; B is n x nrhs.
+// Left solve A X = B with A = L L^T given by its L view; B is n x nrhs: the
+// forward and the backward sweep, one launch each.
 template <class T>
 void solve_left(DdlaStream* h, bool upper, int n, int nrhs, const T* F, long long ldf, T* B, long long ldb,
-                const T* dinv)
+                const T* dinv, void* sync, int* error)
 {
-    const LView<T> v{const_cast<T*>(F), ldf, upper};
-    const int nblk = (n + kIB - 1) / kIB;
-    // L Y = B
-    for (int jb = 0; jb < nblk; ++jb) {
-        const long long j0 = (long long)jb * kIB;
-        const int w = (int)std::min<long long>(kIB, n - j0);
-        const T* d = dinv + (long long)jb * kIB * kIB;
-        {
-            LaunchTimer timer(h, kClassSolveDiag, 2.0 * w * w * nrhs);
-            kern::gemm<T>(h->stream, 'N', 'N', w, nrhs, w, T(1), d, kIB, B + j0, ldb, T(0), B + j0, ldb);
-        }
-        const int below = n - (int)j0 - w;
-        if (below > 0) {
-            LaunchTimer timer(h, kClassSolveUpdate, 2.0 * below * w * nrhs, 8.0 * below * w);
-            if (!upper)
-                kern::gemm<T>(h->stream, 'N', 'N', below, nrhs, w, T(-1), v.at(j0 + w, j0), ldf, B + j0, ldb, T(1),
-                              B + j0 + w, ldb);
-            else
-                kern::gemm<T>(h->stream, 'T', 'N', below, nrhs, w, T(-1), v.at(j0, j0 + w), ldf, B + j0, ldb, T(1),
-                              B + j0 + w, ldb);
-        }
-    }
-    // L^T X = Y
-    for (int jb = nblk - 1; jb >= 0; --jb) {
-        const long long j0 = (long long)jb * kIB;
-        const int w = (int)std::min<long long>(kIB, n - j0);
-        const T* d = dinv + (long long)jb * kIB * kIB;
-        {
-            LaunchTimer timer(h, kClassSolveDiag, 2.0 * w * w * nrhs);
-            kern::gemm<T>(h->stream, 'T', 'N', w, nrhs, w, T(1), d, kIB, B + j0, ldb, T(0), B + j0, ldb);
-        }
-        if (j0 > 0) {
-            LaunchTimer timer(h, kClassSolveUpdate, 2.0 * j0 * w * nrhs, 8.0 * j0 * w);
-            if (!upper)
-                kern::gemm<T>(h->stream, 'T', 'N', (int)j0, nrhs, w, T(-1), v.at(j0, 0), ldf, B + j0, ldb, T(1), B,
-                              ldb);
-            else
-                kern::gemm<T>(h->stream, 'N', 'N', (int)j0, nrhs, w, T(-1), v.at(0, j0), ldf, B + j0, ldb, T(1), B,
-                              ldb);
-        }
-    }
+    LaunchTimer timer(h, kClassSolveUpdate, 2.0 * n * n * nrhs, (double)n * n * sizeof(T));
+    kern::potrs_sweeps<T>(h->stream, upper, n, nrhs, F, ldf, dinv, B, ldb, sync, error);
 }
 
 std::size_t diag_inverse_elems(int n) { return (std::size_t)((n + kIB - 1) / kIB) * kIB * kIB; }
@@ -238,8 +230,9 @@ void ppotrs_workspace_size(const DdlaHandle_t&, char side, int n, int nrhs, cons
 {
     *backend_bytes = *host_bytes = 0;
     if (n == 0 || nrhs == 0) return;
-    // [ inverses of the 64 x 64 diagonal blocks | transposed B (side 'R') ]
-    *backend_bytes = detail::padded_bytes<T>(diag_inverse_elems(n));
+    // [ inverses of the 64 x 64 diagonal blocks | sweep counters | error word | transposed B (side 'R') ]
+    *backend_bytes = detail::padded_bytes<T>(diag_inverse_elems(n)) + kern::solve_work_bytes(nrhs) +
+                     detail::padded_bytes<int>(1);
     if (side == 'R') *backend_bytes += detail::padded_bytes<T>((std::size_t)n * nrhs);
 }
 
@@ -255,20 +248,28 @@ void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char, int n, int nr
     const long long lda = da[DDLA_LLD_], ldb = db[DDLA_LLD_];
     char* w = static_cast<char*>(backend_work);
     T* dinv = detail::take<T>(w, diag_inverse_elems(n));
+    void* sync = w;
+    w += kern::solve_work_bytes(nrhs);
+    int* error = detail::take<int>(w, 1);
+    fill_int(h->stream, error, 0);
     {
         LaunchTimer timer(h, kClassTrtri, (double)n * kIB * kIB / 3.0);
         kern::trtri_diag_blocks<T>(h->stream, upper, n, A, lda, dinv);
     }
-    // The factor is symmetric-positive-definite's: trans 'N' and 'C' solve the
-    // same system.  side 'R' (X A = B, B nrhs x n) is A X^T = B^T.
+    // A = L L^T is symmetric: trans 'N' and 'C' solve the same system.
+    // side 'R' (X A = B, B nrhs x n) is A X^T = B^T.
     if (side == 'L') {
-        solve_left<T>(h, upper, n, nrhs, A, lda, B, ldb, dinv);
+        solve_left<T>(h, upper, n, nrhs, A, lda, B, ldb, dinv, sync, error);
     } else {
         T* bt = detail::take<T>(w, (std::size_t)n * nrhs);
         kern::transpose<T>(h->stream, nrhs, n, B, ldb, bt, n);
-        solve_left<T>(h, upper, n, nrhs, A, lda, bt, n, dinv);
+        solve_left<T>(h, upper, n, nrhs, A, lda, bt, n, dinv, sync, error);
         kern::transpose<T>(h->stream, n, nrhs, bt, n, B, ldb);
     }
+    int host_error = 0;
+    DDLA_HIP(hipMemcpyAsync(&host_error, error, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    DDLA_HIP(hipStreamSynchronize(h->stream));
+    if (host_error != 0) throw std::runtime_error("ppotrs: a triangular-solve sweep exceeded its wait limit");
 }
 
 void pposv_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A, int ia,

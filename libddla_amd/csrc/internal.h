@@ -39,6 +39,10 @@ public:
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
+    // Side stream of the look-ahead factorization (non-blocking, default
+    // priority) and the two events that order it against `stream`.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_update = nullptr, ev_panel = nullptr;
     int nprows_ = 0, npcols_ = 0, myprow_ = 0, mypcol_ = 0, nprocs = 0;
     char major = 'R';
 
@@ -65,7 +69,7 @@ public:
 // Brackets one kernel launch with events when the handle profiles.
 class LaunchTimer {
 public:
-    LaunchTimer(DdlaStream* h, int cls, double flops, double bytes = 0);
+    LaunchTimer(DdlaStream* h, int cls, double flops, double bytes = 0, hipStream_t s = nullptr);
     ~LaunchTimer();
     LaunchTimer(const LaunchTimer&) = delete;
     LaunchTimer& operator=(const LaunchTimer&) = delete;
@@ -74,6 +78,7 @@ private:
     DdlaStream* h_;
     int cls_;
     double flops_, bytes_;
+    hipStream_t s_;
     hipEvent_t start_ = nullptr;
 };
 

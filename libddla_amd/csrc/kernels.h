@@ -50,6 +50,17 @@ void potf2_inv(hipStream_t s, bool upper, int w, T* A, long long lda, T* inv, in
 template <class T>
 void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv);
 
+// Forward (L Y = B) then backward (L^T X = Y) substitution with the order-n
+// factor (L view as for potf2_inv) and its inverted 64 x 64 diagonal blocks
+// dinv (from trtri_diag_blocks), in place on B (n x nrhs), one launch per
+// sweep.  work must hold solve_work_bytes(nrhs) bytes; *error (device) is set
+// to 1 if a sweep's internal wait ever exceeded its limit (the result is then
+// not valid).
+size_t solve_work_bytes(int nrhs);
+template <class T>
+void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long long ldl, const T* dinv, T* B,
+                  long long ldb, void* work, int* error);
+
 // diag_logdet reduction over A[i * (lda + 1)], i < n: work must hold
 // logdet_work_bytes(); out (device) receives {sum log|d|, flags} where flags
 // = 1 nonfinite | 2 zero | 4 odd number of negative entries.

@@ -7,6 +7,7 @@
 // in one workgroup on LDS-resident 64 x 64 blocks.
 #include "kernels.h"
 #include "gemm_tile.h"
+#include "solve_tile.h"
 
 #include <cfloat>
 #include <cstdint>
@@ -407,6 +408,32 @@ void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long l
     check_launch("trtri_diag_blocks");
 }
 
+size_t solve_work_bytes(int nrhs)
+{
+    const size_t slabs = (size_t)(nrhs + solve_detail::kSB - 1) / solve_detail::kSB;
+    return (2 * slabs * sizeof(int) + 255) / 256 * 256;
+}
+
+template <class T>
+void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long long ldl, const T* dinv, T* B,
+                  long long ldb, void* work, int* error)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    using namespace solve_detail;
+    const int nblk = (n + kSB - 1) / kSB, slabs = (nrhs + kSB - 1) / kSB;
+    if (slabs > 65535) throw std::invalid_argument("potrs: too many right-hand sides for one launch");
+    SolveArgs<T> a{n, nrhs, nblk, L, ldl, upper ? 1 : 0, dinv, B, ldb, static_cast<int*>(work), error, 1 << 23};
+    const dim3 grid((unsigned)nblk, (unsigned)slabs), block(256);
+    zero_ints_kernel<<<1, 256, 0, s>>>(a.sync, 2 * slabs);
+    check_launch("potrs");
+    solve_sweep_kernel<T, false><<<grid, block, 0, s>>>(a);
+    check_launch("potrs");
+    zero_ints_kernel<<<1, 256, 0, s>>>(a.sync, 2 * slabs);
+    check_launch("potrs");
+    solve_sweep_kernel<T, true><<<grid, block, 0, s>>>(a);
+    check_launch("potrs");
+}
+
 size_t logdet_work_bytes() { return kLogdetParts * (sizeof(double) + sizeof(int)); }
 
 template <class T>
@@ -462,6 +489,8 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void potf2_inv<T>(hipStream_t, bool, int, T*, long long, T*, int*, int);                         \
     template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*);                      \
     template void diag_logdet<T>(hipStream_t, int, const T*, long long, void*, LogdetResult*);                \
+    template void potrs_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, const T*, T*, long long,   \
+                                  void*, int*);                                                               \
     template void add_diagonal<T>(hipStream_t, int, T*, long long, T);                                        \
     template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);
 DDLA_KERN_INSTANTIATE(double)

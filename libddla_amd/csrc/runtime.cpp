@@ -67,12 +67,12 @@ void DdlaStream::release_events()
     events_used = 0;
 }
 
-LaunchTimer::LaunchTimer(DdlaStream* h, int cls, double flops, double bytes)
-    : h_(h), cls_(cls), flops_(flops), bytes_(bytes)
+LaunchTimer::LaunchTimer(DdlaStream* h, int cls, double flops, double bytes, hipStream_t s)
+    : h_(h), cls_(cls), flops_(flops), bytes_(bytes), s_(s != nullptr ? s : (h != nullptr ? h->stream : nullptr))
 {
     if (h_ != nullptr && h_->profiling) {
         start_ = h_->next_event();
-        DDLA_HIP(hipEventRecord(start_, h_->stream));
+        DDLA_HIP(hipEventRecord(start_, s_));
     }
 }
 
@@ -80,7 +80,7 @@ LaunchTimer::~LaunchTimer()
 {
     if (start_ == nullptr) return;
     hipEvent_t stop = h_->next_event();
-    if (hipEventRecord(stop, h_->stream) == hipSuccess)
+    if (hipEventRecord(stop, s_) == hipSuccess)
         h_->records.push_back({cls_, start_, stop, flops_, bytes_});
 }
 
@@ -112,6 +112,9 @@ ddlaStatus_t ddlaSet(DdlaHandle_t handle, int nprows, int npcols, char major)
         DDLA_HIP(hipGetDevice(&handle->device));
         DDLA_HIP(hipStreamCreate(&handle->stream));
         handle->owns_stream = true;
+        DDLA_HIP(hipStreamCreateWithFlags(&handle->side, hipStreamNonBlocking));
+        DDLA_HIP(hipEventCreateWithFlags(&handle->ev_update, hipEventDisableTiming));
+        DDLA_HIP(hipEventCreateWithFlags(&handle->ev_panel, hipEventDisableTiming));
     }
     handle->nprows_ = handle->npcols_ = handle->nprocs = 1;
     handle->myprow_ = handle->mypcol_ = 0;
@@ -127,6 +130,12 @@ ddlaStatus_t ddlaDestroy(DdlaHandle_t& handle)
         (void)hipStreamSynchronize(handle->stream);
         if (handle->scratch != nullptr) (void)hipFree(handle->scratch);
         for (hipEvent_t e : handle->event_pool) (void)hipEventDestroy(e);
+        if (handle->side != nullptr) {
+            (void)hipStreamSynchronize(handle->side);
+            (void)hipStreamDestroy(handle->side);
+        }
+        if (handle->ev_update != nullptr) (void)hipEventDestroy(handle->ev_update);
+        if (handle->ev_panel != nullptr) (void)hipEventDestroy(handle->ev_panel);
         if (handle->owns_stream && handle->stream != nullptr) (void)hipStreamDestroy(handle->stream);
     }
     delete handle;
