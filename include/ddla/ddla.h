@@ -140,6 +140,16 @@ void gemmVbatched(const DdlaHandle_t& handle, char transA, char transB, int* d_m
                   const T* const* d_A_array, int* d_lda, const T* const* d_B_array, int* d_ldb, T beta,
                   T** d_C_array, int* d_ldc, int batch_count);
 
+// Mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
+// semantics): factor in FP32 (factor = 0) on the matrix cores, refine in FP64
+// until every column has ||r||_inf <= tol ||A||_inf ||x||_inf, at most
+// max_iter corrections; otherwise (or if the FP32 factor fails) run pposv.
+// iter = corrections applied, -3 (FP32 factor failed) or -(max_iter + 1)
+// (no convergence, fallback ran).  Side 'L', trans 'N'.
+template <typename T>
+void pposv_mixed(const DdlaHandle_t& handle, char uplo, int n, int nrhs, T* A, int ia, int ja, const int* descA,
+                 T* B, int ib, int jb, const int* descB, int factor, double tol, int max_iter, int& iter, int& info);
+
 namespace cholesky_impl {
 // Three components per routine: <r>_check (validation), <r>_workspace_size
 // (backend and host bytes; no allocation) and <r>_run on caller scratch (no
@@ -171,6 +181,15 @@ template <class T>
 void pposv_run(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, T* A, int ia, int ja,
                const int* descA, T* B, int ib, int jb, const int* descB, int& info, void* backend_work,
                std::size_t backend_bytes, void* host_work, std::size_t host_bytes);
+void pposv_mixed_check(const DdlaHandle_t& h, char uplo, int n, int nrhs, const void* A, int ia, int ja,
+                       const int* descA, const void* B, int ib, int jb, const int* descB);
+template <class T>
+void pposv_mixed_workspace_size(const DdlaHandle_t& h, char uplo, int n, int nrhs, const int* descA,
+                                const int* descB, std::size_t* backend_bytes, std::size_t* host_bytes);
+template <class T>
+void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, const int* descA, T* B,
+                     const int* descB, int factor, double tol, int max_iter, int& iter, int& info,
+                     void* backend_work, std::size_t backend_bytes);
 } // namespace cholesky_impl
 
 } // namespace ddla

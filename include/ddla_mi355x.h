@@ -102,6 +102,22 @@ int ddla_d_pposv_run(void* handle, char side, char uplo, char trans, int n, int 
                      const int* descA, void* B, int ib, int jb, const int* descB, int* info, void* backend_work,
                      long long backend_bytes, void* host_work, long long host_bytes);
 
+/* ---- mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
+ * semantics): FP64 A X = B, side 'L', factored in FP32 on the matrix cores
+ * (factor_precision 0) and refined in FP64 on the GPU until every column has
+ * ||r||_inf <= tol ||A||_inf ||x||_inf (at most max_iter corrections), else
+ * the FP64 pposv runs.  *iter = corrections applied, -3 (FP32 factor failed)
+ * or -(max_iter + 1) (no convergence; fallback ran).  A is unchanged unless
+ * the fallback ran; B receives X. -------------------------------------- */
+int ddla_d_pposv_mixed(void* handle, char uplo, int n, int nrhs, void* A, int ia, int ja, const int* descA, void* B,
+                       int ib, int jb, const int* descB, int factor_precision, double tol, int max_iter, int* iter,
+                       int* info);
+int ddla_d_pposv_mixed_workspace_size(void* handle, char uplo, int n, int nrhs, const int* descA, const int* descB,
+                                      long long* backend_bytes, long long* host_bytes);
+int ddla_d_pposv_mixed_run(void* handle, char uplo, int n, int nrhs, void* A, const int* descA, void* B,
+                           const int* descB, int factor_precision, double tol, int max_iter, int* iter, int* info,
+                           void* backend_work, long long backend_bytes);
+
 /* ---- local potrf, device info (C++ ddla::potrf, include/ddla/ddla.h:2160) */
 int ddla_s_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
 int ddla_d_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
@@ -136,8 +152,9 @@ int ddla_d_spd_generate(void* handle, int n, unsigned long long seed, double del
  * the handle (and clears earlier records); ddla_c_profile_read synchronizes
  * and returns, for kernel class cls, the number of launches, their summed
  * duration in ms and their summed algorithmic flops and bytes.  Classes: 0
- * trailing update, 1 panel trsm, 2 panel update, 3 potf2, 4 solve update, 5
- * solve diagonal, 6 diagonal-block inverse, 7 other. */
+ * trailing update, 1 panel trsm, 2 panel update, 3 potf2, 4 triangular-solve
+ * sweeps, 5 (unused), 6 diagonal-block inverse, 7 refinement residual,
+ * 8 other. */
 int ddla_c_profile(void* handle, int enable);
 int ddla_c_profile_read(void* handle, int cls, int* count, double* ms, double* flops, double* bytes);
 

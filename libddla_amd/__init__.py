@@ -24,6 +24,9 @@ from .ops import (
     ppotrf_run,
     ppotrf_workspace_size,
     pposv,
+    pposv_mixed,
+    pposv_mixed_run,
+    pposv_mixed_workspace_size,
     pposv_run,
     pposv_workspace_size,
     ppotrs,
@@ -33,7 +36,8 @@ from .ops import (
 __all__ = [
     "LIBRARY_PATH", "lib", "check_status", "Backend", "Handle", "DistributedMatrix", "descinit",
     "ppotrf", "ppotrs", "pposv", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
-    "pposv_workspace_size", "pposv_run", "diag_logdet", "DdlaError", "DdlaInvalidHandleError",
+    "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
+    "diag_logdet", "DdlaError", "DdlaInvalidHandleError",
     "DdlaInvalidValueError", "DdlaAllocFailedError", "DdlaBackendUnavailableError", "DdlaInternalError",
     "DdlaExceptionError",
 ]

@@ -80,7 +80,7 @@ class Handle:
 
     # -- per-launch event profiler -------------------------------------------
     KERNEL_CLASSES = ("trailing_update", "panel_trsm", "panel_update", "potf2", "solve_update", "solve_diag",
-                      "trtri", "other")
+                      "trtri", "residual", "other")
 
     def profile(self, enable: bool):
         check_status(lib.ddla_c_profile(self._handle, 1 if enable else 0), "ddla_c_profile")

@@ -318,6 +318,45 @@ DDLA_C_DRIVERS(s, float)
 DDLA_C_DRIVERS(d, double)
 #undef DDLA_C_DRIVERS
 
+// ---- mixed-precision pposv -----------------------------------------------------
+int ddla_d_pposv_mixed(void* handle, char uplo, int n, int nrhs, void* A, int ia, int ja, const int* descA, void* B,
+                       int ib, int jb, const int* descB, int factor_precision, double tol, int max_iter, int* iter,
+                       int* info)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if (iter == nullptr || info == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+        ddla::pposv_mixed<double>(h, uplo, n, nrhs, static_cast<double*>(A), ia, ja, descA, static_cast<double*>(B),
+                                  ib, jb, descB, factor_precision, tol, max_iter, *iter, *info);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_d_pposv_mixed_workspace_size(void* handle, char uplo, int n, int nrhs, const int* descA, const int* descB,
+                                      long long* backend_bytes, long long* host_bytes)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if (!backend_bytes || !host_bytes) return DDLA_C_STATUS_INVALID_VALUE;
+        std::size_t b = 0, hb = 0;
+        ddla::cholesky_impl::pposv_mixed_workspace_size<double>(h, uplo, n, nrhs, descA, descB, &b, &hb);
+        *backend_bytes = (long long)b;
+        *host_bytes = (long long)hb;
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_d_pposv_mixed_run(void* handle, char uplo, int n, int nrhs, void* A, const int* descA, void* B,
+                           const int* descB, int factor_precision, double tol, int max_iter, int* iter, int* info,
+                           void* backend_work, long long backend_bytes)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if (iter == nullptr || info == nullptr || backend_bytes < 0) return DDLA_C_STATUS_INVALID_VALUE;
+        ddla::cholesky_impl::pposv_mixed_run<double>(h, uplo, n, nrhs, static_cast<double*>(A), descA,
+                                                     static_cast<double*>(B), descB, factor_precision, tol, max_iter,
+                                                     *iter, *info, backend_work, (std::size_t)backend_bytes);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
 // ---- harness helpers ---------------------------------------------------------
 int ddla_c_fill_normal(void* handle, unsigned long long seed, unsigned long long stream, unsigned long long first,
                        long long count, double* d_out)

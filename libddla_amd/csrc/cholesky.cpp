@@ -437,6 +437,14 @@ void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* d_m, int
                            d_ldc, batch, host[0], host[1]);
 }
 
+template <class T>
+void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, T* inv)
+{
+    potrf_blocked<T>(h, upper, n, A, lda, info, inv);
+}
+template void potrf_blocked_dispatch<float>(DdlaStream*, bool, int, float*, long long, int*, float*);
+template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, long long, int*, double*);
+
 #define DDLA_INSTANTIATE(T)                                                                                    \
     template void ppotrf<T>(const DdlaHandle_t&, char, int, T*, int, int, const int*, int&);                  \
     template void ppotrs<T>(const DdlaHandle_t&, char, char, char, int, int, const T*, const int*, T*,         \

@@ -28,8 +28,9 @@ enum KernelClass {
     kClassSolveUpdate = 4,     // potrs off-diagonal updates (MFMA)
     kClassSolveDiag = 5,       // potrs diagonal-block applications
     kClassTrtri = 6,           // inversion of the diagonal blocks for potrs
-    kClassOther = 7,
-    kClassCount = 8
+    kClassResidual = 7,        // FP64 residual R = B - A X of the refinement (MFMA)
+    kClassOther = 8,
+    kClassCount = 9
 };
 
 class DdlaStream {

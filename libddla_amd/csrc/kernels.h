@@ -69,6 +69,25 @@ size_t logdet_work_bytes();
 template <class T>
 void diag_logdet(hipStream_t s, int n, const T* A, long long lda, void* work, LogdetResult* out);
 
+// Mixed-precision refinement helpers.
+// Converts the stored triangle (lower or upper) of the order-n symmetric S to
+// D and computes its infinity norm into *rowmax (device); work must hold
+// sym_norm_work_bytes(n) bytes.  Deterministic summation order.
+size_t sym_norm_work_bytes(int n);
+template <class D, class S>
+void convert_sym_with_norm(hipStream_t s, int n, bool lower, const S* in, long long ldi, D* out, long long ldo,
+                           void* work, double* rowmax);
+// out[j] = max_i |M(i, j)| (device, double), NaN propagating.
+template <class T>
+void column_max(hipStream_t s, int m, int n, const T* M, long long ld, double* out);
+// X += (double) W
+template <class S>
+void add_converted(hipStream_t s, int m, int n, const S* W, long long ldw, double* X, long long ldx);
+// R = B - A X, A symmetric given by the stored triangle of S (lower or upper).
+template <class T>
+void symm_residual(hipStream_t s, bool lower, int n, int nrhs, const T* S, long long lds, const T* X, long long ldx,
+                   const T* B, long long ldb, T* R, long long ldr);
+
 // Setup helpers (not on the timed path).
 void fill_normal(hipStream_t s, uint64_t seed, uint64_t stream, uint64_t first, long long count, double* out);
 template <class T>

@@ -241,6 +241,105 @@ __global__ void logdet_final_kernel(const double* pmag, const int* pflags, Logde
 }
 
 // ---------------------------------------------------------------------------
+// Mixed-precision refinement helpers.
+// ---------------------------------------------------------------------------
+// One 64 x 64 tile (ib, jb) of the stored triangle: converts it to D and
+// writes, for the infinity norm of the symmetric matrix, the tile's row sums
+// of |a| (prow[jb * n + i]) and its column sums of the mirrored entries
+// (pcol[ib * n + j]; the diagonal element is counted once, in its row).
+template <class D, class S>
+__global__ __launch_bounds__(256) void convert_sym_norm_kernel(int n, int lower, const S* in, long long ldi, D* out,
+                                                               long long ldo, double* prow, double* pcol)
+{
+    const int ib = blockIdx.x, jb = blockIdx.y;
+    if (lower ? jb > ib : jb < ib) return;
+    __shared__ double Ts[64][65];
+    const long long i0 = (long long)ib * 64, j0 = (long long)jb * 64;
+    const int rows = (int)min(64LL, n - i0), cols = (int)min(64LL, n - j0);
+    const bool diag = ib == jb;
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        double v = 0;
+        if (r < rows && c < cols && (!diag || (lower ? r >= c : r <= c))) {
+            const S x = in[(i0 + r) + (j0 + c) * ldi];
+            out[(i0 + r) + (j0 + c) * ldo] = (D)x;
+            v = fabs((double)x);
+        }
+        Ts[r][c] = v;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 64) {
+        if (t < rows) {
+            double sum = 0;
+            for (int c = 0; c < 64; ++c) sum += Ts[t][c];
+            prow[(long long)jb * n + i0 + t] = sum;
+        }
+    } else if (t < 128) {
+        const int c = t - 64;
+        if (c < cols) {
+            double sum = 0;
+            for (int r = 0; r < 64; ++r)
+                if (!(diag && r == c)) sum += Ts[r][c];
+            pcol[(long long)ib * n + j0 + c] = sum;
+        }
+    }
+}
+
+// rowsum[i] = sum over the tiles of row i's block row (prow) and block column
+// (pcol), in a fixed order.
+__global__ void sym_rowsum_kernel(int n, int nblk, int lower, const double* prow, const double* pcol, double* rowsum)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int bi = i / 64;
+        double s = 0;
+        if (lower) {
+            for (int jb = 0; jb <= bi; ++jb) s += prow[(long long)jb * n + i];
+            for (int ib = bi; ib < nblk; ++ib) s += pcol[(long long)ib * n + i];
+        } else {
+            for (int jb = bi; jb < nblk; ++jb) s += prow[(long long)jb * n + i];
+            for (int ib = 0; ib <= bi; ++ib) s += pcol[(long long)ib * n + i];
+        }
+        rowsum[i] = s;
+    }
+}
+
+// out[j] = max_i |M(i, j)|, i < m, one workgroup per column (NaN propagates).
+template <class T>
+__global__ __launch_bounds__(256) void colmax_kernel(int m, const T* M, long long ld, double* out)
+{
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    double v = 0;
+    for (int i = threadIdx.x; i < m; i += 256) {
+        const double x = fabs((double)M[i + (long long)j * ld]);
+        v = (x > v || x != x) ? x : v;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// X(i, j) += (double) W(i, j)
+template <class S>
+__global__ void add_converted_kernel(int m, int n, const S* W, long long ldw, double* X, long long ldx)
+{
+    const long long total = (long long)m * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % m), j = (int)(t / m);
+        X[i + j * ldx] += (double)W[i + j * ldw];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Setup helpers.
 // ---------------------------------------------------------------------------
 __device__ void philox4x64_10(uint64_t c[4], uint64_t k0, uint64_t k1)
@@ -423,7 +522,7 @@ void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
     const int nblk = (n + kSB - 1) / kSB, slabs = (nrhs + kSB - 1) / kSB;
     if (slabs > 65535) throw std::invalid_argument("potrs: too many right-hand sides for one launch");
     SolveArgs<T> a{n, nrhs, nblk, L, ldl, upper ? 1 : 0, dinv, B, ldb, static_cast<int*>(work), error, 1 << 23};
-    const dim3 grid((unsigned)nblk, (unsigned)slabs), block(256);
+    const dim3 grid((unsigned)nblk, (unsigned)slabs), block(kThreads);
     zero_ints_kernel<<<1, 256, 0, s>>>(a.sync, 2 * slabs);
     check_launch("potrs");
     solve_sweep_kernel<T, false><<<grid, block, 0, s>>>(a);
@@ -432,6 +531,59 @@ void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
     check_launch("potrs");
     solve_sweep_kernel<T, true><<<grid, block, 0, s>>>(a);
     check_launch("potrs");
+}
+
+size_t sym_norm_work_bytes(int n)
+{
+    const size_t nblk = (size_t)(n + 63) / 64;
+    return ((2 * nblk + 1) * (size_t)n * sizeof(double) + 255) / 256 * 256;
+}
+
+template <class D, class S>
+void convert_sym_with_norm(hipStream_t s, int n, bool lower, const S* in, long long ldi, D* out, long long ldo,
+                           void* work, double* rowmax)
+{
+    if (n <= 0) return;
+    const int nblk = (n + 63) / 64;
+    if (nblk > 65535) throw std::invalid_argument("convert_sym_with_norm: order too large");
+    double* prow = static_cast<double*>(work);
+    double* pcol = prow + (size_t)nblk * n;
+    double* rowsum = pcol + (size_t)nblk * n;
+    convert_sym_norm_kernel<D, S><<<dim3(nblk, nblk), 256, 0, s>>>(n, lower ? 1 : 0, in, ldi, out, ldo, prow, pcol);
+    check_launch("convert_sym_with_norm");
+    sym_rowsum_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nblk, lower ? 1 : 0, prow, pcol, rowsum);
+    check_launch("convert_sym_with_norm");
+    colmax_kernel<double><<<1, 256, 0, s>>>(n, rowsum, n, rowmax);
+    check_launch("convert_sym_with_norm");
+}
+
+template <class T>
+void column_max(hipStream_t s, int m, int n, const T* M, long long ld, double* out)
+{
+    if (m <= 0 || n <= 0) return;
+    colmax_kernel<T><<<n, 256, 0, s>>>(m, M, ld, out);
+    check_launch("column_max");
+}
+
+template <class S>
+void add_converted(hipStream_t s, int m, int n, const S* W, long long ldw, double* X, long long ldx)
+{
+    if (m <= 0 || n <= 0) return;
+    add_converted_kernel<S><<<grid_for((long long)m * n, 256, 1 << 20), 256, 0, s>>>(m, n, W, ldw, X, ldx);
+    check_launch("add_converted");
+}
+
+template <class T>
+void symm_residual(hipStream_t s, bool lower, int n, int nrhs, const T* S, long long lds, const T* X, long long ldx,
+                   const T* B, long long ldb, T* R, long long ldr)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    using namespace solve_detail;
+    const int nblk = (n + kSB - 1) / kSB, slabs = (nrhs + kSB - 1) / kSB;
+    if (slabs > 65535) throw std::invalid_argument("symm_residual: too many right-hand sides for one launch");
+    SymmArgs<T> a{n, nrhs, nblk, S, lds, lower ? 1 : 0, X, ldx, B, ldb, R, ldr};
+    symm_residual_kernel<T><<<dim3((unsigned)nblk, (unsigned)slabs), kThreads, 0, s>>>(a);
+    check_launch("symm_residual");
 }
 
 size_t logdet_work_bytes() { return kLogdetParts * (sizeof(double) + sizeof(int)); }
@@ -497,6 +649,13 @@ DDLA_KERN_INSTANTIATE(double)
 DDLA_KERN_INSTANTIATE(float)
 #undef DDLA_KERN_INSTANTIATE
 template void convert<float, double>(hipStream_t, int, int, const double*, long long, float*, long long);
+template void convert_sym_with_norm<float, double>(hipStream_t, int, bool, const double*, long long, float*, long long,
+                                                   void*, double*);
+template void column_max<double>(hipStream_t, int, int, const double*, long long, double*);
+template void column_max<float>(hipStream_t, int, int, const float*, long long, double*);
+template void add_converted<float>(hipStream_t, int, int, const float*, long long, double*, long long);
+template void symm_residual<double>(hipStream_t, bool, int, int, const double*, long long, const double*, long long,
+                                    const double*, long long, double*, long long);
 template void convert<double, float>(hipStream_t, int, int, const float*, long long, double*, long long);
 template void convert<double, double>(hipStream_t, int, int, const double*, long long, double*, long long);
 

@@ -12,8 +12,11 @@
 // before its own, i.e. to workgroups that are already running, so the sweep
 // always progresses; every wait is nevertheless bounded by a spin limit, after
 // which the workgroup flags an error and carries on (the host reports it).
-// Publication uses agent-scope release / acquire fences, so blocks written on
-// one XCD are seen on the others.
+// Publication: the writer's stores are made visible with an agent-scope
+// release fence (L2 write-back) before the counter is stored with release
+// order; readers poll the counter and read the published block with
+// agent-scope atomic loads, which see the written-back data on every XCD
+// without invalidating their L2.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -42,42 +45,97 @@ struct SolveArgs {
 constexpr int kSB = 64;       // block rows and right-hand-side columns per workgroup
 constexpr int kSL = kSB + 1;  // LDS leading dimension
 
+constexpr int kThreads = 512;                  // 8 wavefronts per workgroup
+constexpr int kPer = kSB * kSB / kThreads;     // elements of a 64 x 64 block per thread
+
 // Ls[k][r] := block(r, k), rows r < rows, k < cols of the L view starting at
-// (i0, j0); transposed: block(r, k) = L(j0 + k, i0 + r).
+// (i0, j0); transposed: block(r, k) = L(j0 + k, i0 + r).  All loads of a
+// thread are issued before any is stored to LDS.
 template <class T>
 __device__ inline void load_factor_block(const SolveArgs<T>& a, long long i0, long long j0, int rows, int cols,
                                          bool transposed, T (*Ls)[kSL])
 {
-    // element (r, k) of the block is L(i, j) with (i, j) = (i0 + r, j0 + k), or (j0 + k, i0 + r) transposed
-    // stored at L[i + j*ldl] (lower) or L[j + i*ldl] (upper); choose the thread order so that
-    // consecutive threads touch consecutive addresses.
+    // element (r, k) is L(i, j), (i, j) = (i0 + r, j0 + k) or (j0 + k, i0 + r) transposed, stored at
+    // L[i + j*ldl] (lower) or L[j + i*ldl] (upper); the thread order makes consecutive threads touch
+    // consecutive addresses.
     const bool rows_contiguous = (!transposed && !a.upper) || (transposed && a.upper);
-    for (int idx = threadIdx.x; idx < kSB * kSB; idx += 256) {
+    T v[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e;
         const int r = rows_contiguous ? (idx & (kSB - 1)) : (idx >> 6);
         const int k = rows_contiguous ? (idx >> 6) : (idx & (kSB - 1));
-        T v = T(0);
+        v[e] = T(0);
         if (r < rows && k < cols) {
             const long long i = transposed ? j0 + k : i0 + r;
             const long long j = transposed ? i0 + r : j0 + k;
-            v = a.upper ? a.L[j + i * a.ldl] : a.L[i + j * a.ldl];
+            v[e] = a.upper ? a.L[j + i * a.ldl] : a.L[i + j * a.ldl];
         }
-        Ls[k][r] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e;
+        const int r = rows_contiguous ? (idx & (kSB - 1)) : (idx >> 6);
+        const int k = rows_contiguous ? (idx >> 6) : (idx & (kSB - 1));
+        Ls[k][r] = v[e];
+    }
+}
+
+// Xs[k][c] := B(j0 + k, c0 + c), k < rows, c < cols (zero elsewhere), for a
+// block published by another workgroup: read with agent-scope atomic loads,
+// which are served coherently across the XCDs' L2 caches, so no L2
+// invalidation is needed (the writer released the block before publishing).
+template <class T>
+__device__ inline void load_rhs_block(const SolveArgs<T>& a, long long j0, int c0, int rows, int cols, T (*Xs)[kSL])
+{
+    T v[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e, k = idx & (kSB - 1), c = idx >> 6;
+        v[e] = (k < rows && c < cols) ? __hip_atomic_load(a.B + (j0 + k) + (long long)(c0 + c) * a.ldb,
+                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                      : T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e, k = idx & (kSB - 1), c = idx >> 6;
+        Xs[k][c] = v[e];
+    }
+}
+
+// Wavefront w owns column tile (w & 3) and row tiles 2(w >> 2) and 2(w >> 2) + 1
+// of a 64 x 64 block.  acc[i] += sign * As(row, k) * Bs(k, col) over k < 64,
+// with As given as As[k][row] and Bs as Bs[k][col] (BT: Bs(k, col) = Bs[col][k]).
+template <class T, bool BT>
+__device__ inline void block_mma(typename Mfma<T>::acc_t (&acc)[2], const T (*As)[kSL], const T (*Bs)[kSL], T sign)
+{
+    using Op = Mfma<T>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = (wave & 3) * 16 + (lane & 15), rt0 = (wave >> 2) * 2;
+#pragma unroll 4
+    for (int ks = 0; ks < kSB; ks += 4) {
+        const int kk = ks + (lane >> 4);
+        const T b = BT ? Bs[col][kk] : Bs[kk][col];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i] = Op::mma(sign * As[kk][(rt0 + i) * 16 + (lane & 15)], b, acc[i]);
     }
 }
 
 template <class T, bool BACKWARD>
-__global__ __launch_bounds__(256) void solve_sweep_kernel(SolveArgs<T> a)
+__global__ __launch_bounds__(kThreads) void solve_sweep_kernel(SolveArgs<T> a)
 {
     using Op = Mfma<T>;
     using acc_t = typename Op::acc_t;
-    __shared__ T Ls[kSB][kSL];  // [k][row] of the factor block (or of the diagonal inverse)
-    __shared__ T Xs[kSB][kSL];  // [k][column] of a published solution block (or of the accumulator)
+    __shared__ T Ls[kSB][kSL];  // [k][row] of the factor block, later of M = D * L(t, t-1)
+    __shared__ T Xs[kSB][kSL];  // [k][column] of a published solution block, or of the accumulator
+    __shared__ T Ds[kSB][kSL];  // [k][row] of the diagonal block's inverse (or its transpose)
     __shared__ int s_ticket, s_avail;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int slab = blockIdx.y, c0 = slab * kSB, nc = min(kSB, a.nrhs - c0);
     int* ticket_ctr = a.sync + 2 * slab;
     int* done_ctr = a.sync + 2 * slab + 1;
+    const int col = (wave & 3) * 16 + (lane & 15), rt0 = (wave >> 2) * 2;
 
     if (tid == 0) s_ticket = atomicAdd(ticket_ctr, 1);
     __syncthreads();
@@ -86,26 +144,40 @@ __global__ __launch_bounds__(256) void solve_sweep_kernel(SolveArgs<T> a)
     const long long r0 = (long long)bt * kSB;
     const int w = (int)min((long long)kSB, a.n - r0);
 
-    // accumulator: wavefront `wave` owns right-hand-side columns [16 wave, 16 wave + 16)
-    // and the 4 row tiles of the block.
-    acc_t acc[4];
+    // Ds[k][row] = D(row, k), D = Dinv (forward) or Dinv^T (backward)
+    {
+        const T* d = a.dinv + (long long)bt * kSB * kSB;
+        T v[kPer];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
+        for (int e = 0; e < kPer; ++e) {
+            const int idx = tid + kThreads * e, row = idx & (kSB - 1), k = idx >> 6;
+            v[e] = BACKWARD ? d[k + row * kSB] : d[row + k * kSB];
+        }
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int idx = tid + kThreads * e, row = idx & (kSB - 1), k = idx >> 6;
+            Ds[k][row] = v[e];
+        }
+    }
+    acc_t acc[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = rt * 16 + Op::row(lane, r), col = wave * 16 + (lane & 15);
-            acc[rt][r] = (row < w && col < nc) ? a.B[(r0 + row) + (long long)(c0 + col) * a.ldb] : T(0);
+            const int row = (rt0 + i) * 16 + Op::row(lane, r);
+            acc[i][r] = (row < w && col < nc) ? a.B[(r0 + row) + (long long)(c0 + col) * a.ldb] : T(0);
         }
 
-    int processed = 0;
+    int avail = 0;
     bool failed = false;
-    while (processed < t) {
+    // Returns false if the spin limit was hit.  Afterwards blocks [0, avail) are published and visible.
+    auto wait_for = [&](int q) -> bool {
+        if (q < avail) return true;
         if (tid == 0) {
-            // relaxed polling (no cache invalidation per poll); the acquire
-            // fence below orders the reads of the published blocks
+            // relaxed polling; the published blocks are then read with coherent loads (load_rhs_block)
             int d = __hip_atomic_load(done_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int spins = 0;
-            while (d <= processed) {
+            while (d <= q) {
                 if (++spins > a.spin_limit) {
                     d = -1;
                     break;
@@ -116,71 +188,190 @@ __global__ __launch_bounds__(256) void solve_sweep_kernel(SolveArgs<T> a)
             s_avail = d;
         }
         __syncthreads();
-        const int avail = s_avail;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __syncthreads();
-        if (avail < 0) {
+        avail = s_avail;
+        return avail >= 0;
+    };
+    auto block_of = [&](int q, long long& j0, int& wj) {
+        const int bj = BACKWARD ? a.nblk - 1 - q : q;
+        j0 = (long long)bj * kSB;
+        wj = (int)min((long long)kSB, a.n - j0);
+    };
+
+    // every published block but the last: acc -= L(t, q) X_q
+    for (int q = 0; q + 1 < t; ++q) {
+        long long j0;
+        int wj;
+        block_of(q, j0, wj);
+        load_factor_block(a, r0, j0, w, wj, BACKWARD, Ls);
+        if (!wait_for(q)) {
             failed = true;
             break;
         }
-        const int upto = min(avail, t);
-        for (int q = processed; q < upto; ++q) {
-            const int bj = BACKWARD ? a.nblk - 1 - q : q;
-            const long long j0 = (long long)bj * kSB;
-            const int wj = (int)min((long long)kSB, a.n - j0);
-            // block(r, k) = L(r0 + r, j0 + k) forward, L(j0 + k, r0 + r) backward; r < w, k < wj
-            load_factor_block(a, r0, j0, w, wj, BACKWARD, Ls);
-            for (int idx = tid; idx < kSB * kSB; idx += 256) {
-                const int k = idx & (kSB - 1), col = idx >> 6;
-                Xs[k][col] = (k < wj && col < nc) ? a.B[(j0 + k) + (long long)(c0 + col) * a.ldb] : T(0);
-            }
-            __syncthreads();
-#pragma unroll 4
-            for (int ks = 0; ks < kSB; ks += 4) {
-                const int kk = ks + (lane >> 4);
-                const T b = Xs[kk][wave * 16 + (lane & 15)];
+        load_rhs_block(a, j0, c0, wj, nc, Xs);
+        __syncthreads();
+        block_mma<T, false>(acc, Ls, Xs, T(-1));
+        __syncthreads();
+    }
+
+    // stage the accumulator as Xs[row][col]
+    __syncthreads();
 #pragma unroll
-                for (int rt = 0; rt < 4; ++rt) acc[rt] = Op::mma(-Ls[kk][rt * 16 + (lane & 15)], b, acc[rt]);
-            }
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xs[(rt0 + i) * 16 + Op::row(lane, r)][col] = acc[i][r];
+    if (t >= 1 && !failed) {
+        // X_t = D (acc - L(t, t-1) X_{t-1}) = Z - M X_{t-1}: Z = D acc and
+        // M = D L(t, t-1) are formed before X_{t-1} is available, so the
+        // only product left after it is published is M X_{t-1}.
+        long long j0;
+        int wj;
+        block_of(t - 1, j0, wj);
+        load_factor_block(a, r0, j0, w, wj, BACKWARD, Ls);
+        __syncthreads();
+        acc_t z[2], m[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) z[i] = m[i] = acc_t{0, 0, 0, 0};
+        block_mma<T, false>(z, Ds, Xs, T(1));  // Z(row, col) = sum_k D(row, k) acc(k, col)
+        block_mma<T, true>(m, Ds, Ls, T(1));   // M(row, c) = sum_k D(row, k) L(k, c), L(k, c) = Ls[c][k]
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Ls[col][(rt0 + i) * 16 + Op::row(lane, r)] = m[i][r];  // Ls[k][row] = M(row, k)
+        if (!wait_for(t - 1)) {
+            failed = true;
+        } else {
+            load_rhs_block(a, j0, c0, wj, nc, Xs);
             __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 2; ++i) acc[i] = z[i];
+            block_mma<T, false>(acc, Ls, Xs, T(-1));
         }
-        processed = upto;
+    } else {
+        __syncthreads();
+        acc_t x[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) x[i] = acc_t{0, 0, 0, 0};
+        block_mma<T, false>(x, Ds, Xs, T(1));
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i] = x[i];
     }
     if (failed && tid == 0) atomicExch(a.error, 1);
-
-    // X_t = Dinv_bt * acc (forward) or Dinv_bt^T * acc (backward)
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Xs[rt * 16 + Op::row(lane, r)][wave * 16 + (lane & 15)] = acc[rt][r];
-    const T* d = a.dinv + (long long)bt * kSB * kSB;
-    for (int idx = tid; idx < kSB * kSB; idx += 256) {
-        const int row = idx & (kSB - 1), k = idx >> 6;
-        // Ls[k][row] = D(row, k), D = Dinv (forward) or Dinv^T (backward)
-        Ls[k][row] = BACKWARD ? d[k + row * kSB] : d[row + k * kSB];
-    }
-    __syncthreads();
-    acc_t x[4];
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt) x[rt] = acc_t{0, 0, 0, 0};
-#pragma unroll 4
-    for (int ks = 0; ks < kSB; ks += 4) {
-        const int kk = ks + (lane >> 4);
-        const T b = Xs[kk][wave * 16 + (lane & 15)];
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt) x[rt] = Op::mma(Ls[kk][rt * 16 + (lane & 15)], b, x[rt]);
-    }
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = rt * 16 + Op::row(lane, r), col = wave * 16 + (lane & 15);
-            if (row < w && col < nc) a.B[(r0 + row) + (long long)(c0 + col) * a.ldb] = x[rt][r];
+            const int row = (rt0 + i) * 16 + Op::row(lane, r);
+            if (row < w && col < nc) a.B[(r0 + row) + (long long)(c0 + col) * a.ldb] = acc[i][r];
         }
     // publish block t: every wavefront's stores are released before the counter moves
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (tid == 0) __hip_atomic_store(done_ctr, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Residual R = B - A X of a symmetric A given by one stored triangle
+// (lower: A(i, k) = S[i + k*lds] for i >= k, S[k + i*lds] otherwise; upper:
+// the mirror).  One 512-thread workgroup per 64-row block of R and 64-column
+// slab: it walks the 64-column blocks k of A, staging A(i-block, k-block) and
+// X(k-block) in LDS, and accumulates -A X on the matrix cores.  Off-diagonal
+// blocks are read entirely from the stored triangle or entirely from its
+// mirror (coalesced either way); diagonal blocks element by element.
+// ---------------------------------------------------------------------------
+template <class T>
+struct SymmArgs {
+    int n, nrhs, nblk;
+    const T* S;
+    long long lds;
+    int lower;
+    const T* X;
+    long long ldx;
+    const T* B;
+    long long ldb;
+    T* R;
+    long long ldr;
+};
+
+template <class T>
+__device__ inline void load_sym_block(const SymmArgs<T>& a, long long i0, long long k0, int rows, int cols,
+                                      bool diagonal, bool direct, T (*Ls)[kSL])
+{
+    // direct: element (r, k) of the block is S[(i0 + r) + (k0 + k) * lds] (contiguous along r);
+    // otherwise (mirrored) S[(k0 + k) + (i0 + r) * lds] (contiguous along k).
+    const bool rows_contiguous = diagonal || direct;
+    T v[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e;
+        const int r = rows_contiguous ? (idx & (kSB - 1)) : (idx >> 6);
+        const int k = rows_contiguous ? (idx >> 6) : (idx & (kSB - 1));
+        v[e] = T(0);
+        if (r < rows && k < cols) {
+            const bool d = diagonal ? (a.lower ? r >= k : r <= k) : direct;
+            const long long i = i0 + r, j = k0 + k;
+            v[e] = d ? a.S[i + j * a.lds] : a.S[j + i * a.lds];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e;
+        const int r = rows_contiguous ? (idx & (kSB - 1)) : (idx >> 6);
+        const int k = rows_contiguous ? (idx >> 6) : (idx & (kSB - 1));
+        Ls[k][r] = v[e];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void symm_residual_kernel(SymmArgs<T> a)
+{
+    using Op = Mfma<T>;
+    using acc_t = typename Op::acc_t;
+    __shared__ T Ls[kSB][kSL];
+    __shared__ T Xs[kSB][kSL];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ib = blockIdx.x, slab = blockIdx.y, c0 = slab * kSB, nc = min(kSB, a.nrhs - c0);
+    const int col = (wave & 3) * 16 + (lane & 15), rt0 = (wave >> 2) * 2;
+    const long long i0 = (long long)ib * kSB;
+    const int w = (int)min((long long)kSB, a.n - i0);
+    acc_t acc[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = (rt0 + i) * 16 + Op::row(lane, r);
+            acc[i][r] = (row < w && col < nc) ? a.B[(i0 + row) + (long long)(c0 + col) * a.ldb] : T(0);
+        }
+    for (int kb = 0; kb < a.nblk; ++kb) {
+        const long long k0 = (long long)kb * kSB;
+        const int wk = (int)min((long long)kSB, a.n - k0);
+        const bool diagonal = kb == ib;
+        const bool direct = a.lower ? kb < ib : kb > ib;
+        load_sym_block(a, i0, k0, w, wk, diagonal, direct, Ls);
+        {
+            T v[kPer];
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int idx = threadIdx.x + kThreads * e, k = idx & (kSB - 1), c = idx >> 6;
+                v[e] = (k < wk && c < nc) ? a.X[(k0 + k) + (long long)(c0 + c) * a.ldx] : T(0);
+            }
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int idx = threadIdx.x + kThreads * e, k = idx & (kSB - 1), c = idx >> 6;
+                Xs[k][c] = v[e];
+            }
+        }
+        __syncthreads();
+        block_mma<T, false>(acc, Ls, Xs, T(-1));
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = (rt0 + i) * 16 + Op::row(lane, r);
+            if (row < w && col < nc) a.R[(i0 + row) + (long long)(c0 + col) * a.ldr] = acc[i][r];
+        }
 }
 
 __global__ void zero_ints_kernel(int* p, int count)

@@ -88,6 +88,56 @@ def pposv_run(A, B, uplo, backend_work: int, backend_bytes: int, side: str = "L"
     return info.value
 
 
+FACTOR_FP32 = 0
+
+
+def _mixed_args(A, B, uplo):
+    if A.type_char != "d" or B.type_char != "d":
+        raise TypeError("pposv_mixed solves FP64 systems (float64 A and B)")
+    return char(uplo), A.n, B.n
+
+
+def pposv_mixed(A, B, uplo: str = "L", factor: int = FACTOR_FP32, tol: float = None, max_iter: int = 30):
+    """FP64 A X = B by an FP32 factorization plus FP64 iterative refinement (LAPACK dsposv semantics).
+
+    Returns (B, iter, info): iter >= 0 is the number of corrections applied;
+    -3 (FP32 factor failed) or -(max_iter + 1) (no convergence) mean the FP64
+    pposv ran instead, and info is its result.  tol defaults to sqrt(n) * eps(FP64).
+    """
+    c_uplo, n, nrhs = _mixed_args(A, B, uplo)
+    if tol is None:
+        tol = max(1.0, float(n)) ** 0.5 * float(np.finfo(np.float64).eps)
+    it, info = ctypes.c_int(0), ctypes.c_int(0)
+    st = lib.ddla_d_pposv_mixed(A.handle.ptr, c_uplo, n, nrhs, ctypes.c_void_p(A.data_ptr), 1, 1, A.desc_ptr,
+                                ctypes.c_void_p(B.data_ptr), 1, 1, B.desc_ptr, factor, tol, max_iter,
+                                ctypes.byref(it), ctypes.byref(info))
+    check_status(st, "ddla_d_pposv_mixed")
+    return B, it.value, info.value
+
+
+def pposv_mixed_workspace_size(A, B, uplo: str = "L"):
+    c_uplo, n, nrhs = _mixed_args(A, B, uplo)
+    bb, hb = ctypes.c_longlong(), ctypes.c_longlong()
+    check_status(lib.ddla_d_pposv_mixed_workspace_size(A.handle.ptr, c_uplo, n, nrhs, A.desc_ptr, B.desc_ptr,
+                                                       ctypes.byref(bb), ctypes.byref(hb)),
+                 "ddla_d_pposv_mixed_workspace_size")
+    return bb.value, hb.value
+
+
+def pposv_mixed_run(A, B, uplo, backend_work: int, backend_bytes: int, factor: int = FACTOR_FP32,
+                    tol: float = None, max_iter: int = 30):
+    c_uplo, n, nrhs = _mixed_args(A, B, uplo)
+    if tol is None:
+        tol = max(1.0, float(n)) ** 0.5 * float(np.finfo(np.float64).eps)
+    it, info = ctypes.c_int(0), ctypes.c_int(0)
+    st = lib.ddla_d_pposv_mixed_run(A.handle.ptr, c_uplo, n, nrhs, ctypes.c_void_p(A.data_ptr), A.desc_ptr,
+                                    ctypes.c_void_p(B.data_ptr), B.desc_ptr, factor, tol, max_iter,
+                                    ctypes.byref(it), ctypes.byref(info), ctypes.c_void_p(backend_work),
+                                    backend_bytes)
+    check_status(st, "ddla_d_pposv_mixed_run")
+    return it.value, info.value
+
+
 def diag_logdet(A, n: int = None):
     """Principal log of the product of the diagonal: complex (sum log|d|, phase)."""
     n = A.m if n is None else n
