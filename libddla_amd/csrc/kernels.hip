@@ -8,6 +8,7 @@
 #include "kernels.h"
 #include "gemm_tile.h"
 #include "solve_tile.h"
+#include "potf2_tile.h"
 
 #include <cfloat>
 #include <cstdint>
@@ -69,124 +70,40 @@ __global__ __launch_bounds__(256) void vbatched_validate_kernel(int na, int nb, 
 // ---------------------------------------------------------------------------
 constexpr int kIB = 64;
 
-// One 64 x 64 diagonal block in one 256-thread workgroup.  Thread t owns row
-// i = t / 4 and the 16 columns c = t % 4 + 4q (q < 16) of both the factor L
-// and of X = L^-1, in registers.  Before step j, column j of L and row j of X
-// are published in LDS (two alternating buffers, one barrier per step); at
-// step j every thread reads the published values it needs into registers and
-// applies, with FACTOR, the right-looking Cholesky step (pivot sqrt(L(j,j)),
-// column scaling, rank-1 update of the trailing lower triangle) and, in both
-// cases, the matching step of the forward elimination L X = I (row j of X
-// divided by L(j,j), then subtracted from the rows below); the owners of
-// column j+1 and of row j+1 then publish them.  Rows and columns at or beyond
-// the block order w carry the identity and are neither factored nor written
-// back.  Returns 0, or j + 1 for the first pivot that is not positive (or is
-// NaN), in which case X is not written.
-template <class T, bool FACTOR>
-__device__ int factor_invert_block(bool upper, int w, T* A, long long lda, T* inv, T (*col)[kIB], T (*xrow)[kIB])
-{
-    const int tid = threadIdx.x, i = tid >> 2, c0 = tid & 3;
-    T Lr[16], Xr[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int c = c0 + 4 * q;
-        T v = T(0);
-        if (i < w && c < w) {
-            if (c <= i) v = upper ? A[c + (long long)i * lda] : A[i + (long long)c * lda];
-        } else if (i == c) {
-            v = T(1);
-        }
-        Lr[q] = v;
-        Xr[q] = (i == c) ? T(1) : T(0);
-    }
-    // publish column 0 and row 0
-    if (c0 == 0) col[0][i] = Lr[0];
-    if (i == 0) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) xrow[0][c0 + 4 * q] = Xr[q];
-    }
-    __syncthreads();
-    int fail = 0;
-    for (int j = 0; j < w; ++j) {
-        const int cur = j & 1, nxt = cur ^ 1;
-        T cv[16], xv[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            cv[q] = col[cur][c0 + 4 * q];
-            xv[q] = xrow[cur][c0 + 4 * q];
-        }
-        const T d = col[cur][j];
-        const T ci = col[cur][i];
-        if (FACTOR && !(d > T(0))) {
-            fail = j + 1;
-            break;
-        }
-        const T ljj = FACTOR ? T(sqrt(d)) : d;
-        const T rinv = T(1) / ljj;
-        const T li = (i > j) ? (FACTOR ? ci * rinv : ci) : T(0);
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int c = c0 + 4 * q;
-            if (FACTOR) {
-                const T upd = Lr[q] - li * (cv[q] * rinv);
-                const T lo = (c == j) ? li : ((c > j && c <= i) ? upd : Lr[q]);
-                const T dg = (c == j) ? ljj : Lr[q];
-                Lr[q] = (i > j) ? lo : ((i == j) ? dg : Lr[q]);
-            }
-            const T xu = Xr[q] - li * (xv[q] * rinv);
-            const T xs = Xr[q] * rinv;
-            Xr[q] = (c <= j) ? ((i > j) ? xu : ((i == j) ? xs : Xr[q])) : Xr[q];
-        }
-        // publish column j + 1 of L and row j + 1 of X for the next step
-        const int j1 = j + 1;
-        if (c0 == (j1 & 3) && i >= j1) {
-            T v = T(0);
-#pragma unroll
-            for (int q = 0; q < 16; ++q)
-                if (q == (j1 >> 2)) v = Lr[q];
-            col[nxt][i] = v;
-        }
-        if (i == j1) {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) xrow[nxt][c0 + 4 * q] = Xr[q];
-        }
-        __syncthreads();
-    }
-    if (FACTOR) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int c = c0 + 4 * q;
-            if (i < w && c < w && c <= i) {
-                if (upper) A[c + (long long)i * lda] = Lr[q];
-                else A[i + (long long)c * lda] = Lr[q];
-            }
-        }
-    }
-    if (fail == 0) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) inv[i + 64 * (c0 + 4 * q)] = Xr[q];
-    }
-    return fail;
-}
-
+// Factors the w x w diagonal block at A (uplo L or U) and writes
+// inv = L^-1 (64 x 64, zeros above the diagonal and the identity beyond w).
+// info: skipped when *info != 0 on entry; on a failed pivot j it receives
+// info_base + j (1-based) and inv is not written.
 template <class T>
 __global__ __launch_bounds__(256) void potf2_inv_kernel(bool upper, int w, T* A, long long lda, T* inv, int* info,
                                                         int info_base)
 {
+    using namespace potf2_detail;
     if (*info != 0) return;
-    __shared__ T col[2][kIB], xrow[2][kIB];
-    const int fail = factor_invert_block<T, true>(upper, w, A, lda, inv, col, xrow);
-    if (fail != 0 && threadIdx.x == 0) *info = info_base + fail;
+    __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
+    __shared__ int s_flag;
+    load_block<T>(upper, w, A, lda, S);
+    __syncthreads();
+    const int fail = factor_invert64<T>(S, X, rdiag, &s_flag);
+    store_factor<T>(upper, w, S, A, lda);
+    if (fail == 0) store_inverse<T>(X, inv);
+    else if (threadIdx.x == 0) *info = info_base + fail;
 }
 
+// inv(block b) = L(b, b)^-1 for every 64 x 64 diagonal block of a factor.
 template <class T>
 __global__ __launch_bounds__(256) void trtri_kernel(bool upper, int n, const T* A, long long lda, T* inv)
 {
-    __shared__ T col[2][kIB], xrow[2][kIB];
+    using namespace potf2_detail;
+    __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
     const long long j0 = (long long)blockIdx.x * kIB;
     const int w = (int)min((long long)kIB, n - j0);
-    factor_invert_block<T, false>(upper, w, const_cast<T*>(A) + j0 + j0 * lda, lda,
-                                  inv + (long long)blockIdx.x * kIB * kIB, col, xrow);
+    load_block<T>(upper, w, A + j0 + j0 * lda, lda, S);
+    __syncthreads();
+    if (threadIdx.x < kB) rdiag[threadIdx.x] = T(1) / S[threadIdx.x * kL + threadIdx.x];
+    __syncthreads();
+    invert_lower<T>(S, rdiag, X);
+    store_inverse<T>(X, inv + (long long)blockIdx.x * kIB * kIB);
 }
 
 // ---------------------------------------------------------------------------
