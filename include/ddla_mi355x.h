@@ -152,9 +152,8 @@ int ddla_d_spd_generate(void* handle, int n, unsigned long long seed, double del
  * the handle (and clears earlier records); ddla_c_profile_read synchronizes
  * and returns, for kernel class cls, the number of launches, their summed
  * duration in ms and their summed algorithmic flops and bytes.  Classes: 0
- * trailing update, 1 panel trsm, 2 panel update, 3 potf2, 4 triangular-solve
- * sweeps, 5 (unused), 6 diagonal-block inverse, 7 refinement residual,
- * 8 other. */
+ * trailing update, 1 panel factorization, 2 triangular-solve sweeps, 3
+ * diagonal-block inverses, 4 refinement residual, 5 other. */
 int ddla_c_profile(void* handle, int enable);
 int ddla_c_profile_read(void* handle, int cls, int* count, double* ms, double* flops, double* bytes);
 

@@ -66,61 +66,32 @@ void update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& v, int
     }
 }
 
-// X_L(ri.., ci..) (M x w) := X_L * inv^T, inv the 64 x 64 inverse of the
-// diagonal block (in place: one workgroup covers the w columns).
-template <class T>
-void panel_trsm(DdlaStream* h, hipStream_t st, const LView<T>& v, int M, int w, long long ri, long long ci,
-                const T* inv, const int* abort)
-{
-    if (M <= 0) return;
-    LaunchTimer timer(h, kClassPanelTrsm, 2.0 * M * w * w, 0, st);
-    if (!v.upper)
-        kern::gemm<T>(st, 'N', 'T', M, w, w, T(1), v.at(ri, ci), v.lda, inv, kIB, T(0), v.at(ri, ci), v.lda,
-                      kTriNone, 0, abort);
-    else
-        kern::gemm<T>(st, 'N', 'N', w, M, w, T(1), inv, kIB, v.at(ci, ri), v.lda, T(0), v.at(ci, ri), v.lda,
-                      kTriNone, 0, abort);
-}
-
 // Factors the panel of columns [k, k + kb) of the order-n L view (its
 // diagonal block and everything below it), assuming every earlier trailing
-// update has been applied to these columns: for each 64-column block, potf2 +
-// inverse, the triangular solve of the rows below, and the update of the
-// panel's remaining columns.
+// update has been applied to these columns: one persistent launch.
 template <class T>
-void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k, int kb, int* info, T* inv)
+void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k, int kb, int* info, void* work)
 {
-    for (int c = 0; c < kb; c += kIB) {
-        const int w = std::min(kIB, kb - c);
-        const long long r0 = (long long)k + c;
-        {
-            LaunchTimer timer(h, kClassPotf2, (double)w * w * w / 3.0, 0, st);
-            kern::potf2_inv<T>(st, v.upper, w, v.at(r0, r0), v.lda, inv, info, (int)r0);
-        }
-        const int below = n - (int)r0 - w;
-        if (below <= 0) continue;
-        panel_trsm<T>(h, st, v, below, w, r0 + w, r0, inv, info);
-        const int ncols = k + kb - (int)r0 - w;
-        if (ncols > 0)
-            update_lower<T>(h, st, kClassPanelUpdate, v, below, ncols, w, T(-1), r0 + w, r0, r0 + w, r0, T(1), r0 + w,
-                            r0 + w, true, info, 2.0 * below * ncols * w);
-    }
+    const double below = (double)n - k - kb;
+    LaunchTimer timer(h, kClassPanel, (double)kb * kb * kb / 3.0 + below * kb * kb, 0, st);
+    kern::panel_factor<T>(st, v.upper, n, v.A, v.lda, k, kb, info, work);
 }
 
 // Blocked right-looking Cholesky of the order-n L view with look-ahead; *info
 // (device) must be zero on entry and receives the 1-based index of the first
-// failed pivot.  After panel k is factored, the trailing update is split: the
+// failed pivot; work holds kern::panel_work_bytes<T>(n) bytes.  After panel k is factored, the trailing update is split: the
 // columns of panel k+1 are updated first on the handle's stream, panel k+1 is
 // then factored on the side stream while the rest of the trailing matrix is
 // updated on the handle's stream (the two touch disjoint columns).  Returns
 // with all work ordered on the handle's stream.
 template <class T>
-void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, T* inv)
+void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work)
 {
     const LView<T> v{A, lda, upper};
     const int NB = outer_block<T>();
     hipStream_t main = h->stream, side = h->side;
-    factor_panel<T>(h, main, v, n, 0, std::min(NB, n), info, inv);
+    kern::panel_reset<T>(main, n, work);
+    factor_panel<T>(h, main, v, n, 0, std::min(NB, n), info, work);
     for (int k = 0; k < n; k += NB) {
         const int kb = std::min(NB, n - k);
         const int m = n - k - kb;
@@ -132,7 +103,7 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
                         (double)kb2 * (kb2 + 1) / 2 * 2 * kb + 2.0 * (m - kb2) * kb2 * kb);
         DDLA_HIP(hipEventRecord(h->ev_update, main));
         DDLA_HIP(hipStreamWaitEvent(side, h->ev_update, 0));
-        factor_panel<T>(h, side, v, n, (int)r, kb2, info, inv);
+        factor_panel<T>(h, side, v, n, (int)r, kb2, info, work);
         DDLA_HIP(hipEventRecord(h->ev_panel, side));
         // the rest of the trailing matrix: rows and columns [r + kb2, n)
         const int mr = m - kb2;
@@ -149,7 +120,7 @@ template <class T>
 void solve_left(DdlaStream* h, bool upper, int n, int nrhs, const T* F, long long ldf, T* B, long long ldb,
                 const T* dinv, void* sync, int* error)
 {
-    LaunchTimer timer(h, kClassSolveUpdate, 2.0 * n * n * nrhs, (double)n * n * sizeof(T));
+    LaunchTimer timer(h, kClassSolve, 2.0 * n * n * nrhs, (double)n * n * sizeof(T));
     kern::potrs_sweeps<T>(h->stream, upper, n, nrhs, F, ldf, dinv, B, ldb, sync, error);
 }
 
@@ -182,8 +153,8 @@ void ppotrf_workspace_size(const DdlaHandle_t&, char, int n, T*, const int*, std
 {
     *backend_bytes = *host_bytes = 0;
     if (n <= 0) return;
-    // [ info word | inverse of the current 64 x 64 diagonal block ]
-    *backend_bytes = detail::padded_bytes<int>(1) + detail::padded_bytes<T>((std::size_t)kIB * kIB);
+    // [ info word | panel work: diagonal-block inverses, panel counters, error word ]
+    *backend_bytes = detail::padded_bytes<int>(1) + detail::padded_bytes<char>(kern::panel_work_bytes<T>(n));
 }
 
 template <class T>
@@ -199,11 +170,15 @@ void ppotrf_run(const DdlaHandle_t& h, char uplo, int n, T* A, int, int, const i
     if (backend_bytes < need || backend_work == nullptr) bad_workspace("ppotrf");
     char* w = static_cast<char*>(backend_work);
     int* d_info = detail::take<int>(w, 1);
-    T* inv = detail::take<T>(w, (std::size_t)kIB * kIB);
+    char* panel = detail::take<char>(w, kern::panel_work_bytes<T>(n));
     fill_int(h->stream, d_info, 0);
-    potrf_blocked<T>(h, uplo == 'U', n, A, d[DDLA_LLD_], d_info, inv);
+    potrf_blocked<T>(h, uplo == 'U', n, A, d[DDLA_LLD_], d_info, panel);
+    int error = 0;
     DDLA_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    DDLA_HIP(hipMemcpyAsync(&error, kern::panel_error_word<T>(n, panel), sizeof(int), hipMemcpyDeviceToHost,
+                            h->stream));
     DDLA_HIP(hipStreamSynchronize(h->stream));
+    if (error != 0) throw std::runtime_error("ppotrf: a panel factorization wait exceeded its limit");
 }
 
 void ppotrs_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A,
@@ -369,9 +344,9 @@ void potrf(const DdlaHandle_t& h, char uplo, int n, T* A, int lda, int* info)
     }
     if (A == nullptr) throw std::runtime_error("potrf: null A");
     if (lda < std::max(1, n)) throw std::runtime_error("potrf: lda too small");
-    T* inv = static_cast<T*>(h->ensure_scratch(detail::padded_bytes<T>((std::size_t)kIB * kIB)));
+    void* work = h->ensure_scratch(kern::panel_work_bytes<T>(n));
     fill_int(h->stream, info, 0);
-    potrf_blocked<T>(h, u == 'U', n, A, lda, info, inv);
+    potrf_blocked<T>(h, u == 'U', n, A, lda, info, work);
 }
 
 template <class T>
@@ -438,12 +413,12 @@ void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* d_m, int
 }
 
 template <class T>
-void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, T* inv)
+void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work)
 {
-    potrf_blocked<T>(h, upper, n, A, lda, info, inv);
+    potrf_blocked<T>(h, upper, n, A, lda, info, work);
 }
-template void potrf_blocked_dispatch<float>(DdlaStream*, bool, int, float*, long long, int*, float*);
-template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, long long, int*, double*);
+template void potrf_blocked_dispatch<float>(DdlaStream*, bool, int, float*, long long, int*, void*);
+template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, long long, int*, void*);
 
 #define DDLA_INSTANTIATE(T)                                                                                    \
     template void ppotrf<T>(const DdlaHandle_t&, char, int, T*, int, int, const int*, int&);                  \

@@ -22,15 +22,12 @@ namespace ddla {
 // Kernel classes timed by the optional per-launch event profiler.
 enum KernelClass {
     kClassTrailingUpdate = 0,  // syrk-shaped trailing update of potrf (MFMA)
-    kClassPanelTrsm = 1,       // panel triangular solve (MFMA with the inverted diagonal block)
-    kClassPanelUpdate = 2,     // update of the remaining panel columns (MFMA)
-    kClassPotf2 = 3,           // diagonal 64 x 64 factorization + inverse
-    kClassSolveUpdate = 4,     // potrs off-diagonal updates (MFMA)
-    kClassSolveDiag = 5,       // potrs diagonal-block applications
-    kClassTrtri = 6,           // inversion of the diagonal blocks for potrs
-    kClassResidual = 7,        // FP64 residual R = B - A X of the refinement (MFMA)
-    kClassOther = 8,
-    kClassCount = 9
+    kClassPanel = 1,           // persistent panel factorization (diagonal blocks, inverses, rows below)
+    kClassSolve = 2,           // potrs triangular-solve sweeps (MFMA)
+    kClassTrtri = 3,           // inversion of the diagonal blocks for potrs
+    kClassResidual = 4,        // FP64 residual R = B - A X of the refinement (MFMA)
+    kClassOther = 5,
+    kClassCount = 6
 };
 
 class DdlaStream {

@@ -36,22 +36,32 @@ void gemm_vbatched(hipStream_t s, char ta, char tb, const int* m, const int* n, 
 void vbatched_validate(hipStream_t s, int na, int nb, const int* m, const int* n, const int* k, const int* lda,
                        const int* ldb, const int* ldc, int batch, int* summary);
 
-// One workgroup factors the order-w (w <= 64) diagonal block whose first
-// element is A (L view: element (i, j) at A[i + j*lda], or A[j + i*lda] when
-// upper), writes the factor back and the inverse of the factor into inv
-// (64 x 64, column-major, zero above the diagonal and identity beyond w).
-// On a non-positive or NaN pivot p (0-based) it sets *info = info_base + p + 1
-// and returns.  Does nothing if *info != 0 on entry.
+// Panel factorization of the blocked Cholesky (one persistent launch per
+// panel).  The factor is addressed as an "L view": element (i, j), i >= j, at
+// A[i + j*lda], or A[j + i*lda] when upper.  panel_factor factors columns
+// [k, k + kb) (kb <= 512, every earlier trailing update applied): the
+// diagonal blocks, their inverses and the rows below.  On a non-positive or
+// NaN pivot in global column p (0-based) it sets *info = p + 1; it does
+// nothing if *info != 0 on entry.  work holds panel_work_bytes<T>(n) bytes,
+// prepared once per factorization by panel_reset (one fill kernel); *error
+// (the last int of work, see panel_error_word) is set if an internal wait
+// ever exceeded its limit.
 template <class T>
-void potf2_inv(hipStream_t s, bool upper, int w, T* A, long long lda, T* inv, int* info, int info_base);
+size_t panel_work_bytes(int n);
+template <class T>
+void panel_reset(hipStream_t s, int n, void* work);
+template <class T>
+int* panel_error_word(int n, void* work);
+template <class T>
+void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, int kb, int* info, void* work);
 
 // Inverts every 64 x 64 diagonal block of the order-n factor (L view as for
-// potf2_inv) into inv[b * 4096 ...], identity-padded for a ragged last block.
+// panel_factor) into inv[b * 4096 ...], identity-padded for a ragged last block.
 template <class T>
 void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv);
 
 // Forward (L Y = B) then backward (L^T X = Y) substitution with the order-n
-// factor (L view as for potf2_inv) and its inverted 64 x 64 diagonal blocks
+// factor (L view as for panel_factor) and its inverted 64 x 64 diagonal blocks
 // dinv (from trtri_diag_blocks), in place on B (n x nrhs), one launch per
 // sweep.  work must hold solve_work_bytes(nrhs) bytes; *error (device) is set
 // to 1 if a sweep's internal wait ever exceeded its limit (the result is then

@@ -9,6 +9,7 @@
 #include "gemm_tile.h"
 #include "solve_tile.h"
 #include "potf2_tile.h"
+#include "panel_tile.h"
 
 #include <cfloat>
 #include <cstdint>
@@ -69,26 +70,6 @@ __global__ __launch_bounds__(256) void vbatched_validate_kernel(int na, int nb, 
 // 64 x 64 diagonal blocks: factorization and triangular inverse in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kIB = 64;
-
-// Factors the w x w diagonal block at A (uplo L or U) and writes
-// inv = L^-1 (64 x 64, zeros above the diagonal and the identity beyond w).
-// info: skipped when *info != 0 on entry; on a failed pivot j it receives
-// info_base + j (1-based) and inv is not written.
-template <class T>
-__global__ __launch_bounds__(256) void potf2_inv_kernel(bool upper, int w, T* A, long long lda, T* inv, int* info,
-                                                        int info_base)
-{
-    using namespace potf2_detail;
-    if (*info != 0) return;
-    __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
-    __shared__ int s_flag;
-    load_block<T>(upper, w, A, lda, S);
-    __syncthreads();
-    const int fail = factor_invert64<T>(S, X, rdiag, &s_flag);
-    store_factor<T>(upper, w, S, A, lda);
-    if (fail == 0) store_inverse<T>(X, inv);
-    else if (threadIdx.x == 0) *info = info_base + fail;
-}
 
 // inv(block b) = L(b, b)^-1 for every 64 x 64 diagonal block of a factor.
 template <class T>
@@ -408,12 +389,52 @@ void vbatched_validate(hipStream_t s, int na, int nb, const int* m, const int* n
     check_launch("vbatched_validate");
 }
 
+// panel work: [ kMaxCols inverses | kSyncInts counters per 64-column block | error word ]
 template <class T>
-void potf2_inv(hipStream_t s, bool upper, int w, T* A, long long lda, T* inv, int* info, int info_base)
+size_t panel_work_bytes(int n)
 {
-    if (w <= 0 || w > kIB) throw std::invalid_argument("potf2_inv: block order out of range");
-    potf2_inv_kernel<T><<<1, 256, 0, s>>>(upper, w, A, lda, inv, info, info_base);
-    check_launch("potf2_inv");
+    using namespace panel_detail;
+    const size_t nb = (size_t)(n + kB - 1) / kB;
+    return ((size_t)kMaxCols * kB * kB * sizeof(T) + 255) / 256 * 256 + ((nb * kSyncInts + 1) * sizeof(int) + 255) / 256 * 256;
+}
+
+namespace {
+template <class T>
+int* panel_sync(void* work)
+{
+    using namespace panel_detail;
+    return reinterpret_cast<int*>(static_cast<char*>(work) + ((size_t)kMaxCols * kB * kB * sizeof(T) + 255) / 256 * 256);
+}
+} // namespace
+
+template <class T>
+int* panel_error_word(int n, void* work)
+{
+    using namespace panel_detail;
+    return panel_sync<T>(work) + (size_t)((n + kB - 1) / kB) * kSyncInts;
+}
+
+template <class T>
+void panel_reset(hipStream_t s, int n, void* work)
+{
+    using namespace panel_detail;
+    const int count = ((n + kB - 1) / kB) * kSyncInts + 1;
+    solve_detail::zero_ints_kernel<<<grid_for(count, 256), 256, 0, s>>>(panel_sync<T>(work), count);
+    check_launch("panel_reset");
+}
+
+template <class T>
+void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, int kb, int* info, void* work)
+{
+    using namespace panel_detail;
+    if (kb <= 0 || k < 0 || k + kb > n) return;
+    if (kb > kMaxCols * kB) throw std::invalid_argument("panel_factor: panel wider than 512 columns");
+    if (k % kB != 0) throw std::invalid_argument("panel_factor: panel must start on a 64-column boundary");
+    PanelArgs<T> a{A, lda, upper ? 1 : 0, n, k, kb, info, static_cast<T*>(work),
+                   panel_sync<T>(work) + (size_t)(k / kB) * kSyncInts, panel_error_word<T>(n, work), 1 << 23};
+    const int rows = (n - k + kB - 1) / kB;
+    panel_kernel<T><<<rows, 256, 0, s>>>(a);
+    check_launch("panel_factor");
 }
 
 template <class T>
@@ -555,7 +576,10 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void gemm_vbatched<T>(hipStream_t, char, char, const int*, const int*, const int*, T,            \
                                    const T* const*, const int*, const T* const*, const int*, T, T* const*,    \
                                    const int*, int, int, int);                                                \
-    template void potf2_inv<T>(hipStream_t, bool, int, T*, long long, T*, int*, int);                         \
+    template size_t panel_work_bytes<T>(int);                                                                 \
+    template void panel_reset<T>(hipStream_t, int, void*);                                                    \
+    template int* panel_error_word<T>(int, void*);                                                            \
+    template void panel_factor<T>(hipStream_t, bool, int, T*, long long, int, int, int*, void*);              \
     template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*);                      \
     template void diag_logdet<T>(hipStream_t, int, const T*, long long, void*, LogdetResult*);                \
     template void potrs_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, const T*, T*, long long,   \

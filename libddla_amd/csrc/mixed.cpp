@@ -20,7 +20,7 @@ namespace ddla {
 
 // Schedules shared with cholesky.cpp.
 template <class T>
-void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, T* inv);
+void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work);
 
 namespace {
 
@@ -29,7 +29,7 @@ constexpr int kIB = 64;
 std::size_t diag_inverse_elems(int n) { return (std::size_t)((n + kIB - 1) / kIB) * kIB * kIB; }
 
 struct MixedLayout {
-    std::size_t a32, norm, scalars, x, r, w, info, inv, dinv, sync, error, total;
+    std::size_t a32, norm, scalars, x, r, w, info, panel, dinv, sync, error, total;
 };
 
 MixedLayout mixed_layout(int n, int nrhs)
@@ -49,7 +49,7 @@ MixedLayout mixed_layout(int n, int nrhs)
     l.r = put(nr * sizeof(double));
     l.w = put(nr * sizeof(float));
     l.info = put(sizeof(int));
-    l.inv = put((std::size_t)kIB * kIB * sizeof(float));
+    l.panel = put(kern::panel_work_bytes<float>(n));
     l.dinv = put(diag_inverse_elems(n) * sizeof(float));
     l.sync = put(kern::solve_work_bytes(nrhs));
     l.error = put(sizeof(int));
@@ -107,7 +107,6 @@ void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, co
     double* r = reinterpret_cast<double*>(base + l.r);
     float* w = reinterpret_cast<float*>(base + l.w);
     int* info32 = reinterpret_cast<int*>(base + l.info);
-    float* inv = reinterpret_cast<float*>(base + l.inv);
     float* dinv = reinterpret_cast<float*>(base + l.dinv);
     int* error = reinterpret_cast<int*>(base + l.error);
     const bool upper = uplo == 'U';
@@ -120,10 +119,13 @@ void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, co
         kern::convert_sym_with_norm<float, double>(s, n, !upper, A, lda, a32, n, base + l.norm, scalars);
     }
     fill_int(s, info32, 0);
-    potrf_blocked_dispatch<float>(h, upper, n, a32, n, info32, inv);
-    int host_info = 0;
+    potrf_blocked_dispatch<float>(h, upper, n, a32, n, info32, base + l.panel);
+    int host_info = 0, panel_error = 0;
     DDLA_HIP(hipMemcpyAsync(&host_info, info32, sizeof(int), hipMemcpyDeviceToHost, s));
+    DDLA_HIP(hipMemcpyAsync(&panel_error, kern::panel_error_word<float>(n, base + l.panel), sizeof(int),
+                            hipMemcpyDeviceToHost, s));
     DDLA_HIP(hipStreamSynchronize(s));
+    if (panel_error != 0) throw std::runtime_error("pposv_mixed: a panel factorization wait exceeded its limit");
 
     bool converged = false;
     if (host_info == 0) {
@@ -133,7 +135,7 @@ void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, co
             kern::trtri_diag_blocks<float>(s, upper, n, a32, n, dinv);
         }
         auto solve32 = [&]() {
-            LaunchTimer timer(h, kClassSolveUpdate, 2.0 * n * n * nrhs, (double)n * n * sizeof(float));
+            LaunchTimer timer(h, kClassSolve, 2.0 * n * n * nrhs, (double)n * n * sizeof(float));
             kern::potrs_sweeps<float>(s, upper, n, nrhs, a32, n, dinv, w, n, base + l.sync, error);
         };
         kern::convert<float, double>(s, n, nrhs, B, ldb, w, n);
