@@ -6,7 +6,7 @@
 //   memory helpers          include/ddla/ddla_handle_t.h:130-284
 //   descriptor              include/ddla/ddla_desc.h:10-135
 //   ppotrf / ppotrs / pposv include/ddla/ddla.h:1065, 1149, 1184
-//   gemmVbatched            include/ddla/ddla.h:1836
+//   gemmVbatched(2s)        include/ddla/ddla.h:1836, 1857
 //   potrf                   include/ddla/ddla.h:2160
 //   diag_logdet             include/ddla/ddla.h:2901
 //   check / size / run      src/Cholesky/cholesky_workspace.h:133-249
@@ -140,6 +140,18 @@ void gemmVbatched(const DdlaHandle_t& handle, char transA, char transB, int* d_m
                   const T* const* d_A_array, int* d_lda, const T* const* d_B_array, int* d_ldb, T beta,
                   T** d_C_array, int* d_ldc, int batch_count);
 
+// Two dependent variable-size stages sharing a temporary (reference
+// include/ddla/ddla.h:1857-1877): per host segment, in order, stage 0 writes
+// C0 (beta_0 must be zero), then stage 1 computes C1 = alpha_1 op(C0) op(AB1)
+// + beta_1 C1 (C0_left) or alpha_1 op(AB1) op(C0) + beta_1 C1.
+template <typename T>
+void gemmVbatched2s(const DdlaHandle_t& handle, char transA_0, char transB_0, int* d_m_0, int* d_n_0, int* d_k_0,
+                    T alpha_0, const T* const* d_A_array_0, int* d_lda_0, const T* const* d_B_array_0, int* d_ldb_0,
+                    T beta_0, T** d_C_array_0, int* d_ldc_0, char transA_1, char transB_1, int* d_m_1, int* d_n_1,
+                    int* d_k_1, T alpha_1, const T* const* d_AB_array_1, int* d_lda_1, int* d_ldb_1, T beta_1,
+                    T** d_C_array_1, int* d_ldc_1, bool C0_left, int batch_count, const int* segment_sizes,
+                    int segment_count);
+
 // Mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
 // semantics): factor in FP32 (factor = 0) on the matrix cores, refine in FP64
 // until every column has ||r||_inf <= tol ||A||_inf ||x||_inf, at most
@@ -149,6 +161,34 @@ void gemmVbatched(const DdlaHandle_t& handle, char transA, char transB, int* d_m
 template <typename T>
 void pposv_mixed(const DdlaHandle_t& handle, char uplo, int n, int nrhs, T* A, int ia, int ja, const int* descA,
                  T* B, int ib, int jb, const int* descB, int factor, double tol, int max_iter, int& iter, int& info);
+
+// Batched SPD solves of uneven order (no counterpart in the reference): for
+// every i < batch_count, independently, factor A_i (order d_n[i] <= max_n <=
+// 1024, triangle uplo, leading dimension d_lda[i]) in place, solve A_i X_i =
+// B_i (d_n[i] x nrhs, d_ldb[i]) in place when the factorization succeeds, and
+// write d_logdet[i] = log det A_i = 2 sum log L_ii.  d_info[i] = 0, the
+// 1-based failed pivot (B_i untouched, logdet NaN), or -1 for an invalid
+// order or leading dimension.  All arrays are device resident; the call is
+// asynchronous on the handle's stream (fused potrf + potrs + logdet in one
+// persistent launch).
+template <typename T>
+void pposvVbatched(const DdlaHandle_t& handle, char uplo, const int* d_n, T* const* d_A, const int* d_lda, int nrhs,
+                   T* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch_count, int max_n);
+// Times the batched schedules on copies of these inputs (left unchanged) and
+// keeps the fastest for later calls on this handle; synchronizes.
+template <typename T>
+void pposvVbatchedTune(const DdlaHandle_t& handle, char uplo, const int* d_n, T* const* d_A, const int* d_lda,
+                       int nrhs, T* const* d_B, const int* d_ldb, int batch_count, int max_n, int* occupancy,
+                       bool* ordered);
+// Captures one batched call (tuning first if the handle has not been tuned)
+// into an instantiated hipGraph that owns its workspace; replay it with
+// batchedGraphLaunch on the handle's stream, free it with batchedGraphDestroy.
+template <typename T>
+void* pposvVbatchedGraph(const DdlaHandle_t& handle, char uplo, const int* d_n, T* const* d_A, const int* d_lda,
+                         int nrhs, T* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch_count,
+                         int max_n);
+void batchedGraphLaunch(void* graph, const DdlaHandle_t& handle);
+void batchedGraphDestroy(void* graph);
 
 namespace cholesky_impl {
 // Three components per routine: <r>_check (validation), <r>_workspace_size
@@ -190,6 +230,16 @@ template <class T>
 void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, const int* descA, T* B,
                      const int* descB, int factor, double tol, int max_iter, int& iter, int& info,
                      void* backend_work, std::size_t backend_bytes);
+void pposv_vbatched_check(const DdlaHandle_t& h, char uplo, int batch, int nrhs, int max_n, const void* d_n,
+                          const void* d_A, const void* d_lda, const void* d_B, const void* d_ldb, const void* d_info,
+                          const void* d_logdet);
+template <class T>
+void pposv_vbatched_workspace_size(const DdlaHandle_t& h, int batch, int max_n, std::size_t* backend_bytes);
+// No allocation and no synchronization: may be captured into a graph.
+template <class T>
+void pposv_vbatched_run(const DdlaHandle_t& h, char uplo, const int* d_n, T* const* d_A, const int* d_lda, int nrhs,
+                        T* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch, int max_n,
+                        void* backend_work, std::size_t backend_bytes);
 } // namespace cholesky_impl
 
 } // namespace ddla

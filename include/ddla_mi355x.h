@@ -137,6 +137,65 @@ int ddla_d_gemm_vbatched(void* handle, char transA, char transB, int* d_m, int* 
                          const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb, const double* beta,
                          void** d_C, int* d_ldc, int batch_count);
 
+/* ---- gemmVbatched2s (C++ ddla::gemmVbatched2s, include/ddla/ddla.h:1857-1877;
+ * no C entry in the reference): per host segment (segment_sizes is host
+ * memory), stage 0 C0 = alpha0 op(A0) op(B0) (beta0 must be zero), then
+ * stage 1 C1 = alpha1 op(C0) op(AB1) + beta1 C1 (C0_left != 0) or
+ * alpha1 op(AB1) op(C0) + beta1 C1; lda1 / ldb1 are the stage-1 left /
+ * right leading dimensions. ----------------------------------------------- */
+int ddla_s_gemm_vbatched2s(void* handle, char transA_0, char transB_0, int* d_m_0, int* d_n_0, int* d_k_0,
+                           const float* alpha_0, const void* const* d_A_0, int* d_lda_0, const void* const* d_B_0,
+                           int* d_ldb_0, const float* beta_0, void** d_C_0, int* d_ldc_0, char transA_1,
+                           char transB_1, int* d_m_1, int* d_n_1, int* d_k_1, const float* alpha_1,
+                           const void* const* d_AB_1, int* d_lda_1, int* d_ldb_1, const float* beta_1, void** d_C_1,
+                           int* d_ldc_1, int C0_left, int batch_count, const int* segment_sizes, int segment_count);
+int ddla_d_gemm_vbatched2s(void* handle, char transA_0, char transB_0, int* d_m_0, int* d_n_0, int* d_k_0,
+                           const double* alpha_0, const void* const* d_A_0, int* d_lda_0, const void* const* d_B_0,
+                           int* d_ldb_0, const double* beta_0, void** d_C_0, int* d_ldc_0, char transA_1,
+                           char transB_1, int* d_m_1, int* d_n_1, int* d_k_1, const double* alpha_1,
+                           const void* const* d_AB_1, int* d_lda_1, int* d_ldb_1, const double* beta_1,
+                           void** d_C_1, int* d_ldc_1, int C0_left, int batch_count, const int* segment_sizes,
+                           int segment_count);
+
+/* ---- batched SPD solves of uneven order (no counterpart in the reference;
+ * per member the semantics of pposv and 2 * Re diag_logdet of the factor):
+ * factor A_i (order d_n[i] <= max_n <= 1024, triangle uplo), solve A_i X_i =
+ * B_i (d_n[i] x nrhs) in place when the factorization succeeds, and write
+ * d_logdet[i] = log det A_i; d_info[i] = 0, the failed pivot (B_i untouched,
+ * logdet NaN) or -1 (invalid order / leading dimension).  Arrays are device
+ * resident; asynchronous on the handle's stream.  _run allocates and
+ * synchronizes nothing (graph-capturable); _tune times the schedules on
+ * copies of the inputs and keeps the fastest in the handle; _graph captures
+ * one call into a replayable hipGraph owning its workspace. -------------- */
+int ddla_s_pposv_vbatched(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, int nrhs,
+                          void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch_count,
+                          int max_n);
+int ddla_d_pposv_vbatched(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, int nrhs,
+                          void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch_count,
+                          int max_n);
+int ddla_s_pposv_vbatched_workspace_size(void* handle, int batch_count, int max_n, long long* backend_bytes);
+int ddla_d_pposv_vbatched_workspace_size(void* handle, int batch_count, int max_n, long long* backend_bytes);
+int ddla_s_pposv_vbatched_run(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, int nrhs,
+                              void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch_count,
+                              int max_n, void* backend_work, long long backend_bytes);
+int ddla_d_pposv_vbatched_run(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, int nrhs,
+                              void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch_count,
+                              int max_n, void* backend_work, long long backend_bytes);
+int ddla_s_pposv_vbatched_tune(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, int nrhs,
+                               void* const* d_B, const int* d_ldb, int batch_count, int max_n, int* occupancy,
+                               int* ordered);
+int ddla_d_pposv_vbatched_tune(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, int nrhs,
+                               void* const* d_B, const int* d_ldb, int batch_count, int max_n, int* occupancy,
+                               int* ordered);
+int ddla_s_pposv_vbatched_graph(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda,
+                                int nrhs, void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet,
+                                int batch_count, int max_n, void** graph);
+int ddla_d_pposv_vbatched_graph(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda,
+                                int nrhs, void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet,
+                                int batch_count, int max_n, void** graph);
+int ddla_c_graph_launch(void* graph, void* handle);
+int ddla_c_graph_destroy(void* graph);
+
 /* ---- harness helpers (no counterpart in the reference) ------------------
  * Philox4x64-10 + Box-Muller normals of the project's seed convention
  * (element e of stream s: counter {e/4, s, 0, 0}, key {seed, 0}), and the
@@ -146,6 +205,12 @@ int ddla_c_fill_normal(void* handle, unsigned long long seed, unsigned long long
                        unsigned long long first, long long count, double* d_out);
 int ddla_d_spd_generate(void* handle, int n, unsigned long long seed, double delta, double* A, int lda,
                         double* work);
+/* Batch of the same convention: member i (host arrays n, A, lda, B, ldb of
+ * device pointers) gets A_i = G_i G_i^T / n_i + delta I, G_i the normals of
+ * Philox stream 2i, and B_i (n_i x nrhs) the normals of stream 2i + 1, formed
+ * in FP64 and stored as float when fp32 != 0.  work: 2 max(n_i)^2 doubles. */
+int ddla_c_spd_generate_batch(void* handle, int batch, const int* n, unsigned long long seed, double delta, int fp32,
+                              void* const* A, const int* lda, int nrhs, void* const* B, const int* ldb, double* work);
 
 /* ---- per-launch event profiler (no counterpart in the reference) --------
  * enable = 1 starts recording an event pair around every kernel launch of

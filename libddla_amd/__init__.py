@@ -17,6 +17,14 @@ from ._capi import (
     lib,
 )
 from .backend import Backend, Handle
+from .batched import (
+    BatchedGraph,
+    VBatch,
+    pposv_vbatched,
+    pposv_vbatched_run,
+    pposv_vbatched_tune,
+    pposv_vbatched_workspace_size,
+)
 from .matrix import DistributedMatrix, descinit
 from .ops import (
     diag_logdet,
@@ -37,7 +45,8 @@ __all__ = [
     "LIBRARY_PATH", "lib", "check_status", "Backend", "Handle", "DistributedMatrix", "descinit",
     "ppotrf", "ppotrs", "pposv", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
     "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
-    "diag_logdet", "DdlaError", "DdlaInvalidHandleError",
+    "diag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",
+    "pposv_vbatched_workspace_size", "DdlaError", "DdlaInvalidHandleError",
     "DdlaInvalidValueError", "DdlaAllocFailedError", "DdlaBackendUnavailableError", "DdlaInternalError",
     "DdlaExceptionError",
 ]

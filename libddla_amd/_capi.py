@@ -96,6 +96,10 @@ _SIGNATURES = {
     "ddla_d_pposv_mixed_workspace_size": (_i, [_vp, _c, _i, _i, _ip, _ip, _llp, _llp]),
     "ddla_d_pposv_mixed_run": (_i, [_vp, _c, _i, _i, _vp, _ip, _vp, _ip, _i, ctypes.c_double, _i, _ip, _ip, _vp,
                                     _ll]),
+    "ddla_c_spd_generate_batch": (_i, [_vp, _i, _ip, ctypes.c_ulonglong, ctypes.c_double, _i, _vp, _ip, _i, _vp,
+                                       _ip, _vp]),
+    "ddla_c_graph_launch": (_i, [_vp, _vp]),
+    "ddla_c_graph_destroy": (_i, [_vp]),
     "ddla_c_profile": (_i, [_vp, _i]),
     "ddla_c_profile_read": (_i, [_vp, _i, _ip, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_double)]),
@@ -113,8 +117,17 @@ for _p in ("s", "d"):
         f"ddla_{_p}_pposv_run": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _i, _i, _ip, _vp, _i, _i, _ip, _ip, _vp, _ll,
                                       _vp, _ll]),
         f"ddla_{_p}_potrf": (_i, [_vp, _c, _i, _vp, _i, _vp]),
+        f"ddla_{_p}_pposv_vbatched": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i]),
+        f"ddla_{_p}_pposv_vbatched_workspace_size": (_i, [_vp, _i, _i, _llp]),
+        f"ddla_{_p}_pposv_vbatched_run": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _ll]),
+        f"ddla_{_p}_pposv_vbatched_tune": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _ip, _ip]),
+        f"ddla_{_p}_pposv_vbatched_graph": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i,
+                                                 ctypes.POINTER(_vp)]),
         f"ddla_{_p}_diag_logdet": (_i, [_vp, _i, _vp, _i, _vp]),
         f"ddla_{_p}_gemm_vbatched": (_i, [_vp, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+        f"ddla_{_p}_gemm_vbatched2s": (_i, [_vp, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _ip,
+                                            _i]),
     })
 
 for _name, (_res, _args) in _SIGNATURES.items():

@@ -1,0 +1,453 @@
+// Batched Cholesky solve of many independent SPD systems of uneven order in
+// one persistent launch (device code, included by kernels.hip).
+//
+// A grid of resident workgroups pulls matrix indices from an atomic queue, in
+// the order given by `order` (largest order first, from order_by_blocks), so
+// the long factorizations start first and the short ones fill the tail.  For
+// matrix i (order n, stored triangle uplo), one 256-thread workgroup:
+//   * factors it blocked right-looking on 64 x 64 blocks: the diagonal block
+//     in LDS (potf2_tile.h), the blocks below it as A(r, j) D_j^T and the
+//     trailing lower triangle A(r, c) -= L(r, j) L(c, j)^T on the matrix
+//     cores, all in place in global memory (L2-resident at these orders);
+//     D_j = L(j, j)^-1 goes to the workgroup's slot of xinv;
+//   * if every pivot is positive, solves A X = B in place (forward sweep
+//     y_j = D_j (b_j - sum_{i<j} L(j, i) y_i), backward sweep
+//     x_j = D_j^T (y_j - sum_{r>j} L(r, j)^T x_r)) in 16-column slabs;
+//   * writes logdet[i] = 2 sum_k log L(k, k) (double, fixed summation order)
+//     and info[i] = 0; on a failed pivot p, info[i] = p (1-based), B is left
+//     untouched and logdet[i] = NaN.  An order above 64 * max_blocks, a
+//     negative order or a leading dimension below max(1, n) gives info[i] =
+//     -1, logdet[i] = NaN and touches nothing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gemm_tile.h"
+#include "potf2_tile.h"
+
+namespace ddla {
+namespace kern {
+namespace batched_detail {
+
+using gemm_detail::Mfma;
+using potf2_detail::kB;
+using potf2_detail::kL;
+
+constexpr int kMaxBlocks = 16;  // orders up to 1024
+constexpr int kThreads = 256;
+constexpr int kPer = kB * kB / kThreads;
+constexpr int kSlab = 16;       // right-hand-side columns per solve pass
+
+template <class T>
+struct BatchArgs {
+    int batch, nrhs, upper, max_blocks;
+    const int* n;
+    T* const* A;
+    const int* lda;
+    T* const* B;
+    const int* ldb;
+    int* info;
+    double* logdet;
+    const int* order;  // processing order (a permutation of [0, batch))
+    int* queue;        // work counter, zero on entry
+    T* xinv;           // gridDim.x slots of max_blocks 64 x 64 inverses
+};
+
+struct View {
+    long long lda;
+    int upper;
+};
+
+// buf[c * kL + r] = L(i0 + r, j0 + c) (L view of a stored triangle), zero outside rows x cols.
+template <class T>
+__device__ inline void load_view_block(const T* A, View v, long long i0, long long j0, int rows, int cols, T* buf)
+{
+    T x[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e, fast = idx & (kB - 1), slow = idx >> 6;
+        const int r = v.upper ? slow : fast, c = v.upper ? fast : slow;
+        x[e] = T(0);
+        if (r < rows && c < cols)
+            x[e] = v.upper ? A[(j0 + c) + (i0 + r) * v.lda] : A[(i0 + r) + (j0 + c) * v.lda];
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e, fast = idx & (kB - 1), slow = idx >> 6;
+        const int r = v.upper ? slow : fast, c = v.upper ? fast : slow;
+        buf[c * kL + r] = x[e];
+    }
+}
+
+// L(i0 + r, j0 + c) = buf[c * kL + r] for r < rows, c < cols (and r >= c when lower_only).
+template <class T>
+__device__ inline void store_view_block(T* A, View v, long long i0, long long j0, int rows, int cols,
+                                        bool lower_only, const T* buf)
+{
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = threadIdx.x + kThreads * e, fast = idx & (kB - 1), slow = idx >> 6;
+        const int r = v.upper ? slow : fast, c = v.upper ? fast : slow;
+        if (r < rows && c < cols && (!lower_only || r >= c)) {
+            if (v.upper) A[(j0 + c) + (i0 + r) * v.lda] = buf[c * kL + r];
+            else A[(i0 + r) + (j0 + c) * v.lda] = buf[c * kL + r];
+        }
+    }
+}
+
+// 64 x 64 accumulator tile of the workgroup (layout as in panel_tile.h).
+template <class T>
+struct Tile {
+    typename Mfma<T>::acc_t v[2][2];
+    __device__ void load(const T* buf)
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int ro = 32 * (wave >> 1), co = 32 * (wave & 1);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    v[i][jj][e] = buf[(co + 16 * jj + (lane & 15)) * kL + ro + 16 * i + Mfma<T>::row(lane, e)];
+    }
+    __device__ void store(T* buf) const
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int ro = 32 * (wave >> 1), co = 32 * (wave & 1);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    buf[(co + 16 * jj + (lane & 15)) * kL + ro + 16 * i + Mfma<T>::row(lane, e)] = v[i][jj][e];
+    }
+    __device__ void zero()
+    {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) v[i][jj] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    }
+    // += sign * As Bs^T, As(row, k) = As[k * kL + row], Bs(col, k) = Bs[k * kL + col]
+    __device__ void mma_nt(const T* As, const T* Bs, T sign)
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+        const int ro = 32 * (wave >> 1), co = 32 * (wave & 1);
+#pragma unroll 4
+        for (int ks = 0; ks < kB; ks += 4) {
+            const int kk = (ks + lk) * kL;
+            const T a0 = sign * As[kk + ro + li], a1 = sign * As[kk + ro + 16 + li];
+            const T b0 = Bs[kk + co + li], b1 = Bs[kk + co + 16 + li];
+            v[0][0] = Mfma<T>::mma(a0, b0, v[0][0]);
+            v[0][1] = Mfma<T>::mma(a0, b1, v[0][1]);
+            v[1][0] = Mfma<T>::mma(a1, b0, v[1][0]);
+            v[1][1] = Mfma<T>::mma(a1, b1, v[1][1]);
+        }
+    }
+};
+
+// out(r, c) = sum_k M(r, k) V(k, c) for r < rows, k < cols, c < nc (nc <=
+// kSlab): thread (r, q) = (tid % 64, tid / 64) sums k in [16q, 16q + 16)
+// with M(r, k) = m[r * rs + k * ks] read from global memory (coalesced along
+// r when rs = 1) and V(k, c) = V[c * kB + k] from LDS, then the four partial
+// sums are added in a fixed order.  part holds kSlab x 4 x kB elements of LDS
+// scratch; thread (r, q) receives column c = q + 4 s in out[s].
+template <class T>
+__device__ inline void block_apply(const T* m, long long rs, long long ks, int rows, int cols, const T* V, int nc,
+                                   T* part, T (&out)[kSlab / 4])
+{
+    const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+    T mv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        mv[k] = (r < rows && 16 * q + k < cols) ? m[r * rs + (long long)(16 * q + k) * ks] : T(0);
+    for (int c = 0; c < nc; ++c) {
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s = fma(mv[k], V[c * kB + 16 * q + k], s);
+        part[(c * 4 + q) * kB + r] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kSlab / 4; ++s) {
+        const int c = q + 4 * s;
+        out[s] = c < nc ? ((part[(c * 4 + 0) * kB + r] + part[(c * 4 + 1) * kB + r]) +
+                           (part[(c * 4 + 2) * kB + r] + part[(c * 4 + 3) * kB + r]))
+                        : T(0);
+    }
+    __syncthreads();
+}
+
+// OCC = resident workgroups per CU the register allocation is sized for (1:
+// no spills; 2: two workgroups per CU share each SIMD, at the cost of a few
+// spilled registers); the autotuner picks one.
+template <class T, int OCC>
+__global__ __launch_bounds__(kThreads, OCC) void batched_pposv_kernel(BatchArgs<T> a)
+{
+    __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
+    __shared__ int s_item, s_flag;
+    static_assert(kSlab * 4 * kB <= kB * kL && kSlab * kB <= kB * kL, "solve scratch must fit the factor buffers");
+    T* const part = S;  // the solves reuse the factorization's buffers
+    T* const V = X;
+    const int tid = threadIdx.x;
+    T* const slot = a.xinv + (long long)blockIdx.x * a.max_blocks * kB * kB;
+
+    for (;;) {
+        if (tid == 0) {
+            const int q = atomicAdd(a.queue, 1);
+            s_item = q < a.batch ? a.order[q] : -1;
+        }
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item < 0) return;
+
+        const int n = a.n[item];
+        const long long lda = a.lda[item], ldb = a.ldb[item];
+        const int nb = (n + kB - 1) / kB;
+        if (n < 0 || n > a.max_blocks * kB || lda < (n > 1 ? n : 1) || (a.nrhs > 0 && ldb < (n > 1 ? n : 1))) {
+            if (tid == 0) {
+                a.info[item] = -1;
+                a.logdet[item] = __builtin_nan("");
+            }
+            continue;
+        }
+        T* A = a.A[item];
+        const View v{lda, a.upper};
+        int fail = 0;
+        double logsum = 0.0;  // meaningful in thread 0
+
+        for (int j = 0; j < nb && fail == 0; ++j) {
+            const long long j0 = (long long)kB * j;
+            const int wj = min(kB, n - (int)j0);
+            T* Ajj = A + j0 + j0 * lda;
+            potf2_detail::load_block<T>(a.upper != 0, wj, Ajj, lda, S);
+            __syncthreads();
+            const int f = potf2_detail::factor_invert64<T>(S, X, rdiag, &s_flag);
+            potf2_detail::store_factor<T>(a.upper != 0, wj, S, Ajj, lda);
+            if (f != 0) {
+                fail = (int)j0 + f;
+                break;
+            }
+            // log of the diagonal: wave 0, fixed-order butterfly, then thread 0 in block order
+            if (tid < 64) {
+                double l = tid < wj ? log((double)S[tid * kL + tid]) : 0.0;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+                if (tid == 0) logsum += l;
+            }
+            // D_j with exact zeros above the diagonal: kept in X for the panel, saved for the solves
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int idx = tid + kThreads * e, r = idx & (kB - 1), c = idx >> 6;
+                if (r < c) X[c * kL + r] = T(0);
+            }
+            __syncthreads();
+            potf2_detail::store_inverse<T>(X, slot + (long long)j * kB * kB);
+            // blocks below: L(r, j) = A(r, j) D_j^T
+            for (int r = j + 1; r < nb; ++r) {
+                const long long r0 = (long long)kB * r;
+                const int wr = min(kB, n - (int)r0);
+                load_view_block<T>(A, v, r0, j0, wr, wj, S);
+                __syncthreads();
+                Tile<T> l;
+                l.zero();
+                l.mma_nt(S, X, T(1));
+                __syncthreads();
+                l.store(S);
+                __syncthreads();
+                store_view_block<T>(A, v, r0, j0, wr, wj, false, S);
+                __syncthreads();
+            }
+            // trailing update: A(r, c) -= L(r, j) L(c, j)^T, j < c <= r
+            for (int c = j + 1; c < nb; ++c) {
+                const long long c0 = (long long)kB * c;
+                const int wc = min(kB, n - (int)c0);
+                __syncthreads();
+                load_view_block<T>(A, v, c0, j0, wc, wj, X);  // L(c, j)
+                for (int r = c; r < nb; ++r) {
+                    const long long r0 = (long long)kB * r;
+                    const int wr = min(kB, n - (int)r0);
+                    load_view_block<T>(A, v, r0, c0, wr, wc, S);  // A(r, c)
+                    __syncthreads();
+                    Tile<T> acc;
+                    acc.load(S);
+                    __syncthreads();
+                    load_view_block<T>(A, v, r0, j0, wr, wj, S);  // L(r, j)
+                    __syncthreads();
+                    acc.mma_nt(S, X, T(-1));
+                    __syncthreads();
+                    acc.store(S);
+                    __syncthreads();
+                    store_view_block<T>(A, v, r0, c0, wr, wc, r == c, S);
+                    __syncthreads();
+                }
+            }
+        }
+        __syncthreads();
+        if (fail != 0) {
+            if (tid == 0) {
+                a.info[item] = fail;
+                a.logdet[item] = __builtin_nan("");
+            }
+            continue;
+        }
+
+        // solves, 16 right-hand-side columns at a time
+        T* Bm = a.B[item];
+        for (int c0 = 0; c0 < a.nrhs; c0 += kSlab) {
+            const int nc = min(kSlab, a.nrhs - c0);
+            // forward: y_j = D_j (b_j - sum_{i<j} L(j, i) y_i), y written over b
+            for (int j = 0; j < nb; ++j) {
+                const long long j0 = (long long)kB * j;
+                const int wj = min(kB, n - (int)j0);
+                T acc[kSlab / 4];
+                {
+                    const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) {
+                        const int c = q + 4 * s;
+                        acc[s] = (r < wj && c < nc) ? Bm[(j0 + r) + (c0 + c) * ldb] : T(0);
+                    }
+                }
+                for (int i = 0; i < j; ++i) {
+                    const long long i0 = (long long)kB * i;
+                    // V = y_i
+                    const int wi = min(kB, n - (int)i0);
+                    for (int e = tid; e < kB * kSlab; e += kThreads) {
+                        const int k = e & (kB - 1), c = e >> 6;
+                        V[c * kB + k] = (c < nc && k < wi) ? Bm[(i0 + k) + (c0 + c) * ldb] : T(0);
+                    }
+                    __syncthreads();
+                    T t[kSlab / 4];
+                    // M = L(j, i): element (r, k) at (j0 + r, i0 + k) of the L view
+                    const T* m = a.upper ? A + i0 + j0 * lda : A + j0 + i0 * lda;
+                    block_apply<T>(m, a.upper ? lda : 1, a.upper ? 1 : lda, wj, wi, V, nc, part, t);
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) acc[s] -= t[s];
+                }
+                // V = acc, then y_j = D_j V
+                {
+                    const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) {
+                        const int c = q + 4 * s;
+                        if (c < kSlab) V[c * kB + r] = acc[s];
+                    }
+                }
+                __syncthreads();
+                T y[kSlab / 4];
+                block_apply<T>(slot + j0 * kB, 1, kB, kB, kB, V, nc, part, y);
+                {
+                    const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) {
+                        const int c = q + 4 * s;
+                        if (r < wj && c < nc) Bm[(j0 + r) + (c0 + c) * ldb] = y[s];
+                    }
+                }
+                __syncthreads();
+            }
+            // backward: x_j = D_j^T (y_j - sum_{r>j} L(r, j)^T x_r)
+            for (int j = nb - 1; j >= 0; --j) {
+                const long long j0 = (long long)kB * j;
+                const int wj = min(kB, n - (int)j0);
+                T acc[kSlab / 4];
+                {
+                    const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) {
+                        const int c = q + 4 * s;
+                        acc[s] = (r < wj && c < nc) ? Bm[(j0 + r) + (c0 + c) * ldb] : T(0);
+                    }
+                }
+                for (int i = j + 1; i < nb; ++i) {
+                    const long long i0 = (long long)kB * i;
+                    const int wi = min(kB, n - (int)i0);
+                    for (int e = tid; e < kB * kSlab; e += kThreads) {
+                        const int k = e & (kB - 1), c = e >> 6;
+                        V[c * kB + k] = (c < nc && k < wi) ? Bm[(i0 + k) + (c0 + c) * ldb] : T(0);
+                    }
+                    __syncthreads();
+                    T t[kSlab / 4];
+                    // M = L(i, j)^T: element (r, k) = L(i0 + k, j0 + r)
+                    const T* m = a.upper ? A + j0 + i0 * lda : A + i0 + j0 * lda;
+                    block_apply<T>(m, a.upper ? 1 : lda, a.upper ? lda : 1, wj, wi, V, nc, part, t);
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) acc[s] -= t[s];
+                }
+                {
+                    const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) {
+                        const int c = q + 4 * s;
+                        if (c < kSlab) V[c * kB + r] = acc[s];
+                    }
+                }
+                __syncthreads();
+                T x[kSlab / 4];
+                // D_j^T: element (r, k) = D_j(k, r) = slot[j][k + 64 r]
+                block_apply<T>(slot + j0 * kB, kB, 1, kB, kB, V, nc, part, x);
+                {
+                    const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                    for (int s = 0; s < kSlab / 4; ++s) {
+                        const int c = q + 4 * s;
+                        if (r < wj && c < nc) Bm[(j0 + r) + (c0 + c) * ldb] = x[s];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            a.info[item] = 0;
+            a.logdet[item] = 2.0 * logsum;
+        }
+    }
+}
+
+// order[] = the matrix indices sorted by block count, largest first, stable
+// (one 256-thread workgroup: per-thread bucket counts over a contiguous index
+// range, an exclusive scan in bucket-major order, then a scatter in index
+// order).  Orders outside [0, 64 * max_blocks] go last.
+constexpr int kOrderThreads = 256;
+__global__ __launch_bounds__(kOrderThreads) void order_by_blocks_kernel(int batch, int max_blocks, const int* n,
+                                                                        int* order)
+{
+    constexpr int kBuckets = kMaxBlocks + 2;
+    __shared__ int cnt[kBuckets][kOrderThreads];
+    __shared__ int total[kBuckets];
+    const int tid = threadIdx.x;
+    const int per = (batch + kOrderThreads - 1) / kOrderThreads;
+    const int lo = min(batch, tid * per), hi = min(batch, lo + per);
+    auto bucket = [&](int i) {
+        const int m = n[i];
+        if (m < 0 || m > max_blocks * kB) return kBuckets - 1;
+        return kMaxBlocks - (m + kB - 1) / kB;  // 16 blocks -> 0, an empty matrix -> 16
+    };
+    for (int b = 0; b < kBuckets; ++b) cnt[b][tid] = 0;
+    for (int i = lo; i < hi; ++i) ++cnt[bucket(i)][tid];
+    __syncthreads();
+    if (tid < kBuckets) {
+        int s = 0;
+        for (int t = 0; t < kOrderThreads; ++t) {
+            const int c = cnt[tid][t];
+            cnt[tid][t] = s;
+            s += c;
+        }
+        total[tid] = s;
+    }
+    __syncthreads();
+    if (tid < kBuckets) {
+        int base = 0;
+        for (int b = 0; b < tid; ++b) base += total[b];
+        for (int t = 0; t < kOrderThreads; ++t) cnt[tid][t] += base;
+    }
+    __syncthreads();
+    for (int i = lo; i < hi; ++i) order[cnt[bucket(i)][tid]++] = i;
+}
+
+} // namespace batched_detail
+} // namespace kern
+} // namespace ddla

@@ -312,11 +312,120 @@ void* ddla_c_get_stream(void* handle)
                                   reinterpret_cast<T**>(d_C), d_ldc, batch_count);                             \
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_gemm_vbatched2s(                                                                            \
+        void* handle, char ta0, char tb0, int* d_m0, int* d_n0, int* d_k0, const T* alpha0,                    \
+        const void* const* d_A0, int* d_lda0, const void* const* d_B0, int* d_ldb0, const T* beta0,            \
+        void** d_C0, int* d_ldc0, char ta1, char tb1, int* d_m1, int* d_n1, int* d_k1, const T* alpha1,        \
+        const void* const* d_AB1, int* d_lda1, int* d_ldb1, const T* beta1, void** d_C1, int* d_ldc1,          \
+        int C0_left, int batch_count, const int* segment_sizes, int segment_count)                             \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (!alpha0 || !beta0 || !alpha1 || !beta1) return DDLA_C_STATUS_INVALID_VALUE;                    \
+            ddla::gemmVbatched2s<T>(h, ta0, tb0, d_m0, d_n0, d_k0, *alpha0,                                    \
+                                    reinterpret_cast<const T* const*>(d_A0), d_lda0,                           \
+                                    reinterpret_cast<const T* const*>(d_B0), d_ldb0, *beta0,                   \
+                                    reinterpret_cast<T**>(d_C0), d_ldc0, ta1, tb1, d_m1, d_n1, d_k1, *alpha1,  \
+                                    reinterpret_cast<const T* const*>(d_AB1), d_lda1, d_ldb1, *beta1,          \
+                                    reinterpret_cast<T**>(d_C1), d_ldc1, C0_left != 0, batch_count,            \
+                                    segment_sizes, segment_count);                                             \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
     }
 
 DDLA_C_DRIVERS(s, float)
 DDLA_C_DRIVERS(d, double)
 #undef DDLA_C_DRIVERS
+
+// ---- batched SPD solves --------------------------------------------------------
+#define DDLA_C_BATCHED(P, T)                                                                                   \
+    int ddla_##P##_pposv_vbatched(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, \
+                                  int nrhs, void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, \
+                                  int batch_count, int max_n)                                                  \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::pposvVbatched<T>(h, uplo, d_n, reinterpret_cast<T* const*>(d_A), d_lda, nrhs,                \
+                                   reinterpret_cast<T* const*>(d_B), d_ldb, d_info, d_logdet, batch_count,     \
+                                   max_n);                                                                     \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pposv_vbatched_workspace_size(void* handle, int batch_count, int max_n,                     \
+                                                 long long* backend_bytes)                                     \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (backend_bytes == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                  \
+            if (batch_count < 0 || max_n < 0 || max_n > 1024) return DDLA_C_STATUS_INVALID_VALUE;              \
+            std::size_t b = 0;                                                                                 \
+            ddla::cholesky_impl::pposv_vbatched_workspace_size<T>(h, batch_count, max_n, &b);                  \
+            *backend_bytes = (long long)b;                                                                     \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pposv_vbatched_run(void* handle, char uplo, const int* d_n, void* const* d_A,               \
+                                      const int* d_lda, int nrhs, void* const* d_B, const int* d_ldb,          \
+                                      int* d_info, double* d_logdet, int batch_count, int max_n,               \
+                                      void* backend_work, long long backend_bytes)                             \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (backend_bytes < 0) return DDLA_C_STATUS_INVALID_VALUE;                                         \
+            ddla::cholesky_impl::pposv_vbatched_check(h, uplo, batch_count, nrhs, max_n, d_n, d_A, d_lda, d_B, \
+                                                      d_ldb, d_info, d_logdet);                                \
+            ddla::cholesky_impl::pposv_vbatched_run<T>(h, uplo, d_n, reinterpret_cast<T* const*>(d_A), d_lda,  \
+                                                       nrhs, reinterpret_cast<T* const*>(d_B), d_ldb, d_info,  \
+                                                       d_logdet, batch_count, max_n, backend_work,             \
+                                                       (std::size_t)backend_bytes);                            \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pposv_vbatched_tune(void* handle, char uplo, const int* d_n, void* const* d_A,              \
+                                       const int* d_lda, int nrhs, void* const* d_B, const int* d_ldb,         \
+                                       int batch_count, int max_n, int* occupancy, int* ordered)               \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            int occ = 0;                                                                                       \
+            bool ord = false;                                                                                  \
+            ddla::pposvVbatchedTune<T>(h, uplo, d_n, reinterpret_cast<T* const*>(d_A), d_lda, nrhs,            \
+                                       reinterpret_cast<T* const*>(d_B), d_ldb, batch_count, max_n, &occ,      \
+                                       &ord);                                                                  \
+            if (occupancy) *occupancy = occ;                                                                   \
+            if (ordered) *ordered = ord ? 1 : 0;                                                               \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pposv_vbatched_graph(void* handle, char uplo, const int* d_n, void* const* d_A,             \
+                                        const int* d_lda, int nrhs, void* const* d_B, const int* d_ldb,        \
+                                        int* d_info, double* d_logdet, int batch_count, int max_n,             \
+                                        void** graph)                                                          \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (graph == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                          \
+            *graph = ddla::pposvVbatchedGraph<T>(h, uplo, d_n, reinterpret_cast<T* const*>(d_A), d_lda, nrhs,  \
+                                                 reinterpret_cast<T* const*>(d_B), d_ldb, d_info, d_logdet,    \
+                                                 batch_count, max_n);                                          \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }
+DDLA_C_BATCHED(s, float)
+DDLA_C_BATCHED(d, double)
+#undef DDLA_C_BATCHED
+
+int ddla_c_graph_launch(void* graph, void* handle)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if (graph == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+        ddla::batchedGraphLaunch(graph, h);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_graph_destroy(void* graph)
+{
+    return guarded([&]() -> int {
+        ddla::batchedGraphDestroy(graph);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
 
 // ---- mixed-precision pposv -----------------------------------------------------
 int ddla_d_pposv_mixed(void* handle, char uplo, int n, int nrhs, void* A, int ia, int ja, const int* descA, void* B,
@@ -379,6 +488,43 @@ int ddla_d_spd_generate(void* handle, int n, unsigned long long seed, double del
         ddla::kern::fill_normal(h->stream, seed, 0, 0, (long long)n * n, work);
         ddla::kern::gemm<double>(h->stream, 'N', 'T', n, n, n, 1.0 / n, work, n, work, n, 0.0, A, lda);
         ddla::kern::add_diagonal<double>(h->stream, n, A, lda, delta);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_spd_generate_batch(void* handle, int batch, const int* n, unsigned long long seed, double delta, int fp32,
+                              void* const* A, const int* lda, int nrhs, void* const* B, const int* ldb, double* work)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        ddla::require_gpu(h, "spd_generate_batch");
+        if (batch < 0 || nrhs < 0 || (batch > 0 && (!n || !A || !lda || !work || (nrhs > 0 && (!B || !ldb)))))
+            return DDLA_C_STATUS_INVALID_VALUE;
+        for (int i = 0; i < batch; ++i) {
+            const int m = n[i];
+            if (m < 0 || lda[i] < std::max(1, m) || (nrhs > 0 && ldb[i] < std::max(1, m)))
+                return DDLA_C_STATUS_INVALID_VALUE;
+            if (m == 0) continue;
+            const long long mm = (long long)m * m;
+            double* g = work;
+            double* a64 = fp32 ? work + mm : static_cast<double*>(A[i]);
+            const int ld64 = fp32 ? m : lda[i];
+            ddla::kern::fill_normal(h->stream, seed, 2ull * i, 0, mm, g);
+            ddla::kern::gemm<double>(h->stream, 'N', 'T', m, m, m, 1.0 / m, g, m, g, m, 0.0, a64, ld64);
+            ddla::kern::add_diagonal<double>(h->stream, m, a64, ld64, delta);
+            if (fp32) ddla::kern::convert<float, double>(h->stream, m, m, a64, ld64, static_cast<float*>(A[i]), lda[i]);
+            if (nrhs > 0) {
+                double* b64 = fp32 ? g : static_cast<double*>(B[i]);
+                const int ldb64 = fp32 ? m : ldb[i];
+                if (ldb64 == m || fp32) {
+                    ddla::kern::fill_normal(h->stream, seed, 2ull * i + 1, 0, (long long)m * nrhs, b64);
+                } else {
+                    ddla::kern::fill_normal(h->stream, seed, 2ull * i + 1, 0, (long long)m * nrhs, g);
+                    ddla::kern::convert<double, double>(h->stream, m, nrhs, g, m, b64, ldb64);
+                }
+                if (fp32)
+                    ddla::kern::convert<float, double>(h->stream, m, nrhs, b64, m, static_cast<float*>(B[i]), ldb[i]);
+            }
+        }
         return DDLA_C_STATUS_SUCCESS;
     });
 }

@@ -383,33 +383,101 @@ void diag_logdet(const DdlaHandle_t& h, int n, const T* a, int lda, std::complex
     result = {static_cast<R>(out.magnitude), static_cast<R>(phase)};
 }
 
+namespace {
+
+bool valid_op(char op) { return op == 'N' || op == 'n' || op == 'T' || op == 't' || op == 'C' || op == 'c'; }
+bool is_n_op(char op) { return op == 'N' || op == 'n'; }
+
+void check_vbatched_handle(const DdlaHandle_t& h, const char* who)
+{
+    if (h == nullptr || !h->initialized || h->backend != DdlaBackend::GPU)
+        throw std::invalid_argument(std::string(who) + ": invalid DDLA handle");
+}
+
+// Validates one stage's device dimension arrays (one reduction kernel and a
+// host read of its 4-int summary) and returns the maxima of m, n, k.
+void vbatched_maxima(const DdlaHandle_t& h, const char* who, char ta, char tb, const int* m, const int* n,
+                     const int* k, const int* lda, const int* ldb, const int* ldc, int batch, int* summary, int out[3])
+{
+    if (!m || !n || !k || !lda || !ldb || !ldc) throw std::invalid_argument(std::string(who) + ": null dimension array");
+    kern::vbatched_validate(h->stream, is_n_op(ta), is_n_op(tb), m, n, k, lda, ldb, ldc, batch, summary);
+    int host[4] = {0, 0, 0, 0};
+    DDLA_HIP(hipMemcpyAsync(host, summary, sizeof(host), hipMemcpyDeviceToHost, h->stream));
+    DDLA_HIP(hipStreamSynchronize(h->stream));
+    if (host[3] != 0)
+        throw std::invalid_argument(std::string(who) + ": invalid dimensions at batch " + std::to_string(host[3] - 1));
+    out[0] = host[0];
+    out[1] = host[1];
+    out[2] = host[2];
+}
+
+} // namespace
+
 template <class T>
 void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* d_m, int* d_n, int* d_k, T alpha,
                   const T* const* d_A, int* d_lda, const T* const* d_B, int* d_ldb, T beta, T** d_C, int* d_ldc,
                   int batch)
 {
-    if (h == nullptr || !h->initialized || h->backend != DdlaBackend::GPU)
-        throw std::invalid_argument("gemmVbatched: invalid DDLA handle");
+    check_vbatched_handle(h, "gemmVbatched");
     if (batch < 0) throw std::invalid_argument("gemmVbatched: negative batch count");
     if (batch == 0) return;
     if (d_A == nullptr || d_B == nullptr || d_C == nullptr)
         throw std::invalid_argument("gemmVbatched: null matrix pointer array");
-    auto valid = [](char op) {
-        return op == 'N' || op == 'n' || op == 'T' || op == 't' || op == 'C' || op == 'c';
-    };
-    if (!valid(transA) || !valid(transB)) throw std::invalid_argument("gemmVbatched: invalid transpose operation");
-    if (!d_m || !d_n || !d_k || !d_lda || !d_ldb || !d_ldc)
-        throw std::invalid_argument("gemmVbatched: null dimension array");
+    if (!valid_op(transA) || !valid_op(transB)) throw std::invalid_argument("gemmVbatched: invalid transpose operation");
     int* summary = static_cast<int*>(h->ensure_scratch(detail::padded_bytes<int>(4)));
-    const bool na = transA == 'N' || transA == 'n', nb = transB == 'N' || transB == 'n';
-    kern::vbatched_validate(h->stream, na, nb, d_m, d_n, d_k, d_lda, d_ldb, d_ldc, batch, summary);
-    int host[4] = {0, 0, 0, 0};
-    DDLA_HIP(hipMemcpyAsync(host, summary, sizeof(host), hipMemcpyDeviceToHost, h->stream));
-    DDLA_HIP(hipStreamSynchronize(h->stream));
-    if (host[3] != 0)
-        throw std::invalid_argument("gemmVbatched: invalid dimensions at batch " + std::to_string(host[3] - 1));
+    int mx[3];
+    vbatched_maxima(h, "gemmVbatched", transA, transB, d_m, d_n, d_k, d_lda, d_ldb, d_ldc, batch, summary, mx);
     kern::gemm_vbatched<T>(h->stream, transA, transB, d_m, d_n, d_k, alpha, d_A, d_lda, d_B, d_ldb, beta, d_C,
-                           d_ldc, batch, host[0], host[1]);
+                           d_ldc, batch, mx[0], mx[1]);
+}
+
+// src/BLAS/gemm_vbatched.cpp:420-523 and
+// src/vbatched/gemm_template_kernel_vbatched.hpp:439-497: per host segment, in
+// order, stage 0 C0 = alpha0 op(A0) op(B0) (beta0 must be zero) and then
+// stage 1 C1 = alpha1 op(C0) op(AB1) + beta1 C1 (C0_left) or
+// alpha1 op(AB1) op(C0) + beta1 C1, with stage-1 leading dimensions lda1 for
+// the left operand and ldb1 for the right one.  Segments may reuse C0
+// storage: stream order runs each segment's two stages before the next.
+template <class T>
+void gemmVbatched2s(const DdlaHandle_t& h, char transA_0, char transB_0, int* d_m_0, int* d_n_0, int* d_k_0,
+                    T alpha_0, const T* const* d_A_0, int* d_lda_0, const T* const* d_B_0, int* d_ldb_0, T beta_0,
+                    T** d_C_0, int* d_ldc_0, char transA_1, char transB_1, int* d_m_1, int* d_n_1, int* d_k_1,
+                    T alpha_1, const T* const* d_AB_1, int* d_lda_1, int* d_ldb_1, T beta_1, T** d_C_1,
+                    int* d_ldc_1, bool C0_left, int batch, const int* segment_sizes, int segment_count)
+{
+    check_vbatched_handle(h, "gemmVbatched2s");
+    if (batch < 0) throw std::invalid_argument("gemmVbatched2s: negative batch count");
+    if (batch == 0) return;
+    if (beta_0 != T(0)) throw std::invalid_argument("gemmVbatched2s: stage-zero beta must be zero");
+    if (segment_sizes == nullptr || segment_count <= 0)
+        throw std::invalid_argument("gemmVbatched2s: missing segment sizes");
+    long long total = 0;
+    for (int s = 0; s < segment_count; ++s) {
+        if (segment_sizes[s] <= 0) throw std::invalid_argument("gemmVbatched2s: segment sizes must be positive");
+        total += segment_sizes[s];
+    }
+    if (total != batch) throw std::invalid_argument("gemmVbatched2s: segment total does not match batch count");
+    if (!d_A_0 || !d_B_0 || !d_C_0 || !d_AB_1 || !d_C_1)
+        throw std::invalid_argument("gemmVbatched2s: null matrix pointer array");
+    if (!valid_op(transA_0) || !valid_op(transB_0) || !valid_op(transA_1) || !valid_op(transB_1))
+        throw std::invalid_argument("gemmVbatched2s: invalid transpose operation");
+    int* summary = static_cast<int*>(h->ensure_scratch(detail::padded_bytes<int>(4)));
+    int mx0[3], mx1[3];
+    vbatched_maxima(h, "gemmVbatched2s", transA_0, transB_0, d_m_0, d_n_0, d_k_0, d_lda_0, d_ldb_0, d_ldc_0, batch,
+                    summary, mx0);
+    vbatched_maxima(h, "gemmVbatched2s", transA_1, transB_1, d_m_1, d_n_1, d_k_1, d_lda_1, d_ldb_1, d_ldc_1, batch,
+                    summary, mx1);
+    const T* const* C0 = const_cast<const T* const*>(d_C_0);
+    for (int s = 0, c = 0; s < segment_count; c += segment_sizes[s], ++s) {
+        const int nb = segment_sizes[s];
+        kern::gemm_vbatched<T>(h->stream, transA_0, transB_0, d_m_0 + c, d_n_0 + c, d_k_0 + c, alpha_0, d_A_0 + c,
+                               d_lda_0 + c, d_B_0 + c, d_ldb_0 + c, beta_0, d_C_0 + c, d_ldc_0 + c, nb, mx0[0],
+                               mx0[1]);
+        const T* const* left = C0_left ? C0 + c : d_AB_1 + c;
+        const T* const* right = C0_left ? d_AB_1 + c : C0 + c;
+        kern::gemm_vbatched<T>(h->stream, transA_1, transB_1, d_m_1 + c, d_n_1 + c, d_k_1 + c, alpha_1, left,
+                               d_lda_1 + c, right, d_ldb_1 + c, beta_1, d_C_1 + c, d_ldc_1 + c, nb, mx1[0], mx1[1]);
+    }
 }
 
 template <class T>
@@ -430,6 +498,9 @@ template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, lo
     template void diag_logdet<T>(const DdlaHandle_t&, int, const T*, int, std::complex<real_type<T>::type>&);  \
     template void gemmVbatched<T>(const DdlaHandle_t&, char, char, int*, int*, int*, T, const T* const*, int*, \
                                   const T* const*, int*, T, T**, int*, int);                                   \
+    template void gemmVbatched2s<T>(const DdlaHandle_t&, char, char, int*, int*, int*, T, const T* const*,     \
+                                    int*, const T* const*, int*, T, T**, int*, char, char, int*, int*, int*, T, \
+                                    const T* const*, int*, int*, T, T**, int*, bool, int, const int*, int);    \
     template void cholesky_impl::ppotrf_workspace_size<T>(const DdlaHandle_t&, char, int, T*, const int*,      \
                                                           std::size_t*, std::size_t*);                         \
     template void cholesky_impl::ppotrf_run<T>(const DdlaHandle_t&, char, int, T*, int, int, const int*, int&, \

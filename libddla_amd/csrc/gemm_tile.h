@@ -319,6 +319,20 @@ __global__ __launch_bounds__(256, WAVES_PER_EU) void gemm_kernel(GemmProblem<T> 
     gemm_tile<T, BM, BN, BK, TA, TB>(p, ti * BM, tj * BN, smem);
 }
 
+// The same tile loop under its own symbol, launched for the triangle-masked
+// (syrk-shaped) trailing updates of the factorization so that kernel traces
+// report them apart from plain products.
+template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
+__global__ __launch_bounds__(256, WAVES_PER_EU) void syrk_update_kernel(GemmProblem<T> p, TileOrder o,
+                                                                         const int* abort)
+{
+    if (abort != nullptr && *abort != 0) return;
+    int ti, tj;
+    if (!map_tile(o, blockIdx.x, ti, tj)) return;
+    __shared__ T smem[lds_elems<BM, BN, BK>()];
+    gemm_tile<T, BM, BN, BK, TA, TB>(p, ti * BM, tj * BN, smem);
+}
+
 template <class T>
 struct VBatch {
     const int *m, *n, *k;

@@ -43,6 +43,16 @@ public:
     hipEvent_t ev_update = nullptr, ev_panel = nullptr;
     int nprows_ = 0, npcols_ = 0, myprow_ = 0, mypcol_ = 0, nprocs = 0;
     char major = 'R';
+    int num_cus = 0;  // compute units of the device (set by ddlaSet)
+
+    // Batched-solve schedule chosen by the autotuner, per element size:
+    // workgroups per CU (kernel variant and grid) and largest-first order.
+    struct BatchedTuning {
+        bool tuned = false;
+        int occupancy = 1;
+        bool ordered = true;
+    };
+    BatchedTuning batched_tuning[2];  // [0] float, [1] double
 
     // Cached device scratch for the public entry points (grown on demand).
     void* scratch = nullptr;

@@ -71,6 +71,26 @@ template <class T>
 void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long long ldl, const T* dinv, T* B,
                   long long ldb, void* work, int* error);
 
+// Batched fused potrf + potrs + logdet of independent matrices (order n[i] <=
+// 64 * max_blocks <= 1024, stored triangle uplo, leading dimension lda[i]; B[i]
+// n[i] x nrhs with ldb[i]) in one persistent launch of `grid` workgroups, after
+// a one-workgroup ordering kernel (largest first) and a counter reset; all of
+// it is stream-ordered and graph-capturable.  info[i] / logdet[i] as in
+// batched_tile.h.  work holds batched_work_bytes(sizeof(T), batch, grid,
+// max_blocks) bytes.  ordered: largest matrices first (else index order);
+// occupancy 1 or 2: the kernel variant sized for that many workgroups per CU.
+constexpr int kBatchedMaxBlocks = 16;
+size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks);
+template <class T>
+void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
+                   T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
+                   int occupancy, void* work);
+// Batched copy of matrix i (rows n[i], columns cols or n[i] when cols < 0)
+// from src[i] (leading dimension lds[i]) to dst[i] (leading dimension ldd[i]).
+template <class T>
+void batched_copy(hipStream_t s, int batch, const int* n, int cols, T* const* src, const int* lds, T* const* dst,
+                  const int* ldd, int max_n);
+
 // diag_logdet reduction over A[i * (lda + 1)], i < n: work must hold
 // logdet_work_bytes(); out (device) receives {sum log|d|, flags} where flags
 // = 1 nonfinite | 2 zero | 4 odd number of negative entries.

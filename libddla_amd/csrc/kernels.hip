@@ -10,6 +10,7 @@
 #include "solve_tile.h"
 #include "potf2_tile.h"
 #include "panel_tile.h"
+#include "batched_tile.h"
 
 #include <cfloat>
 #include <cstdint>
@@ -334,10 +335,20 @@ void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, c
     o.group = 8;
     o.per_xcd = (o.total + 7) / 8;
     const dim3 grid((unsigned)(o.per_xcd * 8)), block(256);
-    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
-    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
-    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
-    else gemm_kernel<T, BM, BN, BK, true, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    if (p.tri != kTriNoneD) {
+        // the trailing updates: C (N T) or its mirror (T N)
+        if (!ta && tb) syrk_update_kernel<T, BM, BN, BK, false, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
+        else if (ta && !tb) syrk_update_kernel<T, BM, BN, BK, true, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
+        else throw std::invalid_argument("gemm: triangle-masked products are N T or T N");
+    } else if (!ta && !tb) {
+        gemm_kernel<T, BM, BN, BK, false, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    } else if (!ta && tb) {
+        gemm_kernel<T, BM, BN, BK, false, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    } else if (ta && !tb) {
+        gemm_kernel<T, BM, BN, BK, true, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    } else {
+        gemm_kernel<T, BM, BN, BK, true, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
+    }
     check_launch("gemm");
 }
 
@@ -443,6 +454,79 @@ void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long l
     if (n <= 0) return;
     trtri_kernel<T><<<(n + kIB - 1) / kIB, 256, 0, s>>>(upper, n, A, lda, inv);
     check_launch("trtri_diag_blocks");
+}
+
+size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks)
+{
+    const size_t inv = (size_t)grid * max_blocks * 64 * 64 * elem;
+    return (inv + 255) / 256 * 256 + ((size_t)batch * sizeof(int) + 255) / 256 * 256 + 256;
+}
+
+namespace {
+__global__ void fill_iota_kernel(int* p, int count)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void batched_copy_kernel(int batch, const int* n, int cols, T* const* src,
+                                                           const int* lds, T* const* dst, const int* ldd)
+{
+    const int i = blockIdx.y;
+    if (i >= batch) return;
+    const int m = n[i], c = cols < 0 ? m : cols;
+    const long long total = (long long)m * c;
+    const T* a = src[i];
+    T* b = dst[i];
+    const long long la = lds[i], lb = ldd[i];
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
+        const long long r = t % m, col = t / m;
+        b[r + col * lb] = a[r + col * la];
+    }
+}
+} // namespace
+
+template <class T>
+void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
+                   T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
+                   int occupancy, void* work)
+{
+    using namespace batched_detail;
+    if (batch <= 0) return;
+    if (max_blocks < 1 || max_blocks > kMaxBlocks) throw std::invalid_argument("batched_pposv: order above 1024");
+    if (grid < 1) throw std::invalid_argument("batched_pposv: empty grid");
+    char* w = static_cast<char*>(work);
+    T* xinv = reinterpret_cast<T*>(w);
+    const size_t inv = ((size_t)grid * max_blocks * 64 * 64 * sizeof(T) + 255) / 256 * 256;
+    int* order = reinterpret_cast<int*>(w + inv);
+    int* queue = reinterpret_cast<int*>(w + inv + ((size_t)batch * sizeof(int) + 255) / 256 * 256);
+    if (ordered) {
+        order_by_blocks_kernel<<<1, kOrderThreads, 0, s>>>(batch, max_blocks, n, order);
+    } else {
+        fill_iota_kernel<<<grid_for(batch, 256), 256, 0, s>>>(order, batch);
+    }
+    check_launch("batched_pposv");
+    solve_detail::zero_ints_kernel<<<1, 64, 0, s>>>(queue, 1);
+    check_launch("batched_pposv");
+    BatchArgs<T> a{batch, nrhs, upper ? 1 : 0, max_blocks, n, A, lda, B, ldb, info, logdet, order, queue, xinv};
+    if (occupancy == 2) batched_pposv_kernel<T, 2><<<grid, kThreads, 0, s>>>(a);
+    else batched_pposv_kernel<T, 1><<<grid, kThreads, 0, s>>>(a);
+    check_launch("batched_pposv");
+}
+
+template <class T>
+void batched_copy(hipStream_t s, int batch, const int* n, int cols, T* const* src, const int* lds, T* const* dst,
+                  const int* ldd, int max_n)
+{
+    if (batch <= 0 || max_n <= 0) return;
+    const long long per = (long long)max_n * (cols < 0 ? max_n : cols);
+    const unsigned gx = (unsigned)std::max(1LL, std::min(64LL, (per + 255) / 256));
+    for (int off = 0; off < batch; off += 65535) {
+        const int nb = std::min(65535, batch - off);
+        batched_copy_kernel<T><<<dim3(gx, nb), 256, 0, s>>>(nb, n + off, cols, src + off, lds + off, dst + off,
+                                                            ldd + off);
+        check_launch("batched_copy");
+    }
 }
 
 size_t solve_work_bytes(int nrhs)
@@ -580,6 +664,10 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void panel_reset<T>(hipStream_t, int, void*);                                                    \
     template int* panel_error_word<T>(int, void*);                                                            \
     template void panel_factor<T>(hipStream_t, bool, int, T*, long long, int, int, int*, void*);              \
+    template void batched_pposv<T>(hipStream_t, bool, int, int, const int*, T* const*, const int*, T* const*,  \
+                                   const int*, int*, double*, int, int, bool, int, void*);                     \
+    template void batched_copy<T>(hipStream_t, int, const int*, int, T* const*, const int*, T* const*,         \
+                                  const int*, int);                                                            \
     template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*);                      \
     template void diag_logdet<T>(hipStream_t, int, const T*, long long, void*, LogdetResult*);                \
     template void potrs_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, const T*, T*, long long,   \

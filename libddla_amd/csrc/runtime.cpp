@@ -110,6 +110,7 @@ ddlaStatus_t ddlaSet(DdlaHandle_t handle, int nprows, int npcols, char major)
     }
     if (!handle->initialized) {
         DDLA_HIP(hipGetDevice(&handle->device));
+        DDLA_HIP(hipDeviceGetAttribute(&handle->num_cus, hipDeviceAttributeMultiprocessorCount, handle->device));
         DDLA_HIP(hipStreamCreate(&handle->stream));
         handle->owns_stream = true;
         DDLA_HIP(hipStreamCreateWithFlags(&handle->side, hipStreamNonBlocking));

@@ -130,3 +130,95 @@ def test_vbatched_zero_batch_is_noop(handle):
     st = dd.lib.ddla_d_gemm_vbatched(handle.ptr, b"Q", b"N", None, None, None, ctypes.byref(one), None, None, None,
                                      None, ctypes.byref(one), None, None, 1)
     assert st == 6
+
+
+def _mat(dtype, rows, cols, seed, ld=None):
+    ld = max(rows, 1) if ld is None else ld
+    a = np.zeros((ld, max(cols, 1)), dtype=dtype, order="F")
+    a[:rows, :cols] = value(seed, rows * cols).reshape(cols, rows).T
+    return a
+
+
+def _ptrs(handle, bufs):
+    return DeviceBuf(handle, np.array([b.ptr.value for b in bufs], dtype=np.uint64))
+
+
+def _ints(handle, xs):
+    return DeviceBuf(handle, np.array(xs, dtype=np.int32))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("c0_left", [True, False])
+def test_vbatched2s_segments_reuse_temporaries(handle, dtype, c0_left):
+    """src/BLAS/gemm_vbatched.cpp:420-523: stage 0 writes C0 = a0 A0 B0^T,
+    stage 1 C1 = a1 C0 AB1 + b1 C1 (C0_left) or a1 AB1^T C0 + b1 C1; the three
+    segments share one pool of C0 buffers, so each segment's stage 1 must run
+    before the next segment's stage 0 overwrites them."""
+    pfx = "d" if dtype == np.float64 else "s"
+    segs = [2, 3, 1]
+    batch = sum(segs)
+    rng = np.random.RandomState(3)
+    dims = [(int(rng.randint(1, 70)), int(rng.randint(1, 70)), int(rng.randint(1, 40)), int(rng.randint(1, 50)))
+            for _ in range(batch)]  # (m0, n0, k0, other)
+    a0s, b0s, ab1s, c1s, keep = [], [], [], [], []
+    m1, n1, k1, lda1, ldb1, ldc1, ldc0 = [], [], [], [], [], [], []
+    pool_ld, pool_cols = 80, 80
+    pool = [DeviceBuf(handle, np.full((pool_cols, pool_ld), np.nan, dtype=dtype)) for _ in range(max(segs))]
+    c0_ptrs = []
+    slot = [i for s in segs for i in range(s)]
+    for b, (m0, n0, k0, o) in enumerate(dims):
+        a0 = _mat(dtype, m0, k0, 11 * b + 1)          # op N
+        b0 = _mat(dtype, n0, k0, 13 * b + 2)          # op T: B0 is n0 x k0
+        c0 = a0[:m0, :k0].astype(np.float64) @ b0[:n0, :k0].astype(np.float64).T * 0.5
+        if c0_left:   # C1 (m0 x o) = C0 (m0 x n0) * AB1 (n0 x o), ops N N
+            ab1 = _mat(dtype, n0, o, 17 * b + 3)
+            c1 = _mat(dtype, m0, o, 19 * b + 4, ld=m0 + 2)
+            want = 1.5 * c0 @ ab1[:n0, :o].astype(np.float64) - 0.25 * c1[:m0, :o].astype(np.float64)
+            m1.append(m0), n1.append(o), k1.append(n0), lda1.append(pool_ld), ldb1.append(ab1.shape[0])
+        else:         # C1 (o x n0) = AB1^T (o x m0) * C0 (m0 x n0), ops T N
+            ab1 = _mat(dtype, m0, o, 17 * b + 3)
+            c1 = _mat(dtype, o, n0, 19 * b + 4, ld=o + 2)
+            want = 1.5 * ab1[:m0, :o].astype(np.float64).T @ c0 - 0.25 * c1[:o, :n0].astype(np.float64)
+            m1.append(o), n1.append(n0), k1.append(m0), lda1.append(ab1.shape[0]), ldb1.append(pool_ld)
+        ldc1.append(c1.shape[0])
+        ldc0.append(pool_ld)
+        bufs = (DeviceBuf(handle, a0.T.copy()), DeviceBuf(handle, b0.T.copy()), DeviceBuf(handle, ab1.T.copy()),
+                DeviceBuf(handle, c1.T.copy()))
+        a0s.append(bufs[0]), b0s.append(bufs[1]), ab1s.append(bufs[2]), c1s.append(bufs[3])
+        keep.append((want, c1.shape))
+        c0_ptrs.append(pool[slot[b]].ptr.value)
+    d = lambda xs: _ints(handle, xs)
+    dm0, dn0, dk0 = d([x[0] for x in dims]), d([x[1] for x in dims]), d([x[2] for x in dims])
+    dlda0, dldb0 = d([_mat(dtype, x[0], x[2], 0).shape[0] for x in dims]), d([max(x[1], 1) for x in dims])
+    dldc0 = d(ldc0)
+    dm1, dn1, dk1, dlda1, dldb1, dldc1 = d(m1), d(n1), d(k1), d(lda1), d(ldb1), d(ldc1)
+    pa0, pb0, pab1, pc1 = _ptrs(handle, a0s), _ptrs(handle, b0s), _ptrs(handle, ab1s), _ptrs(handle, c1s)
+    pc0 = DeviceBuf(handle, np.array(c0_ptrs, dtype=np.uint64))
+    ctype = ctypes.c_double if dtype == np.float64 else ctypes.c_float
+    al0, be0, al1, be1 = ctype(0.5), ctype(0.0), ctype(1.5), ctype(-0.25)
+    seg = (ctypes.c_int * len(segs))(*segs)
+    fn = getattr(dd.lib, f"ddla_{pfx}_gemm_vbatched2s")
+    args = [handle.ptr, b"N", b"T", dm0.ptr, dn0.ptr, dk0.ptr, ctypes.byref(al0), pa0.ptr, dlda0.ptr, pb0.ptr,
+            dldb0.ptr, ctypes.byref(be0), pc0.ptr, dldc0.ptr,
+            b"N" if c0_left else b"T", b"N", dm1.ptr, dn1.ptr, dk1.ptr, ctypes.byref(al1), pab1.ptr, dlda1.ptr,
+            dldb1.ptr, ctypes.byref(be1), pc1.ptr, dldc1.ptr, 1 if c0_left else 0, batch, seg, len(segs)]
+    dd.check_status(fn(*args), "gemm_vbatched2s")
+    handle.synchronize()
+    tol = 1e-12 if dtype == np.float64 else 2e-5
+    for (want, shape), buf in zip(keep, c1s):
+        got = buf.download().T
+        r, c = want.shape
+        assert np.abs(got[:r, :c] - want).max() <= tol * max(1.0, np.abs(want).max())
+        assert np.array_equal(got[r:, :c], np.zeros_like(got[r:, :c]))  # padding rows of C1 untouched
+    # contract violations (host-side checks, no kernel runs)
+    bad_beta = list(args)
+    bad_beta[11] = ctypes.byref(ctype(1.0))
+    assert fn(*bad_beta) == 6 and b"beta" in dd.lib.ddla_c_error_string(6)
+    bad_seg = list(args)
+    bad_seg[-2] = (ctypes.c_int * 2)(2, 3)
+    bad_seg[-1] = 2
+    assert fn(*bad_seg) == 6 and b"segment" in dd.lib.ddla_c_error_string(6)
+    zero_seg = list(args)
+    zero_seg[-2] = (ctypes.c_int * 4)(2, 0, 3, 1)
+    zero_seg[-1] = 4
+    assert fn(*zero_seg) == 6
