@@ -1,0 +1,134 @@
+"""Summaries of rocprofv3 CSV output for the Cholesky path (used to write profiles/).
+
+    python -m libddla_amd.prof_summary --trace DIR [--fetch DIR --write DIR]
+        [--n 32768 --nb 512] [--command "..."] [--md OUT.md] [--traffic-json OUT.json]
+
+--trace is the output directory of `rocprofv3 --kernel-trace --stats
+--output-format csv`; --fetch / --write those of `--pmc FETCH_SIZE` and
+`--pmc WRITE_SIZE` runs of the same program.  The summary lists every kernel
+by total time and, for the trailing update (syrk_update_kernel), its average
+launch duration and HBM traffic per launch.  FETCH_SIZE and WRITE_SIZE (KiB
+per dispatch) are scaled by what they read for the copy kernel of known size
+in the same run (__amd_rocclr_copyBuffer: the largest one moves n^2 doubles
+each way), so traffic is reported in bytes.
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+
+
+def _read(pattern):
+    files = sorted(glob.glob(pattern))
+    if not files:
+        raise FileNotFoundError(pattern)
+    with open(files[0], newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def short_name(name: str) -> str:
+    base = name.replace("void ", "")
+    for prefix in ("(anonymous namespace)::", "ddla::kern::", "ddla::", "gemm_detail::", "panel_detail::",
+                   "solve_detail::", "batched_detail::"):
+        base = base.replace(prefix, "")
+    return base.split("(")[0][:90]
+
+
+def kernel_stats(trace_dir):
+    rows = _read(os.path.join(trace_dir, "*kernel_stats.csv"))
+    out = []
+    for r in rows:
+        out.append({"name": r["Name"], "calls": int(r["Calls"]), "total_ms": float(r["TotalDurationNs"]) / 1e6,
+                    "avg_us": float(r["AverageNs"]) / 1e3, "pct": float(r["Percentage"])})
+    return sorted(out, key=lambda x: -x["total_ms"])
+
+
+def counters(pmc_dir, counter):
+    rows = _read(os.path.join(pmc_dir, "*counter_collection.csv"))
+    return [(r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])) for r in rows
+            if r["Counter_Name"] == counter]
+
+
+def traffic(fetch_dir, write_dir, n):
+    """(fetch scale, write scale, per-launch fetch bytes, per-launch write bytes) of the trailing update."""
+    fetch, write = counters(fetch_dir, "FETCH_SIZE"), counters(write_dir, "WRITE_SIZE")
+    copy_bytes = 8.0 * n * n
+    cf = max(v for k, g, v in fetch if "copyBuffer" in k)
+    cw = max(v for k, g, v in write if "copyBuffer" in k)
+    f_scale = copy_bytes / (cf * 1024.0)
+    w_scale = copy_bytes / (cw * 1024.0)
+    fs = [v for k, g, v in fetch if "syrk_update_kernel" in k]
+    ws = [v for k, g, v in write if "syrk_update_kernel" in k]
+    return f_scale, w_scale, sum(fs) * 1024.0 * f_scale / len(fs), sum(ws) * 1024.0 * w_scale / len(ws), len(fs)
+
+
+def algorithmic_bytes_per_launch(n, nb):
+    """Mean over the trailing-update launches of one factorization: the panel (m x nb) read once and the
+    lower triangle of C read and written (U_next: m x kb2 block, U_rest: the rest)."""
+    total, launches = 0.0, 0
+    for k in range(0, n, nb):
+        m = n - k - nb
+        if m <= 0:
+            break
+        kb2 = min(nb, m)
+        total += 8.0 * (m * nb + kb2 * (kb2 + 1) + 2.0 * (m - kb2) * kb2)  # U_next
+        launches += 1
+        mr = m - kb2
+        if mr > 0:
+            total += 8.0 * (mr * nb + mr * (mr + 1))  # U_rest
+            launches += 1
+    return total / launches, launches
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--nb", type=int, default=512)
+    ap.add_argument("--command", default="")
+    ap.add_argument("--md")
+    ap.add_argument("--traffic-json")
+    args = ap.parse_args()
+
+    stats = kernel_stats(args.trace)
+    lines = ["# rocprofv3 kernel summary", ""]
+    if args.command:
+        lines += ["Command:", "", "```", args.command, "```", ""]
+    lines += ["| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+    for s in stats[:16]:
+        lines.append(f"| `{short_name(s['name'])}` | {s['calls']} | {s['total_ms']:.3f} | {s['avg_us']:.1f} | "
+                     f"{s['pct']:.2f} |")
+    syrk = [s for s in stats if "syrk_update_kernel" in s["name"]]
+    if syrk:
+        s = syrk[0]
+        lines += ["", f"Trailing update (`syrk_update_kernel`): {s['calls']} launches, average "
+                      f"{s['avg_us'] / 1e3:.4f} ms per launch."]
+    result = {}
+    if args.fetch and args.write:
+        f_scale, w_scale, fb, wb, nl = traffic(args.fetch, args.write, args.n)
+        alg, launches = algorithmic_bytes_per_launch(args.n, args.nb)
+        result = {"hbm_bytes_per_launch": fb + wb, "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+                  "algorithmic_bytes_per_launch": alg, "launches_counted": nl, "fetch_scale": f_scale,
+                  "write_scale": w_scale, "n": args.n, "nb": args.nb}
+        lines += ["",
+                  f"HBM traffic per trailing-update launch (mean of {nl} dispatches): FETCH {fb / 1e9:.3f} GB, "
+                  f"WRITE {wb / 1e9:.3f} GB, total {(fb + wb) / 1e9:.3f} GB; algorithmic "
+                  f"{alg / 1e9:.3f} GB (panel read once, lower C read and written).",
+                  f"Scale factors from the {8 * args.n * args.n / 1e9:.2f} GB copy in the same run: FETCH_SIZE x "
+                  f"{f_scale:.3f}, WRITE_SIZE x {w_scale:.3f}."]
+    text = "\n".join(lines) + "\n"
+    if args.md:
+        with open(args.md, "w") as f:
+            f.write(text)
+    else:
+        print(text)
+    if args.traffic_json and result:
+        with open(args.traffic_json, "w") as f:
+            json.dump(result, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
