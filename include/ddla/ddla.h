@@ -190,6 +190,20 @@ void* pposvVbatchedGraph(const DdlaHandle_t& handle, char uplo, const int* d_n, 
 void batchedGraphLaunch(void* graph, const DdlaHandle_t& handle);
 void batchedGraphDestroy(void* graph);
 
+// Reverse-mode adjoints (no counterpart in the reference), for the factor
+// of A = L L^T held in the lower triangle (uplo 'L'); synchronized.
+//   potrf_backward:  Abar = sym(L^-T Phi(L^T Lbar) L^-1) (symmetric, full)
+//   potrs_backward:  X = A^-1 B: Bbar = A^-1 Xbar, Lbar -= tril(Bbar (L^T X)^T + X (L^T Bbar)^T)
+//   logdet_backward: logdet A = 2 sum log L_ii: Lbar_ii += 2 g / L_ii
+template <typename T>
+void potrf_backward(const DdlaHandle_t& handle, char uplo, int n, const T* L, int ldl, const T* Lbar, int ldlb,
+                    T* Abar, int ldab);
+template <typename T>
+void potrs_backward(const DdlaHandle_t& handle, char uplo, int n, int nrhs, const T* L, int ldl, const T* X, int ldx,
+                    const T* Xbar, int ldxb, T* Bbar, int ldbb, T* Lbar, int ldlb);
+template <typename T>
+void logdet_backward(const DdlaHandle_t& handle, int n, const T* L, int ldl, double g, T* Lbar, int ldlb);
+
 namespace cholesky_impl {
 // Three components per routine: <r>_check (validation), <r>_workspace_size
 // (backend and host bytes; no allocation) and <r>_run on caller scratch (no

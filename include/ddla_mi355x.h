@@ -196,6 +196,24 @@ int ddla_d_pposv_vbatched_graph(void* handle, char uplo, const int* d_n, void* c
 int ddla_c_graph_launch(void* graph, void* handle);
 int ddla_c_graph_destroy(void* graph);
 
+/* ---- reverse-mode adjoints (no counterpart in the reference), factor of
+ * A = L L^T in the lower triangle (uplo 'L'); each returns synchronized:
+ * potrf_backward  Abar = sym(L^-T Phi(L^T Lbar) L^-1), Phi = tril with the
+ *                 diagonal halved, sym(S) = (S + S^T)/2;
+ * potrs_backward  for X = A^-1 B: Bbar = A^-1 Xbar and
+ *                 Lbar -= tril(Bbar (L^T X)^T + X (L^T Bbar)^T);
+ * logdet_backward for logdet A = 2 sum log L_ii: Lbar_ii += 2 g / L_ii. ---- */
+int ddla_s_potrf_backward(void* handle, char uplo, int n, const void* L, int ldl, const void* Lbar, int ldlb,
+                          void* Abar, int ldab);
+int ddla_d_potrf_backward(void* handle, char uplo, int n, const void* L, int ldl, const void* Lbar, int ldlb,
+                          void* Abar, int ldab);
+int ddla_s_potrs_backward(void* handle, char uplo, int n, int nrhs, const void* L, int ldl, const void* X, int ldx,
+                          const void* Xbar, int ldxb, void* Bbar, int ldbb, void* Lbar, int ldlb);
+int ddla_d_potrs_backward(void* handle, char uplo, int n, int nrhs, const void* L, int ldl, const void* X, int ldx,
+                          const void* Xbar, int ldxb, void* Bbar, int ldbb, void* Lbar, int ldlb);
+int ddla_s_logdet_backward(void* handle, int n, const void* L, int ldl, double g, void* Lbar, int ldlb);
+int ddla_d_logdet_backward(void* handle, int n, const void* L, int ldl, double g, void* Lbar, int ldlb);
+
 /* ---- harness helpers (no counterpart in the reference) ------------------
  * Philox4x64-10 + Box-Muller normals of the project's seed convention
  * (element e of stream s: counter {e/4, s, 0, 0}, key {seed, 0}), and the

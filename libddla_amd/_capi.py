@@ -57,6 +57,15 @@ def _load():
         raise ImportError(
             f"{LIBRARY_PATH} is missing: build the HIP library first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
+    # One HIP runtime per process.  torch ships its own libamdhip64 and
+    # libhsa-runtime64 under the same sonames as the system ROCm ones this
+    # library links; loaded first, they satisfy this library's dependencies,
+    # whereas loading this library first would leave torch to load a second
+    # runtime that then finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return ctypes.CDLL(LIBRARY_PATH)
 
 
@@ -117,6 +126,9 @@ for _p in ("s", "d"):
         f"ddla_{_p}_pposv_run": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _i, _i, _ip, _vp, _i, _i, _ip, _ip, _vp, _ll,
                                       _vp, _ll]),
         f"ddla_{_p}_potrf": (_i, [_vp, _c, _i, _vp, _i, _vp]),
+        f"ddla_{_p}_potrf_backward": (_i, [_vp, _c, _i, _vp, _i, _vp, _i, _vp, _i]),
+        f"ddla_{_p}_potrs_backward": (_i, [_vp, _c, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i]),
+        f"ddla_{_p}_logdet_backward": (_i, [_vp, _i, _vp, _i, ctypes.c_double, _vp, _i]),
         f"ddla_{_p}_pposv_vbatched": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i]),
         f"ddla_{_p}_pposv_vbatched_workspace_size": (_i, [_vp, _i, _i, _llp]),
         f"ddla_{_p}_pposv_vbatched_run": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _ll]),

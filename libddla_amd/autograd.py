@@ -1,0 +1,179 @@
+"""torch.autograd Functions for the Cholesky path, forward and backward on the
+hand-written gfx950 kernels (C ABI of libddla_mi355x.so).
+
+    L = cholesky(A, h)            A = L L^T (FP32/FP64, CUDA tensor), L lower
+    X = cholesky_solve(B, L, h)   X = (L L^T)^-1 B
+    d = cholesky_logdet(L, h)     log det(L L^T) = 2 sum log L_ii
+
+Backward passes (ddla_{s,d}_*_backward):
+    cholesky:       Abar = sym(L^-T Phi(L^T Lbar) L^-1)  -- the symmetric-input
+                    gradient, as torch.linalg.cholesky's
+    cholesky_solve: Bbar = A^-1 Xbar, Lbar = -tril(Bbar (L^T X)^T + X (L^T Bbar)^T)
+    cholesky_logdet: Lbar = 2 g diag(1 / L_ii)
+The library works on column-major storage; tensors are passed through
+column-major copies (or views, when they already are), and torch's current
+stream is synchronized before each call (every call returns synchronized).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._capi import check_status, lib
+from .matrix import descinit
+
+_PFX = {torch.float64: "d", torch.float32: "s"}
+
+
+def _pfx(t):
+    if t.dtype not in _PFX:
+        raise TypeError("float32 and float64 are built")
+    if not t.is_cuda:
+        raise ValueError("the Cholesky path runs on the GPU: pass CUDA tensors")
+    return _PFX[t.dtype]
+
+
+def _colmajor(t):
+    """A column-major (Fortran-ordered) tensor equal to t (2-D); no copy when t already is."""
+    if t.dim() == 1:
+        t = t.unsqueeze(1)
+    if t.stride(0) == 1 and t.stride(1) == max(1, t.shape[0]):
+        return t
+    return t.t().contiguous().t()
+
+
+def _empty_colmajor(m, n, like):
+    return torch.empty((n, m), dtype=like.dtype, device=like.device).t()
+
+
+def _zeros_colmajor(m, n, like):
+    return torch.zeros((n, m), dtype=like.dtype, device=like.device).t()
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _sync():
+    torch.cuda.current_stream().synchronize()
+
+
+def _desc(h, m, n, ld):
+    d = descinit(h, m, n, 64, 64, 0, 0, ld)
+    return d, d.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Cholesky(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, A, h):
+        p = _pfx(A)
+        n = A.shape[0]
+        Lc = A.detach().t().contiguous().t().clone() if n else A.detach().clone()
+        _sync()
+        info = ctypes.c_int(0)
+        if n:
+            d, dp = _desc(h, n, n, n)
+            check_status(getattr(lib, f"ddla_{p}_ppotrf")(h.ptr, b"L", n, _ptr(Lc), 1, 1, dp, ctypes.byref(info)),
+                         f"ddla_{p}_ppotrf")
+        if info.value != 0:
+            raise RuntimeError(f"cholesky: the leading minor of order {info.value} is not positive definite")
+        L = torch.tril(Lc)
+        ctx.h = h
+        ctx.save_for_backward(L)
+        return L
+
+    @staticmethod
+    def backward(ctx, Lbar):
+        (L,) = ctx.saved_tensors
+        p = _pfx(L)
+        n = L.shape[0]
+        Lc, Bc = _colmajor(L), _colmajor(Lbar)
+        Abar = _empty_colmajor(n, n, L)
+        _sync()
+        if n:
+            check_status(getattr(lib, f"ddla_{p}_potrf_backward")(ctx.h.ptr, b"L", n, _ptr(Lc), n, _ptr(Bc), n,
+                                                                  _ptr(Abar), n), f"ddla_{p}_potrf_backward")
+        return Abar, None
+
+
+class CholeskySolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, B, L, h):
+        p = _pfx(L)
+        n = L.shape[0]
+        vec = B.dim() == 1
+        Lc = _colmajor(L.detach())
+        X = _colmajor(B.detach()).clone()
+        nrhs = X.shape[1]
+        _sync()
+        if n and nrhs:
+            da, dap = _desc(h, n, n, n)
+            db, dbp = _desc(h, n, nrhs, n)
+            check_status(getattr(lib, f"ddla_{p}_ppotrs")(h.ptr, b"L", b"L", b"N", n, nrhs, _ptr(Lc), dap, _ptr(X),
+                                                          dbp), f"ddla_{p}_ppotrs")
+        ctx.h, ctx.vec = h, vec
+        ctx.save_for_backward(Lc, X)
+        return X[:, 0].clone() if vec else X.clone()
+
+    @staticmethod
+    def backward(ctx, Xbar):
+        Lc, X = ctx.saved_tensors
+        p = _pfx(Lc)
+        n, nrhs = X.shape
+        Xb = _colmajor(Xbar)
+        Bbar = _empty_colmajor(n, nrhs, X)
+        Lbar = _zeros_colmajor(n, n, X)
+        _sync()
+        if n and nrhs:
+            check_status(getattr(lib, f"ddla_{p}_potrs_backward")(ctx.h.ptr, b"L", n, nrhs, _ptr(Lc), n, _ptr(X), n,
+                                                                  _ptr(Xb), n, _ptr(Bbar), n, _ptr(Lbar), n),
+                         f"ddla_{p}_potrs_backward")
+        return (Bbar[:, 0].clone() if ctx.vec else Bbar), Lbar, None
+
+
+class CholeskyLogdet(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, L, h):
+        p = _pfx(L)
+        n = L.shape[0]
+        Lc = _colmajor(L.detach())
+        res = np.zeros(2, dtype=np.float64 if L.dtype == torch.float64 else np.float32)
+        _sync()
+        if n:
+            check_status(getattr(lib, f"ddla_{p}_diag_logdet")(h.ptr, n, _ptr(Lc), n, ctypes.c_void_p(res.ctypes.data)),
+                         f"ddla_{p}_diag_logdet")
+        ctx.h = h
+        ctx.save_for_backward(Lc)
+        return torch.tensor(2.0 * float(res[0]), dtype=L.dtype, device=L.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        (Lc,) = ctx.saved_tensors
+        p = _pfx(Lc)
+        n = Lc.shape[0]
+        Lbar = _zeros_colmajor(n, n, Lc)
+        gval = float(g.item())
+        _sync()
+        if n:
+            check_status(getattr(lib, f"ddla_{p}_logdet_backward")(ctx.h.ptr, n, _ptr(Lc), n, gval, _ptr(Lbar), n),
+                         f"ddla_{p}_logdet_backward")
+        return Lbar, None
+
+
+def cholesky(A, h):
+    return Cholesky.apply(A, h)
+
+
+def cholesky_solve(B, L, h):
+    return CholeskySolve.apply(B, L, h)
+
+
+def cholesky_logdet(L, h):
+    return CholeskyLogdet.apply(L, h)
+
+
+def gp_negative_log_likelihood(K, y, h):
+    """0.5 y^T K^-1 y + 0.5 log det K (the constant n/2 log 2 pi omitted)."""
+    L = cholesky(K, h)
+    alpha = cholesky_solve(y, L, h)
+    return 0.5 * torch.dot(y, alpha) + 0.5 * cholesky_logdet(L, h)

@@ -337,6 +337,40 @@ DDLA_C_DRIVERS(s, float)
 DDLA_C_DRIVERS(d, double)
 #undef DDLA_C_DRIVERS
 
+// ---- adjoints ------------------------------------------------------------------
+#define DDLA_C_ADJOINTS(P, T)                                                                                  \
+    int ddla_##P##_potrf_backward(void* handle, char uplo, int n, const void* L, int ldl, const void* Lbar,     \
+                                  int ldlb, void* Abar, int ldab)                                              \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::potrf_backward<T>(h, uplo, n, static_cast<const T*>(L), ldl, static_cast<const T*>(Lbar),    \
+                                    ldlb, static_cast<T*>(Abar), ldab);                                        \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_potrs_backward(void* handle, char uplo, int n, int nrhs, const void* L, int ldl,            \
+                                  const void* X, int ldx, const void* Xbar, int ldxb, void* Bbar, int ldbb,    \
+                                  void* Lbar, int ldlb)                                                        \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::potrs_backward<T>(h, uplo, n, nrhs, static_cast<const T*>(L), ldl, static_cast<const T*>(X), \
+                                    ldx, static_cast<const T*>(Xbar), ldxb, static_cast<T*>(Bbar), ldbb,       \
+                                    static_cast<T*>(Lbar), ldlb);                                              \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_logdet_backward(void* handle, int n, const void* L, int ldl, double g, void* Lbar,          \
+                                   int ldlb)                                                                   \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::logdet_backward<T>(h, n, static_cast<const T*>(L), ldl, g, static_cast<T*>(Lbar), ldlb);     \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }
+DDLA_C_ADJOINTS(s, float)
+DDLA_C_ADJOINTS(d, double)
+#undef DDLA_C_ADJOINTS
+
 // ---- batched SPD solves --------------------------------------------------------
 #define DDLA_C_BATCHED(P, T)                                                                                   \
     int ddla_##P##_pposv_vbatched(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, \

@@ -91,6 +91,27 @@ template <class T>
 void batched_copy(hipStream_t s, int batch, const int* n, int cols, T* const* src, const int* lds, T* const* dst,
                   const int* ldd, int max_n);
 
+// One triangular-solve sweep with the order-n factor (L view) and its inverted
+// diagonal blocks: L Y = B (backward = false) or L^T X = B (backward = true),
+// in place on B (n x nrhs).  work / error as for potrs_sweeps.
+template <class T>
+void trsm_sweep(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L, long long ldl, const T* dinv,
+                T* B, long long ldb, void* work, int* error);
+
+// Elementwise helpers of the adjoints (order n, column-major):
+//   tril_copy:   out = tril(in) (strict upper triangle zero)
+//   phi:         M = tril(M) with the diagonal halved, strict upper zero
+//   sym_half:    out = (S + S^T) / 2
+//   diag_adjoint: Lbar(i, i) += 2 g / L(i, i)
+template <class T>
+void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long long ldo);
+template <class T>
+void phi(hipStream_t s, int n, T* M, long long ld);
+template <class T>
+void sym_half(hipStream_t s, int n, const T* S, long long lds, T* out, long long ldo);
+template <class T>
+void diag_adjoint(hipStream_t s, int n, const T* L, long long ldl, double g, T* Lbar, long long ldlb);
+
 // diag_logdet reduction over A[i * (lda + 1)], i < n: work must hold
 // logdet_work_bytes(); out (device) receives {sum log|d|, flags} where flags
 // = 1 nonfinite | 2 zero | 4 odd number of negative entries.

@@ -555,6 +555,84 @@ void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
     check_launch("potrs");
 }
 
+template <class T>
+void trsm_sweep(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L, long long ldl, const T* dinv,
+                T* B, long long ldb, void* work, int* error)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    using namespace solve_detail;
+    const int nblk = (n + kSB - 1) / kSB, slabs = (nrhs + kSB - 1) / kSB;
+    if (slabs > 65535) throw std::invalid_argument("trsm_sweep: too many right-hand sides for one launch");
+    SolveArgs<T> a{n, nrhs, nblk, L, ldl, upper ? 1 : 0, dinv, B, ldb, static_cast<int*>(work), error, 1 << 23};
+    const dim3 grid((unsigned)nblk, (unsigned)slabs), block(kThreads);
+    zero_ints_kernel<<<1, 256, 0, s>>>(a.sync, 2 * slabs);
+    check_launch("trsm_sweep");
+    if (backward) solve_sweep_kernel<T, true><<<grid, block, 0, s>>>(a);
+    else solve_sweep_kernel<T, false><<<grid, block, 0, s>>>(a);
+    check_launch("trsm_sweep");
+}
+
+namespace {
+// mode 0: out = tril(in); mode 1: out = tril(in) with the diagonal halved;
+// mode 2: out = (in + in^T) / 2.  One thread per element of out.
+template <class T>
+__global__ void square_map_kernel(int n, const T* in, long long ldi, T* out, long long ldo, int mode)
+{
+    const long long total = (long long)n * n;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
+        const int i = (int)(t % n), j = (int)(t / n);
+        T v;
+        if (mode == 2) {
+            v = T(0.5) * (in[i + j * ldi] + in[j + i * ldi]);
+        } else {
+            v = i >= j ? in[i + j * ldi] : T(0);
+            if (mode == 1 && i == j) v *= T(0.5);
+        }
+        out[i + j * ldo] = v;
+    }
+}
+
+template <class T>
+__global__ void diag_adjoint_kernel(int n, const T* L, long long ldl, double g, T* Lbar, long long ldlb)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        Lbar[i + i * ldlb] += (T)(2.0 * g / (double)L[i + i * ldl]);
+}
+} // namespace
+
+template <class T>
+void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long long ldo)
+{
+    if (n <= 0) return;
+    square_map_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, in, ldi, out, ldo, 0);
+    check_launch("tril_copy");
+}
+
+template <class T>
+void phi(hipStream_t s, int n, T* M, long long ld)
+{
+    if (n <= 0) return;
+    square_map_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, M, ld, M, ld, 1);
+    check_launch("phi");
+}
+
+template <class T>
+void sym_half(hipStream_t s, int n, const T* S, long long lds, T* out, long long ldo)
+{
+    if (n <= 0) return;
+    if (S == out) throw std::invalid_argument("sym_half: output must not alias the input");
+    square_map_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, S, lds, out, ldo, 2);
+    check_launch("sym_half");
+}
+
+template <class T>
+void diag_adjoint(hipStream_t s, int n, const T* L, long long ldl, double g, T* Lbar, long long ldlb)
+{
+    if (n <= 0) return;
+    diag_adjoint_kernel<T><<<grid_for(n, 256), 256, 0, s>>>(n, L, ldl, g, Lbar, ldlb);
+    check_launch("diag_adjoint");
+}
+
 size_t sym_norm_work_bytes(int n)
 {
     const size_t nblk = (size_t)(n + 63) / 64;
@@ -673,6 +751,12 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void potrs_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, const T*, T*, long long,   \
                                   void*, int*);                                                               \
     template void add_diagonal<T>(hipStream_t, int, T*, long long, T);                                        \
+    template void trsm_sweep<T>(hipStream_t, bool, bool, int, int, const T*, long long, const T*, T*, long long, \
+                                void*, int*);                                                                 \
+    template void tril_copy<T>(hipStream_t, int, const T*, long long, T*, long long);                         \
+    template void phi<T>(hipStream_t, int, T*, long long);                                                     \
+    template void sym_half<T>(hipStream_t, int, const T*, long long, T*, long long);                          \
+    template void diag_adjoint<T>(hipStream_t, int, const T*, long long, double, T*, long long);              \
     template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);
 DDLA_KERN_INSTANTIATE(double)
 DDLA_KERN_INSTANTIATE(float)
@@ -687,6 +771,7 @@ template void symm_residual<double>(hipStream_t, bool, int, int, const double*, 
                                     const double*, long long, double*, long long);
 template void convert<double, float>(hipStream_t, int, int, const float*, long long, double*, long long);
 template void convert<double, double>(hipStream_t, int, int, const double*, long long, double*, long long);
+template void convert<float, float>(hipStream_t, int, int, const float*, long long, float*, long long);
 
 } // namespace kern
 } // namespace ddla
