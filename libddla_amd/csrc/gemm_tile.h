@@ -131,16 +131,25 @@ __device__ inline bool map_tile(const TileOrder& o, int b, int& ti, int& tj)
 
 constexpr int kPad = 16;  // LDS row padding: the 4 k-rows of a fragment load fall on distinct banks
 
-template <int BM, int BN, int BK>
+template <int BM, int BN, int BK, int NBUF = 2>
 constexpr int lds_elems()
 {
-    return 2 * BK * (BM + kPad) + 2 * BK * (BN + kPad);
+    return NBUF * BK * (BM + kPad) + NBUF * BK * (BN + kPad);
 }
 
 // One BM x BN tile of C at (m0, n0).  FAST: the whole tile and every k slab
 // are in bounds and the operands allow 16-byte vector loads (decided once per
 // tile, so the k loop carries no branches between its loads).
-template <class T, int BM, int BN, int BK, bool TA, bool TB, bool FAST>
+// NBUF = 2: the slabs alternate between two LDS buffers (one barrier per
+// slab); NBUF = 1: one buffer, the next slab waits in registers (two
+// barriers per slab, half the LDS, so more workgroups fit on a CU).
+// PF2 (with NBUF = 2): two register sets, loads run two slabs ahead.
+// CFIRST (alpha = +-1): the accumulators start as beta * C, loaded before the
+// first slab so that its latency overlaps the k loop, and alpha is folded
+// into op(A) as it enters LDS (exact for alpha = +-1); the epilogue then only
+// stores.
+template <class T, int BM, int BN, int BK, bool TA, bool TB, bool FAST, int NBUF = 2, bool CFIRST = false,
+          bool PF2 = false>
 __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, int n0, T* smem)
 {
     using Op = Mfma<T>;
@@ -157,20 +166,37 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     T* As = smem;
-    T* Bs = smem + 2 * BK * LA;
+    T* Bs = smem + NBUF * BK * LA;
+    const bool full = FAST || (m0 + BM <= p.m && n0 + BN <= p.n);
 
     acc_t acc[TM][TN];
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    if (CFIRST && p.beta != T(0)) {
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int y = 0; y < TN; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = m0 + wm * (BM / 2) + x * 16 + Op::row(lane, r);
+                    const int j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
+                    if (full || (i < p.m && j < p.n)) acc[x][y][r] = p.beta * p.C[i + (long long)j * p.ldc];
+                }
+    }
+    const T fold = (CFIRST && p.alpha == T(-1)) ? T(-1) : T(1);
 
     const int nkt = (p.alpha == T(0)) ? 0 : (p.k + BK - 1) / BK;
 
     // Vector v of a slab: TA == false (op(A) = A, contiguous along m):
     // (i, kk) = ((v % (BM/VW)) * VW, v / (BM/VW)); TA == true (contiguous along
     // k): (i, kk) = (v / (BK/VW), (v % (BK/VW)) * VW).  Same for op(B).
-    T ra[AVT][VW], rb[BVT][VW];
+    // register sets of the slabs in flight (PF2: two, so that slab kt + 2 is
+    // loaded while slab kt is multiplied and slab kt + 1 waits in the other)
+    constexpr int NSET = PF2 ? 2 : 1;
+    T ra[NSET][AVT][VW], rb[NSET][BVT][VW];
 
     auto a_coord = [&](int e, int& i, int& kk) {
         const int v = tid + 256 * e;
@@ -182,7 +208,7 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
         if (TB) { j = (v % (BN / VW)) * VW; kk = v / (BN / VW); }
         else { kk = (v % (BK / VW)) * VW; j = v / (BK / VW); }
     };
-    auto load = [&](int k0) {
+    auto load = [&](int k0, int set) {
 #pragma unroll
         for (int e = 0; e < AVT; ++e) {
             int i, kk;
@@ -192,12 +218,12 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
             if (FAST) {
                 const vec_t v = *reinterpret_cast<const vec_t*>(src);
 #pragma unroll
-                for (int q = 0; q < VW; ++q) ra[e][q] = v[q];
+                for (int q = 0; q < VW; ++q) ra[set][e][q] = v[q];
             } else {
 #pragma unroll
                 for (int q = 0; q < VW; ++q) {
                     const long long ii = TA ? gi : gi + q, kq = TA ? gk + q : gk;
-                    ra[e][q] = (ii < p.m && kq < p.k) ? (TA ? p.A[kq + ii * p.lda] : p.A[ii + kq * p.lda]) : T(0);
+                    ra[set][e][q] = (ii < p.m && kq < p.k) ? (TA ? p.A[kq + ii * p.lda] : p.A[ii + kq * p.lda]) : T(0);
                 }
             }
         }
@@ -210,17 +236,17 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
             if (FAST) {
                 const vec_t v = *reinterpret_cast<const vec_t*>(src);
 #pragma unroll
-                for (int q = 0; q < VW; ++q) rb[e][q] = v[q];
+                for (int q = 0; q < VW; ++q) rb[set][e][q] = v[q];
             } else {
 #pragma unroll
                 for (int q = 0; q < VW; ++q) {
                     const long long jj = TB ? gj + q : gj, kq = TB ? gk : gk + q;
-                    rb[e][q] = (jj < p.n && kq < p.k) ? (TB ? p.B[jj + kq * p.ldb] : p.B[kq + jj * p.ldb]) : T(0);
+                    rb[set][e][q] = (jj < p.n && kq < p.k) ? (TB ? p.B[jj + kq * p.ldb] : p.B[kq + jj * p.ldb]) : T(0);
                 }
             }
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, int set) {
         T* as = As + buf * BK * LA;
         T* bs = Bs + buf * BK * LB;
 #pragma unroll
@@ -230,11 +256,11 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
             if (!TA) {
                 vec_t v;
 #pragma unroll
-                for (int q = 0; q < VW; ++q) v[q] = ra[e][q];
+                for (int q = 0; q < VW; ++q) v[q] = CFIRST ? fold * ra[set][e][q] : ra[set][e][q];
                 *reinterpret_cast<vec_t*>(as + kk * LA + i) = v;
             } else {
 #pragma unroll
-                for (int q = 0; q < VW; ++q) as[(kk + q) * LA + i] = ra[e][q];
+                for (int q = 0; q < VW; ++q) as[(kk + q) * LA + i] = CFIRST ? fold * ra[set][e][q] : ra[set][e][q];
             }
         }
 #pragma unroll
@@ -244,23 +270,16 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
             if (TB) {
                 vec_t v;
 #pragma unroll
-                for (int q = 0; q < VW; ++q) v[q] = rb[e][q];
+                for (int q = 0; q < VW; ++q) v[q] = rb[set][e][q];
                 *reinterpret_cast<vec_t*>(bs + kk * LB + j) = v;
             } else {
 #pragma unroll
-                for (int q = 0; q < VW; ++q) bs[(kk + q) * LB + j] = rb[e][q];
+                for (int q = 0; q < VW; ++q) bs[(kk + q) * LB + j] = rb[set][e][q];
             }
         }
     };
 
-    if (nkt > 0) {
-        load(0);
-        store(0);
-    }
-    __syncthreads();
-    for (int kt = 0; kt < nkt; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nkt) load((kt + 1) * BK);
+    auto compute = [&](int buf) {
         const T* as = As + buf * BK * LA + wm * (BM / 2) + (lane & 15);
         const T* bs = Bs + buf * BK * LB + wn * (BN / 2) + (lane & 15);
 #pragma unroll
@@ -276,11 +295,54 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
 #pragma unroll
                 for (int y = 0; y < TN; ++y) acc[x][y] = Op::mma(a[x], b[y], acc[x][y]);
         }
-        if (kt + 1 < nkt) store(buf ^ 1);
+    };
+
+    if (NBUF == 2 && PF2) {
+        // slab s lives in LDS buffer s % 2 and, before that, in register set s % 2
+        if (nkt > 0) {
+            load(0, 0);
+            store(0, 0);
+        }
+        if (nkt > 1) load(BK, 1);
         __syncthreads();
+        for (int kt = 0; kt < nkt; kt += 2) {
+            if (kt + 2 < nkt) load((kt + 2) * BK, 0);
+            compute(0);
+            if (kt + 1 < nkt) store(1, 1);
+            __syncthreads();
+            if (kt + 1 >= nkt) break;
+            if (kt + 3 < nkt) load((kt + 3) * BK, 1);
+            compute(1);
+            if (kt + 2 < nkt) store(0, 0);
+            __syncthreads();
+        }
+    } else if (NBUF == 2) {
+        if (nkt > 0) {
+            load(0, 0);
+            store(0, 0);
+        }
+        __syncthreads();
+        for (int kt = 0; kt < nkt; ++kt) {
+            if (kt + 1 < nkt) load((kt + 1) * BK, 0);
+            compute(kt & 1);
+            if (kt + 1 < nkt) store((kt & 1) ^ 1, 0);
+            __syncthreads();
+        }
+    } else {
+        if (nkt > 0) {
+            load(0, 0);
+            store(0, 0);
+        }
+        __syncthreads();
+        for (int kt = 0; kt < nkt; ++kt) {
+            if (kt + 1 < nkt) load((kt + 1) * BK, 0);
+            compute(0);
+            __syncthreads();
+            if (kt + 1 < nkt) store(0, 0);
+            __syncthreads();
+        }
     }
 
-    const bool full = FAST || (m0 + BM <= p.m && n0 + BN <= p.n);
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
@@ -293,20 +355,24 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
                 if (p.tri == kTriLowerD && (long long)i + p.diag < j) continue;
                 if (p.tri == kTriUpperD && (long long)i + p.diag > j) continue;
                 T* c = p.C + i + (long long)j * p.ldc;
-                T v = p.alpha * acc[x][y][r];
-                if (p.beta != T(0)) v += p.beta * *c;
-                *c = v;
+                if (CFIRST) {
+                    *c = (p.alpha == T(1) || p.alpha == T(-1)) ? acc[x][y][r] : p.alpha * acc[x][y][r];
+                } else {
+                    T v = p.alpha * acc[x][y][r];
+                    if (p.beta != T(0)) v += p.beta * *c;
+                    *c = v;
+                }
             }
 }
 
-template <class T, int BM, int BN, int BK, bool TA, bool TB>
+template <class T, int BM, int BN, int BK, bool TA, bool TB, int NBUF = 2, bool CFIRST = false>
 __device__ __forceinline__ void gemm_tile(const GemmProblem<T>& p, int m0, int n0, T* smem)
 {
     if (p.tri == kTriLowerD && (long long)(m0 + BM - 1) + p.diag < n0) return;
     if (p.tri == kTriUpperD && (long long)m0 + p.diag > n0 + BN - 1) return;
     const bool fast = p.vec_ab && m0 + BM <= p.m && n0 + BN <= p.n && p.k % BK == 0;
-    if (fast) gemm_tile_body<T, BM, BN, BK, TA, TB, true>(p, m0, n0, smem);
-    else gemm_tile_body<T, BM, BN, BK, TA, TB, false>(p, m0, n0, smem);
+    if (fast) gemm_tile_body<T, BM, BN, BK, TA, TB, true, NBUF, CFIRST>(p, m0, n0, smem);
+    else gemm_tile_body<T, BM, BN, BK, TA, TB, false, NBUF, CFIRST>(p, m0, n0, smem);
 }
 
 template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
@@ -322,15 +388,16 @@ __global__ __launch_bounds__(256, WAVES_PER_EU) void gemm_kernel(GemmProblem<T> 
 // The same tile loop under its own symbol, launched for the triangle-masked
 // (syrk-shaped) trailing updates of the factorization so that kernel traces
 // report them apart from plain products.
-template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
+template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU, int NBUF = 2>
 __global__ __launch_bounds__(256, WAVES_PER_EU) void syrk_update_kernel(GemmProblem<T> p, TileOrder o,
                                                                          const int* abort)
 {
     if (abort != nullptr && *abort != 0) return;
     int ti, tj;
     if (!map_tile(o, blockIdx.x, ti, tj)) return;
-    __shared__ T smem[lds_elems<BM, BN, BK>()];
-    gemm_tile<T, BM, BN, BK, TA, TB>(p, ti * BM, tj * BN, smem);
+    __shared__ T smem[lds_elems<BM, BN, BK, NBUF>()];
+    if (p.alpha == T(1) || p.alpha == T(-1)) gemm_tile<T, BM, BN, BK, TA, TB, NBUF, true>(p, ti * BM, tj * BN, smem);
+    else gemm_tile<T, BM, BN, BK, TA, TB, NBUF, false>(p, ti * BM, tj * BN, smem);
 }
 
 template <class T>
