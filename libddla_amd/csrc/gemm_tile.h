@@ -144,12 +144,11 @@ constexpr int lds_elems()
 // slab); NBUF = 1: one buffer, the next slab waits in registers (two
 // barriers per slab, half the LDS, so more workgroups fit on a CU).
 // PF2 (with NBUF = 2): two register sets, loads run two slabs ahead.
-// CFIRST (alpha = +-1): the accumulators start as beta * C, loaded before the
-// first slab so that its latency overlaps the k loop, and alpha is folded
-// into op(A) as it enters LDS (exact for alpha = +-1); the epilogue then only
-// stores.
-template <class T, int BM, int BN, int BK, bool TA, bool TB, bool FAST, int NBUF = 2, bool CFIRST = false,
-          bool PF2 = false>
+// The accumulators start at zero and C is read once, in the epilogue
+// (C := alpha * acc + beta * C): starting them at beta * C instead rounds
+// every partial sum at the scale of C, which in FP32 visibly slows the
+// iterative refinement of nearly cancelling Schur complements.
+template <class T, int BM, int BN, int BK, bool TA, bool TB, bool FAST, int NBUF = 2, bool PF2 = false>
 __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, int n0, T* smem)
 {
     using Op = Mfma<T>;
@@ -174,19 +173,6 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-    if (CFIRST && p.beta != T(0)) {
-#pragma unroll
-        for (int x = 0; x < TM; ++x)
-#pragma unroll
-            for (int y = 0; y < TN; ++y)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = m0 + wm * (BM / 2) + x * 16 + Op::row(lane, r);
-                    const int j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
-                    if (full || (i < p.m && j < p.n)) acc[x][y][r] = p.beta * p.C[i + (long long)j * p.ldc];
-                }
-    }
-    const T fold = (CFIRST && p.alpha == T(-1)) ? T(-1) : T(1);
 
     const int nkt = (p.alpha == T(0)) ? 0 : (p.k + BK - 1) / BK;
 
@@ -256,11 +242,11 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
             if (!TA) {
                 vec_t v;
 #pragma unroll
-                for (int q = 0; q < VW; ++q) v[q] = CFIRST ? fold * ra[set][e][q] : ra[set][e][q];
+                for (int q = 0; q < VW; ++q) v[q] = ra[set][e][q];
                 *reinterpret_cast<vec_t*>(as + kk * LA + i) = v;
             } else {
 #pragma unroll
-                for (int q = 0; q < VW; ++q) as[(kk + q) * LA + i] = CFIRST ? fold * ra[set][e][q] : ra[set][e][q];
+                for (int q = 0; q < VW; ++q) as[(kk + q) * LA + i] = ra[set][e][q];
             }
         }
 #pragma unroll
@@ -355,24 +341,20 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
                 if (p.tri == kTriLowerD && (long long)i + p.diag < j) continue;
                 if (p.tri == kTriUpperD && (long long)i + p.diag > j) continue;
                 T* c = p.C + i + (long long)j * p.ldc;
-                if (CFIRST) {
-                    *c = (p.alpha == T(1) || p.alpha == T(-1)) ? acc[x][y][r] : p.alpha * acc[x][y][r];
-                } else {
-                    T v = p.alpha * acc[x][y][r];
-                    if (p.beta != T(0)) v += p.beta * *c;
-                    *c = v;
-                }
+                T v = p.alpha * acc[x][y][r];
+                if (p.beta != T(0)) v += p.beta * *c;
+                *c = v;
             }
 }
 
-template <class T, int BM, int BN, int BK, bool TA, bool TB, int NBUF = 2, bool CFIRST = false>
+template <class T, int BM, int BN, int BK, bool TA, bool TB, int NBUF = 2>
 __device__ __forceinline__ void gemm_tile(const GemmProblem<T>& p, int m0, int n0, T* smem)
 {
     if (p.tri == kTriLowerD && (long long)(m0 + BM - 1) + p.diag < n0) return;
     if (p.tri == kTriUpperD && (long long)m0 + p.diag > n0 + BN - 1) return;
     const bool fast = p.vec_ab && m0 + BM <= p.m && n0 + BN <= p.n && p.k % BK == 0;
-    if (fast) gemm_tile_body<T, BM, BN, BK, TA, TB, true, NBUF, CFIRST>(p, m0, n0, smem);
-    else gemm_tile_body<T, BM, BN, BK, TA, TB, false, NBUF, CFIRST>(p, m0, n0, smem);
+    if (fast) gemm_tile_body<T, BM, BN, BK, TA, TB, true, NBUF>(p, m0, n0, smem);
+    else gemm_tile_body<T, BM, BN, BK, TA, TB, false, NBUF>(p, m0, n0, smem);
 }
 
 template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
@@ -396,8 +378,7 @@ __global__ __launch_bounds__(256, WAVES_PER_EU) void syrk_update_kernel(GemmProb
     int ti, tj;
     if (!map_tile(o, blockIdx.x, ti, tj)) return;
     __shared__ T smem[lds_elems<BM, BN, BK, NBUF>()];
-    if (p.alpha == T(1) || p.alpha == T(-1)) gemm_tile<T, BM, BN, BK, TA, TB, NBUF, true>(p, ti * BM, tj * BN, smem);
-    else gemm_tile<T, BM, BN, BK, TA, TB, NBUF, false>(p, ti * BM, tj * BN, smem);
+    gemm_tile<T, BM, BN, BK, TA, TB, NBUF>(p, ti * BM, tj * BN, smem);
 }
 
 template <class T>
