@@ -276,10 +276,12 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
             for (int t = 0; t < TM; ++t) a[t] = as[kr * LA + t * 16];
 #pragma unroll
             for (int t = 0; t < TN; ++t) b[t] = bs[kr * LB + t * 16];
+            // D^T = op(B) op(A)^T: lane l % 16 of an accumulator runs along i, the
+            // contiguous dimension of C
 #pragma unroll
             for (int x = 0; x < TM; ++x)
 #pragma unroll
-                for (int y = 0; y < TN; ++y) acc[x][y] = Op::mma(a[x], b[y], acc[x][y]);
+                for (int y = 0; y < TN; ++y) acc[x][y] = Op::mma(b[y], a[x], acc[x][y]);
         }
     };
 
@@ -329,14 +331,51 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
         }
     }
 
+    // Epilogue.  Accumulator (x, y, r) of lane l holds C(i, j) with
+    // i = 16x + l % 16 and j = 16y + row(l, r) within the wavefront's block.
+    // Interior tiles (in bounds and wholly on the written side of a triangle
+    // mask) read a row of 16 x 16 blocks of C at a time before writing it, so
+    // the reads of that row are in flight together (a write between two reads
+    // would have to complete first, since the compiler cannot tell the
+    // addresses apart); the other tiles go element by element.
+    const int ci = wm * (BM / 2) + (lane & 15);
+    bool interior = full;
+    if (p.tri == kTriLowerD) interior = interior && (long long)m0 + p.diag >= n0 + BN - 1;
+    if (p.tri == kTriUpperD) interior = interior && (long long)m0 + BM - 1 + p.diag <= n0;
+    if (interior) {
+        T* Ct = p.C + m0 + ci + (long long)(n0 + wn * (BN / 2)) * p.ldc;
+        auto off = [&](int x, int y, int r) { return x * 16 + (long long)(y * 16 + Op::row(lane, r)) * p.ldc; };
+        if (p.beta != T(0)) {
+#pragma unroll
+            for (int x = 0; x < TM; ++x) {
+                T cv[TN][4];
+#pragma unroll
+                for (int y = 0; y < TN; ++y)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) cv[y][r] = Ct[off(x, y, r)];
+#pragma unroll
+                for (int y = 0; y < TN; ++y)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Ct[off(x, y, r)] = p.alpha * acc[x][y][r] + p.beta * cv[y][r];
+            }
+        } else {
+#pragma unroll
+            for (int x = 0; x < TM; ++x)
+#pragma unroll
+                for (int y = 0; y < TN; ++y)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Ct[off(x, y, r)] = p.alpha * acc[x][y][r];
+        }
+        return;
+    }
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
         for (int y = 0; y < TN; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = m0 + wm * (BM / 2) + x * 16 + Op::row(lane, r);
-                const int j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
+                const int i = m0 + ci + x * 16;
+                const int j = n0 + wn * (BN / 2) + y * 16 + Op::row(lane, r);
                 if (!full && (i >= p.m || j >= p.n)) continue;
                 if (p.tri == kTriLowerD && (long long)i + p.diag < j) continue;
                 if (p.tri == kTriUpperD && (long long)i + p.diag > j) continue;
