@@ -153,7 +153,9 @@ void gemmVbatched2s(const DdlaHandle_t& handle, char transA_0, char transB_0, in
                     int segment_count);
 
 // Mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
-// semantics): factor in FP32 (factor = 0) on the matrix cores, refine in FP64
+// semantics): factor in FP32 on the matrix cores -- trailing updates in FP32
+// (factor = 0) or as split bfloat16 products accumulated in FP32, bf16x3
+// (factor = 1, ~2^-16 relative) or bf16x6 (factor = 2, ~FP32) -- refine in FP64
 // until every column has ||r||_inf <= tol ||A||_inf ||x||_inf, at most
 // max_iter corrections; otherwise (or if the FP32 factor fails) run pposv.
 // iter = corrections applied, -3 (FP32 factor failed) or -(max_iter + 1)

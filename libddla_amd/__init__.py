@@ -27,6 +27,9 @@ from .batched import (
 )
 from .matrix import DistributedMatrix, descinit
 from .ops import (
+    FACTOR_BF16X3,
+    FACTOR_BF16X6,
+    FACTOR_FP32,
     diag_logdet,
     ppotrf,
     ppotrf_run,
@@ -45,6 +48,7 @@ __all__ = [
     "LIBRARY_PATH", "lib", "check_status", "Backend", "Handle", "DistributedMatrix", "descinit",
     "ppotrf", "ppotrs", "pposv", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
     "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
+    "FACTOR_FP32", "FACTOR_BF16X3", "FACTOR_BF16X6",
     "diag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",
     "pposv_vbatched_workspace_size", "DdlaError", "DdlaInvalidHandleError",
     "DdlaInvalidValueError", "DdlaAllocFailedError", "DdlaBackendUnavailableError", "DdlaInternalError",

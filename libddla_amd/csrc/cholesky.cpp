@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstdint>
 #include <limits>
+#include <stdexcept>
+#include <type_traits>
 
 namespace ddla {
 
@@ -77,39 +79,90 @@ void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k
     kern::panel_factor<T>(st, v.upper, n, v.A, v.lda, k, kb, info, work);
 }
 
+// BF16 split-product trailing updates (FP32 factorization only): after panel
+// k is factored, its rows below the diagonal block are split into bfloat16
+// planes in buf[(k / NB) % 2] (split_buffer_bytes(n, products) bytes each), and
+// both trailing updates of step k read that buffer.  Two buffers: the split of
+// panel k + 1 (side stream) overlaps the update of step k (main stream).
+struct SplitPlan {
+    int products;  // 3 (bf16x3) or 6 (bf16x6)
+    void* buf[2];
+};
+
+// C_L(r + a_row.., r + b_row..) -= P P^T over the split panel of step k (rows
+// r.. of the L view split into buf).
+template <class T>
+void split_update_lower(DdlaStream* h, hipStream_t st, const LView<T>& v, const SplitPlan& sp, const void* buf,
+                        int m_split, int M, int N, int kb, long long r, int a_row, int b_row, const int* abort,
+                        double flops)
+{
+    if (M <= 0 || N <= 0) return;
+    LaunchTimer timer(h, kClassTrailingUpdate, flops, 0, st);
+    if constexpr (std::is_same<T, float>::value) {
+        T* C = v.upper ? v.at(r + b_row, r + a_row) : v.at(r + a_row, r + b_row);
+        kern::split_update(st, sp.products, v.upper, M, N, kb, buf, m_split, a_row, b_row, C, v.lda, true,
+                           (long long)a_row - b_row, abort);
+    } else {
+        throw std::logic_error("split-product updates apply to the FP32 factorization");
+    }
+}
+
 // Blocked right-looking Cholesky of the order-n L view with look-ahead; *info
 // (device) must be zero on entry and receives the 1-based index of the first
 // failed pivot; work holds kern::panel_work_bytes<T>(n) bytes.  After panel k is factored, the trailing update is split: the
 // columns of panel k+1 are updated first on the handle's stream, panel k+1 is
 // then factored on the side stream while the rest of the trailing matrix is
 // updated on the handle's stream (the two touch disjoint columns).  Returns
-// with all work ordered on the handle's stream.
+// with all work ordered on the handle's stream.  split (FP32 only, optional):
+// the trailing updates run as BF16 split products.
 template <class T>
-void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work)
+void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work,
+                   const SplitPlan* split = nullptr)
 {
     const LView<T> v{A, lda, upper};
     const int NB = outer_block<T>();
     hipStream_t main = h->stream, side = h->side;
+    // splits rows r.. of the panel at columns [k, k + kb) into the buffer of step k
+    auto split_panel = [&](hipStream_t st, int k, int kb) {
+        const long long r = (long long)k + kb;
+        const int m = n - (int)r;
+        if (!split || m <= 0) return;
+        const T* X = upper ? v.at(k, r) : v.at(r, k);
+        if constexpr (std::is_same<T, float>::value)
+            kern::split_panel(st, split->products, upper, m, kb, X, lda, split->buf[(k / NB) % 2]);
+    };
     kern::panel_reset<T>(main, n, work);
     factor_panel<T>(h, main, v, n, 0, std::min(NB, n), info, work);
+    split_panel(main, 0, std::min(NB, n));
     for (int k = 0; k < n; k += NB) {
         const int kb = std::min(NB, n - k);
         const int m = n - k - kb;
         if (m <= 0) break;
         const int kb2 = std::min(NB, m);  // width of the next panel
         const long long r = (long long)k + kb;
+        const double next_flops = (double)kb2 * (kb2 + 1) / 2 * 2 * kb + 2.0 * (m - kb2) * kb2 * kb;
         // next panel's columns: rows r.., columns [r, r + kb2)
-        update_lower<T>(h, main, kClassTrailingUpdate, v, m, kb2, kb, T(-1), r, k, r, k, T(1), r, r, true, info,
-                        (double)kb2 * (kb2 + 1) / 2 * 2 * kb + 2.0 * (m - kb2) * kb2 * kb);
+        if (split)
+            split_update_lower<T>(h, main, v, *split, split->buf[(k / NB) % 2], m, m, kb2, kb, r, 0, 0, info,
+                                  next_flops);
+        else
+            update_lower<T>(h, main, kClassTrailingUpdate, v, m, kb2, kb, T(-1), r, k, r, k, T(1), r, r, true, info,
+                            next_flops);
         DDLA_HIP(hipEventRecord(h->ev_update, main));
         DDLA_HIP(hipStreamWaitEvent(side, h->ev_update, 0));
         factor_panel<T>(h, side, v, n, (int)r, kb2, info, work);
+        split_panel(side, (int)r, kb2);
         DDLA_HIP(hipEventRecord(h->ev_panel, side));
         // the rest of the trailing matrix: rows and columns [r + kb2, n)
         const int mr = m - kb2;
-        if (mr > 0)
-            update_lower<T>(h, main, kClassTrailingUpdate, v, mr, mr, kb, T(-1), r + kb2, k, r + kb2, k, T(1),
-                            r + kb2, r + kb2, true, info, (double)mr * (mr + 1) * kb);
+        if (mr > 0) {
+            if (split)
+                split_update_lower<T>(h, main, v, *split, split->buf[(k / NB) % 2], m, mr, mr, kb, r, kb2, kb2, info,
+                                      (double)mr * (mr + 1) * kb);
+            else
+                update_lower<T>(h, main, kClassTrailingUpdate, v, mr, mr, kb, T(-1), r + kb2, k, r + kb2, k, T(1),
+                                r + kb2, r + kb2, true, info, (double)mr * (mr + 1) * kb);
+        }
         DDLA_HIP(hipStreamWaitEvent(main, h->ev_panel, 0));
     }
 }
@@ -487,6 +540,13 @@ void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long ld
 }
 template void potrf_blocked_dispatch<float>(DdlaStream*, bool, int, float*, long long, int*, void*);
 template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, long long, int*, void*);
+
+void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long lda, int* info, void* work,
+                          int products, void* buf0, void* buf1)
+{
+    const SplitPlan plan{products, {buf0, buf1}};
+    potrf_blocked<float>(h, upper, n, A, lda, info, work, &plan);
+}
 
 #define DDLA_INSTANTIATE(T)                                                                                    \
     template void ppotrf<T>(const DdlaHandle_t&, char, int, T*, int, int, const int*, int&);                  \

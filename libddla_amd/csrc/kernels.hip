@@ -11,6 +11,7 @@
 #include "potf2_tile.h"
 #include "panel_tile.h"
 #include "batched_tile.h"
+#include "split_tile.h"
 
 #include <cfloat>
 #include <cstdint>
@@ -391,6 +392,71 @@ void gemm_vbatched(hipStream_t s, char ta, char tb, const int* m, const int* n, 
     if (batch <= 0 || max_m <= 0 || max_n <= 0) return;
     VBatch<T> v{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc};
     launch_vbatched_cfg<T, 64, 64, 16>(s, !is_n(ta), !is_n(tb), v, batch, max_m, max_n);
+}
+
+int split_rows(int m) { return (m + split_detail::kTile - 1) / split_detail::kTile * split_detail::kTile + split_detail::kTile; }
+
+size_t split_buffer_bytes(int n, int products)
+{
+    const int planes = products == 6 ? 3 : 2;
+    return (size_t)split_rows(n) * 512 * planes * 2;
+}
+
+namespace {
+void check_products(int products)
+{
+    if (products != 3 && products != 6) throw std::invalid_argument("split update: products must be 3 or 6");
+}
+} // namespace
+
+void split_panel(hipStream_t s, int products, bool upper, int m, int kb, const float* X, long long ldx, void* out)
+{
+    check_products(products);
+    if (m <= 0 || kb <= 0) return;
+    if (kb > 512) throw std::invalid_argument("split_panel: at most 512 columns");
+    const int rows = split_rows(m), nks = (kb + split_detail::kSlab - 1) / split_detail::kSlab;
+    const dim3 grid((unsigned)((rows + 255) / 256), (unsigned)(4 * nks)), block(256);
+    auto* o = static_cast<split_detail::u4_t*>(out);
+    if (products == 6) split_detail::split_panel_kernel<3><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, o);
+    else split_detail::split_panel_kernel<2><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, o);
+    check_launch("split_panel");
+}
+
+void split_update(hipStream_t s, int products, bool upper, int m, int n2, int kb, const void* P, int m_split,
+                  int a_row, int b_row, float* C, long long ldc, bool tri, long long diag, const int* abort)
+{
+    check_products(products);
+    if (m <= 0 || n2 <= 0 || kb <= 0) return;
+    using split_detail::kTile;
+    const int planes = products == 6 ? 3 : 2, rows = split_rows(m_split);
+    const int ntm = (m + kTile - 1) / kTile, ntn = (n2 + kTile - 1) / kTile;
+    if (kb > 512 || a_row % 16 != 0 || b_row % 16 != 0 || a_row < 0 || b_row < 0 ||
+        a_row + (long long)ntm * kTile > rows || b_row + (long long)ntn * kTile > rows)
+        throw std::invalid_argument("split_update: rows outside the split buffer");
+    if ((long long)kTile * ldc >= INT_MAX) throw std::invalid_argument("split_update: leading dimension too large");
+    const int nks = (kb + split_detail::kSlab - 1) / split_detail::kSlab;
+    const long long rb_units = (long long)nks * planes * split_detail::kFragU4;  // 16-byte units per 16-row block
+    const auto* base = static_cast<const split_detail::u4_t*>(P);
+    split_detail::SplitProblem p{m, n2, nks, base + (a_row / 16) * rb_units, base + (b_row / 16) * rb_units, C,
+                                 ldc, tri ? 1 : 0, diag};
+    TileOrder o;
+    o.ntm = ntm;
+    o.ntn = ntn;
+    o.lower_enum = (tri && diag == 0 && m == n2) ? 1 : 0;
+    const long long total = o.lower_enum ? (long long)ntm * (ntm + 1) / 2 : (long long)ntm * ntn;
+    if (total > (long long)INT_MAX / 2) throw std::invalid_argument("split_update: problem too large for one launch");
+    o.total = (int)total;
+    o.group = 8;
+    o.per_xcd = (o.total + 7) / 8;
+    const dim3 grid((unsigned)(o.per_xcd * 8)), block(256);
+    if (products == 3) {
+        if (upper) split_detail::split_syrk_kernel<3, true, 2><<<grid, block, 0, s>>>(p, o, abort);
+        else split_detail::split_syrk_kernel<3, false, 2><<<grid, block, 0, s>>>(p, o, abort);
+    } else {
+        if (upper) split_detail::split_syrk_kernel<6, true, 1><<<grid, block, 0, s>>>(p, o, abort);
+        else split_detail::split_syrk_kernel<6, false, 1><<<grid, block, 0, s>>>(p, o, abort);
+    }
+    check_launch("split_update");
 }
 
 void vbatched_validate(hipStream_t s, int na, int nb, const int* m, const int* n, const int* k, const int* lda,

@@ -21,6 +21,8 @@ namespace ddla {
 // Schedules shared with cholesky.cpp.
 template <class T>
 void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work);
+void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long lda, int* info, void* work,
+                          int products, void* buf0, void* buf1);
 
 namespace {
 
@@ -29,7 +31,7 @@ constexpr int kIB = 64;
 std::size_t diag_inverse_elems(int n) { return (std::size_t)((n + kIB - 1) / kIB) * kIB * kIB; }
 
 struct MixedLayout {
-    std::size_t a32, norm, scalars, x, r, w, info, panel, dinv, sync, error, total;
+    std::size_t a32, norm, scalars, x, r, w, info, panel, dinv, sync, error, split0, split1, total;
 };
 
 MixedLayout mixed_layout(int n, int nrhs)
@@ -53,13 +55,16 @@ MixedLayout mixed_layout(int n, int nrhs)
     l.dinv = put(diag_inverse_elems(n) * sizeof(float));
     l.sync = put(kern::solve_work_bytes(nrhs));
     l.error = put(sizeof(int));
+    l.split0 = put(kern::split_buffer_bytes(n, 6));
+    l.split1 = put(kern::split_buffer_bytes(n, 6));
     l.total = off;
     return l;
 }
 
 void check_mixed_options(int factor, double tol, int max_iter)
 {
-    if (factor != 0) throw std::invalid_argument("pposv_mixed: factor precision must be 0 (FP32)");
+    if (factor < 0 || factor > 2)
+        throw std::invalid_argument("pposv_mixed: factor must be 0 (FP32), 1 (bf16x3) or 2 (bf16x6)");
     if (!(tol > 0) || max_iter < 0) throw std::invalid_argument("pposv_mixed: tol must be > 0 and max_iter >= 0");
 }
 
@@ -119,7 +124,11 @@ void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, co
         kern::convert_sym_with_norm<float, double>(s, n, !upper, A, lda, a32, n, base + l.norm, scalars);
     }
     fill_int(s, info32, 0);
-    potrf_blocked_dispatch<float>(h, upper, n, a32, n, info32, base + l.panel);
+    if (factor == 0)
+        potrf_blocked_dispatch<float>(h, upper, n, a32, n, info32, base + l.panel);
+    else
+        potrf_split_dispatch(h, upper, n, a32, n, info32, base + l.panel, factor == 1 ? 3 : 6, base + l.split0,
+                             base + l.split1);
     int host_info = 0, panel_error = 0;
     DDLA_HIP(hipMemcpyAsync(&host_info, info32, sizeof(int), hipMemcpyDeviceToHost, s));
     DDLA_HIP(hipMemcpyAsync(&panel_error, kern::panel_error_word<float>(n, base + l.panel), sizeof(int),

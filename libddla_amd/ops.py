@@ -89,6 +89,8 @@ def pposv_run(A, B, uplo, backend_work: int, backend_bytes: int, side: str = "L"
 
 
 FACTOR_FP32 = 0
+FACTOR_BF16X3 = 1  # trailing updates as three bfloat16 split products (FP32 accumulation)
+FACTOR_BF16X6 = 2  # six split products, about FP32 accuracy
 
 
 def _mixed_args(A, B, uplo):
@@ -99,6 +101,10 @@ def _mixed_args(A, B, uplo):
 
 def pposv_mixed(A, B, uplo: str = "L", factor: int = FACTOR_FP32, tol: float = None, max_iter: int = 30):
     """FP64 A X = B by an FP32 factorization plus FP64 iterative refinement (LAPACK dsposv semantics).
+
+    factor selects the trailing updates of the FP32 factorization: FACTOR_FP32
+    (FP32 MFMA), FACTOR_BF16X3 or FACTOR_BF16X6 (split bfloat16 products,
+    FP32 accumulation).
 
     Returns (B, iter, info): iter >= 0 is the number of corrections applied;
     -3 (FP32 factor failed) or -(max_iter + 1) (no convergence) mean the FP64

@@ -46,17 +46,21 @@ def stored(a, uplo):
     return np.where(mask, a, np.nan)
 
 
+FACTORS = [dd.FACTOR_FP32, dd.FACTOR_BF16X3, dd.FACTOR_BF16X6]
+
+
+@pytest.mark.parametrize("factor", FACTORS)
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("delta", [4e-2, 4e-4, 4e-6])
-def test_mixed_converges_to_fp64_accuracy(handle, uplo, delta):
+def test_mixed_converges_to_fp64_accuracy(handle, uplo, delta, factor):
     n, nrhs, tol = 777, 5, 1e-14
-    print(f"case mixed n={n} delta={delta} uplo={uplo}", flush=True)
+    print(f"case mixed n={n} delta={delta} uplo={uplo} factor={factor}", flush=True)
     a = orc.spd_matrix(n, 2, delta)
     b = orc.normal_matrix(2, 1, n, nrhs)
     A = upload(handle, stored(a, uplo), lld=n + 5)
     before = A.to_host().tobytes(order="F")
     B = upload(handle, b)
-    _, it, info = dd.pposv_mixed(A, B, uplo, tol=tol)
+    _, it, info = dd.pposv_mixed(A, B, uplo, factor=factor, tol=tol)
     x = B.to_global_matrix()
     assert info == 0
     assert 0 <= it <= 30
@@ -66,6 +70,28 @@ def test_mixed_converges_to_fp64_accuracy(handle, uplo, delta):
     xo = oracle_solution(a, b)
     fwd = np.abs(x - xo).max() / np.abs(xo).max()
     assert fwd <= 100 * kappa * EPS, (fwd, kappa)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_split_factor_modes_three_panels(handle, uplo):
+    """n = 1500 (three 512-column panels; the second trailing update starts 512
+    rows into the split panel, and the last tiles are ragged): every factor
+    mode reaches 1e-14; bf16x6 (about FP32 accuracy) needs at most one
+    correction more than FP32, and bf16x3 (about 2^-16) at least as many as
+    bf16x6."""
+    n, nrhs, tol = 1500, 3, 1e-14
+    a = orc.spd_matrix(n, 8, 4e-3)
+    b = orc.normal_matrix(8, 1, n, nrhs)
+    its = {}
+    for factor in FACTORS:
+        print(f"case mixed n={n} uplo={uplo} factor={factor}", flush=True)
+        A, B = upload(handle, stored(a, uplo)), upload(handle, b)
+        _, it, info = dd.pposv_mixed(A, B, uplo, factor=factor, tol=tol)
+        assert info == 0 and 1 <= it <= 30, (factor, it)
+        assert backward_errors(a, B.to_global_matrix(), b).max() <= tol
+        its[factor] = it
+    assert its[dd.FACTOR_BF16X6] <= its[dd.FACTOR_FP32] + 1, its
+    assert its[dd.FACTOR_BF16X3] >= its[dd.FACTOR_BF16X6], its
 
 
 def test_iterations_grow_with_condition(handle):
@@ -167,8 +193,9 @@ def test_workspace_api_matches_driver(handle):
 def test_invalid_arguments(handle):
     a = orc.spd_matrix(64, 1, 1.0)
     A, B = upload(handle, a), upload(handle, np.ones((64, 1)))
-    with pytest.raises(dd.DdlaExceptionError):
-        dd.pposv_mixed(A, B, factor=1)
+    for factor in (3, -1):
+        with pytest.raises(dd.DdlaExceptionError):
+            dd.pposv_mixed(A, B, factor=factor)
     with pytest.raises(dd.DdlaExceptionError):
         dd.pposv_mixed(A, B, tol=0.0)
     with pytest.raises(dd.DdlaExceptionError):
