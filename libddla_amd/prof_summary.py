@@ -1,10 +1,12 @@
 """Summaries of rocprofv3 CSV output for the Cholesky path (used to write profiles/).
 
     python -m libddla_amd.prof_summary --trace DIR [--fetch DIR --write DIR]
-        [--n 32768 --nb 512] [--command "..."] [--md OUT.md] [--traffic-json OUT.json]
+        [--n 32768 --nb 512] [--focus NAME ...] [--command "..."] [--md OUT.md]
+        [--traffic-json OUT.json]
 
 --trace is the output directory of `rocprofv3 --kernel-trace --stats
---output-format csv`; --fetch / --write those of `--pmc FETCH_SIZE` and
+--output-format csv`; --focus names the trailing-update kernels to report with
+their algorithmic rate; --fetch / --write those of `--pmc FETCH_SIZE` and
 `--pmc WRITE_SIZE` runs of the same program.  The summary lists every kernel
 by total time and, for the trailing update (syrk_update_kernel), its average
 launch duration and HBM traffic per launch.  FETCH_SIZE and WRITE_SIZE (KiB
@@ -63,6 +65,24 @@ def traffic(fetch_dir, write_dir, n):
     return f_scale, w_scale, sum(fs) * 1024.0 * f_scale / len(fs), sum(ws) * 1024.0 * w_scale / len(ws), len(fs)
 
 
+def trailing_flops_per_launch(n, nb):
+    """Mean algorithmic flops over the trailing-update launches of one factorization: U_next
+    kb2 (kb2 + 1) kb + 2 (m - kb2) kb2 kb, U_rest m_r (m_r + 1) kb."""
+    total, launches = 0.0, 0
+    for k in range(0, n, nb):
+        m = n - k - nb
+        if m <= 0:
+            break
+        kb2 = min(nb, m)
+        total += kb2 * (kb2 + 1) * nb + 2.0 * (m - kb2) * kb2 * nb
+        launches += 1
+        mr = m - kb2
+        if mr > 0:
+            total += mr * (mr + 1.0) * nb
+            launches += 1
+    return total / launches, launches
+
+
 def algorithmic_bytes_per_launch(n, nb):
     """Mean over the trailing-update launches of one factorization: the panel (m x nb) read once and the
     lower triangle of C read and written (U_next: m x kb2 block, U_rest: the rest)."""
@@ -89,6 +109,8 @@ def main():
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nb", type=int, default=512)
     ap.add_argument("--command", default="")
+    ap.add_argument("--focus", nargs="+", default=["syrk_update_kernel"],
+                    help="name substrings of the trailing-update kernels to report")
     ap.add_argument("--md")
     ap.add_argument("--traffic-json")
     args = ap.parse_args()
@@ -101,11 +123,13 @@ def main():
     for s in stats[:16]:
         lines.append(f"| `{short_name(s['name'])}` | {s['calls']} | {s['total_ms']:.3f} | {s['avg_us']:.1f} | "
                      f"{s['pct']:.2f} |")
-    syrk = [s for s in stats if "syrk_update_kernel" in s["name"]]
-    if syrk:
-        s = syrk[0]
-        lines += ["", f"Trailing update (`syrk_update_kernel`): {s['calls']} launches, average "
-                      f"{s['avg_us'] / 1e3:.4f} ms per launch."]
+    flops, per_fact = trailing_flops_per_launch(args.n, args.nb)
+    for focus in args.focus:
+        for s in [s for s in stats if focus in s["name"]]:
+            lines += ["", f"Trailing update `{short_name(s['name'])}`: {s['calls']} launches, average "
+                          f"{s['avg_us'] / 1e3:.4f} ms per launch; {per_fact} launches per factorization at "
+                          f"n = {args.n}, nb = {args.nb}, mean {flops / 1e9:.1f} GFLOP (FP32- or FP64-equivalent) "
+                          f"per launch -> {flops / (s['avg_us'] * 1e-6) / 1e12:.1f} TFLOP/s."]
     result = {}
     if args.fetch and args.write:
         f_scale, w_scale, fb, wb, nl = traffic(args.fetch, args.write, args.n)
