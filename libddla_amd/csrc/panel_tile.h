@@ -20,8 +20,9 @@
 // progresses; each wait is still bounded (spin limit -> error word, and the
 // panel is abandoned).  A failed pivot sets *info and a failure flag that
 // releases every waiter.
-// Publication: stores, agent-scope release fence, release store of the
-// counter; readers poll relaxed and read blocks published by other
+// Publication: every wavefront's stores acknowledged, a barrier, then one
+// thread's agent-scope release fence (L2 write-back, waited for) and the
+// counter store; readers poll relaxed and read blocks published by other
 // workgroups with agent-scope atomic loads (never served from a stale L1 or
 // L2 line).  A workgroup reads back the blocks it wrote itself with plain
 // loads after a barrier (same-workgroup visibility).
@@ -218,10 +219,17 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> a)
         __syncthreads();
         return s_flag;
     };
+    // every wavefront's stores acknowledged, then one L2 write-back (release
+    // fence), waited for explicitly before the counter moves (the compiler
+    // emits no wait between the write-back and a following store)
     auto publish = [&](int value) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(progress + t, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(progress + t, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     };
 
     const bool diag_row = t < P;
