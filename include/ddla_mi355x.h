@@ -168,7 +168,10 @@ int ddla_d_gemm_vbatched2s(void* handle, char transA_0, char transB_0, int* d_m_
  * resident; asynchronous on the handle's stream.  _run allocates and
  * synchronizes nothing (graph-capturable); _tune times the schedules on
  * copies of the inputs and keeps the fastest in the handle; _graph captures
- * one call into a replayable hipGraph owning its workspace. -------------- */
+ * one call into a replayable hipGraph owning its workspace.  Schedules: the
+ * persistent fused kernel for every member, or -- for the members of more
+ * than split_blocks 64-row blocks -- a blocked factorization whose trailing
+ * updates run as one gemmVbatched launch per 64-column step. ------------ */
 int ddla_s_pposv_vbatched(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, int nrhs,
                           void* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch_count,
                           int max_n);
@@ -197,6 +200,15 @@ int ddla_d_pposv_vbatched_graph(void* handle, char uplo, const int* d_n, void* c
                                 int batch_count, int max_n, void** graph);
 int ddla_c_graph_launch(void* graph, void* handle);
 int ddla_c_graph_destroy(void* graph);
+/* The batched schedule the handle holds for FP32 (element_bytes 4) or FP64
+ * (8): *tuned = 1 once _tune (or _graph) chose it; workgroups per CU of the
+ * persistent kernel, largest-first order, and split_blocks (16: no blocked
+ * path). */
+int ddla_c_vbatched_schedule(void* handle, int element_bytes, int* tuned, int* occupancy, int* ordered,
+                             int* split_blocks);
+/* Pins that schedule without timing (occupancy 1 or 2; split_blocks in
+ * [1, 16], used only with ordered != 0); marks it tuned. */
+int ddla_c_vbatched_set_schedule(void* handle, int element_bytes, int occupancy, int ordered, int split_blocks);
 
 /* ---- reverse-mode adjoints (no counterpart in the reference), factor of
  * A = L L^T in the lower triangle (uplo 'L'); each returns synchronized:

@@ -24,6 +24,8 @@ from .batched import (
     pposv_vbatched_run,
     pposv_vbatched_tune,
     pposv_vbatched_workspace_size,
+    set_vbatched_schedule,
+    vbatched_schedule,
 )
 from .matrix import DistributedMatrix, descinit
 from .ops import (
@@ -50,7 +52,7 @@ __all__ = [
     "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
     "FACTOR_FP32", "FACTOR_BF16X3", "FACTOR_BF16X6",
     "diag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",
-    "pposv_vbatched_workspace_size", "DdlaError", "DdlaInvalidHandleError",
+    "pposv_vbatched_workspace_size", "vbatched_schedule", "set_vbatched_schedule", "DdlaError", "DdlaInvalidHandleError",
     "DdlaInvalidValueError", "DdlaAllocFailedError", "DdlaBackendUnavailableError", "DdlaInternalError",
     "DdlaExceptionError",
 ]

@@ -109,6 +109,8 @@ _SIGNATURES = {
                                        _ip, _vp]),
     "ddla_c_graph_launch": (_i, [_vp, _vp]),
     "ddla_c_graph_destroy": (_i, [_vp]),
+    "ddla_c_vbatched_schedule": (_i, [_vp, _i, _ip, _ip, _ip, _ip]),
+    "ddla_c_vbatched_set_schedule": (_i, [_vp, _i, _i, _i, _i]),
     "ddla_c_profile": (_i, [_vp, _i]),
     "ddla_c_profile_read": (_i, [_vp, _i, _ip, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_double)]),

@@ -184,6 +184,26 @@ def pposv_vbatched_tune(vb: VBatch, uplo: str = "L", max_n: int = None):
     return occ.value, bool(ordered.value)
 
 
+def vbatched_schedule(handle, dtype=np.float64):
+    """The batched schedule the handle holds for dtype: {"tuned", "workgroups_per_cu", "largest_first",
+    "split_blocks"} (members of more than split_blocks 64-row blocks take the blocked path with gemmVbatched
+    trailing updates; 16 means none)."""
+    t, occ, ordered, split = (ctypes.c_int() for _ in range(4))
+    check_status(lib.ddla_c_vbatched_schedule(handle.ptr, np.dtype(dtype).itemsize, ctypes.byref(t),
+                                              ctypes.byref(occ), ctypes.byref(ordered), ctypes.byref(split)),
+                 "ddla_c_vbatched_schedule")
+    return {"tuned": bool(t.value), "workgroups_per_cu": occ.value, "largest_first": bool(ordered.value),
+            "split_blocks": split.value}
+
+
+def set_vbatched_schedule(handle, dtype=np.float64, workgroups_per_cu: int = 1, largest_first: bool = True,
+                          split_blocks: int = 16):
+    """Pins the batched schedule for dtype without timing (see vbatched_schedule)."""
+    check_status(lib.ddla_c_vbatched_set_schedule(handle.ptr, np.dtype(dtype).itemsize, workgroups_per_cu,
+                                                  1 if largest_first else 0, split_blocks),
+                 "ddla_c_vbatched_set_schedule")
+
+
 class BatchedGraph:
     """One batched call captured into a hipGraph (replay with launch())."""
 

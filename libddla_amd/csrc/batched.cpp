@@ -19,6 +19,8 @@ namespace ddla {
 namespace {
 
 constexpr int kBlk = 64;
+// untuned default: members of more than 4 blocks (order > 256) take the blocked path
+constexpr int kDefaultSplitBlocks = 4;
 
 int max_blocks_for(int max_n) { return std::max(1, (max_n + kBlk - 1) / kBlk); }
 
@@ -34,13 +36,17 @@ DdlaStream::BatchedTuning& tuning_of(const DdlaHandle_t& h)
     return h->batched_tuning[sizeof(T) == sizeof(double) ? 1 : 0];
 }
 
+// the blocked path can apply (some member may have more than one block)
+bool blocked_possible(int max_n) { return max_blocks_for(max_n) > 1; }
+
 template <class T>
 void launch_run(const DdlaHandle_t& h, hipStream_t s, char uplo, const int* d_n, T* const* d_A, const int* d_lda,
                 int nrhs, T* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch, int max_n,
-                int occupancy, bool ordered, void* work)
+                int occupancy, bool ordered, int split_blocks, void* work)
 {
     kern::batched_pposv<T>(s, uplo == 'U', batch, nrhs, d_n, d_A, d_lda, d_B, d_ldb, d_info, d_logdet,
-                           grid_for_batch(h, batch, occupancy), max_blocks_for(max_n), ordered, occupancy, work);
+                           grid_for_batch(h, batch, occupancy), max_blocks_for(max_n), ordered, occupancy,
+                           split_blocks, work);
 }
 
 } // namespace
@@ -67,8 +73,9 @@ void pposv_vbatched_workspace_size(const DdlaHandle_t& h, int batch, int max_n, 
     require_gpu(h, "pposvVbatched");
     *backend_bytes = 0;
     if (batch <= 0) return;
-    // sized for the largest grid any variant uses (two workgroups per CU)
-    *backend_bytes = kern::batched_work_bytes(sizeof(T), batch, grid_for_batch(h, batch, 2), max_blocks_for(max_n));
+    // sized for the largest grid any variant uses (two workgroups per CU) and for the blocked path
+    *backend_bytes = kern::batched_work_bytes(sizeof(T), batch, grid_for_batch(h, batch, 2), max_blocks_for(max_n),
+                                              blocked_possible(max_n));
 }
 
 template <class T>
@@ -83,7 +90,8 @@ void pposv_vbatched_run(const DdlaHandle_t& h, char uplo, const int* d_n, T* con
         throw std::invalid_argument("pposvVbatched: backend workspace smaller than the workspace query");
     const auto& t = tuning_of<T>(h);
     launch_run<T>(h, h->stream, uplo, d_n, d_A, d_lda, nrhs, d_B, d_ldb, d_info, d_logdet, batch, max_n,
-                  t.tuned ? t.occupancy : 1, t.tuned ? t.ordered : true, backend_work);
+                  t.tuned ? t.occupancy : 1, t.tuned ? t.ordered : true,
+                  t.tuned ? t.split_blocks : kDefaultSplitBlocks, backend_work);
 }
 
 } // namespace cholesky_impl
@@ -102,8 +110,10 @@ void pposvVbatched(const DdlaHandle_t& h, char uplo, const int* d_n, T* const* d
 }
 
 // Times every schedule (kernel variant for 1 or 2 workgroups per CU, largest
-// first or index order) on copies of the caller's matrices, so the inputs are
-// left unchanged, and keeps the fastest in the handle for later calls.
+// first or index order, and -- largest first only -- the blocked path for the
+// members of more than 2, 3, 4 or 6 blocks) on copies of the caller's
+// matrices, so the inputs are left unchanged, and keeps the fastest in the
+// handle for later calls.
 template <class T>
 void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* const* d_A, const int* d_lda, int nrhs,
                        T* const* d_B, const int* d_ldb, int batch, int max_n, int* occupancy, bool* ordered)
@@ -112,7 +122,7 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
     cholesky_impl::pposv_vbatched_check(h, uplo, batch, nrhs, max_n, d_n, d_A, d_lda, d_B, d_ldb, d_n, d_n);
     auto& tuning = tuning_of<T>(h);
     if (batch == 0) {
-        tuning = DdlaStream::BatchedTuning{true, 1, true};
+        tuning = DdlaStream::BatchedTuning{true, 1, true, kern::kBatchedMaxBlocks};
         if (occupancy) *occupancy = 1;
         if (ordered) *ordered = true;
         return;
@@ -177,9 +187,14 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
     struct Candidate {
         int occupancy;
         bool ordered;
+        int split_blocks;
         float ms;
     };
-    std::vector<Candidate> cands{{1, true, 0}, {2, true, 0}, {1, false, 0}, {2, false, 0}};
+    constexpr int kNone = kern::kBatchedMaxBlocks;
+    std::vector<Candidate> cands{{1, true, kNone, 0}, {2, true, kNone, 0}, {1, false, kNone, 0}, {2, false, kNone, 0}};
+    for (int sb : {2, 3, 4, 6})
+        if (sb < max_blocks_for(max_n))
+            for (int occ : {1, 2}) cands.push_back({occ, true, sb, 0});
     try {
         for (auto& c : cands) {
             float best = 1e30f;
@@ -188,7 +203,7 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
                 if (nrhs > 0) kern::batched_copy<T>(s, batch, dn, nrhs, const_cast<T* const*>(d_B), d_ldb, pB, dld, mx);
                 DDLA_HIP(hipEventRecord(e0, s));
                 launch_run<T>(h, s, uplo, dn, pA, dld, nrhs, pB, dld, info, logdet, batch, max_n, c.occupancy,
-                              c.ordered, work);
+                              c.ordered, c.split_blocks, work);
                 DDLA_HIP(hipEventRecord(e1, s));
                 DDLA_HIP(hipEventSynchronize(e1));
                 float ms = 0;
@@ -209,7 +224,7 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
     DDLA_HIP(hipFree(base));
     const auto best = std::min_element(cands.begin(), cands.end(),
                                        [](const Candidate& a, const Candidate& b) { return a.ms < b.ms; });
-    tuning = DdlaStream::BatchedTuning{true, best->occupancy, best->ordered};
+    tuning = DdlaStream::BatchedTuning{true, best->occupancy, best->ordered, best->split_blocks};
     if (occupancy) *occupancy = best->occupancy;
     if (ordered) *ordered = best->ordered;
 }
@@ -238,7 +253,7 @@ void* pposvVbatchedGraph(const DdlaHandle_t& h, char uplo, const int* d_n, T* co
         DDLA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         DDLA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         launch_run<T>(h, cs, uplo, d_n, d_A, d_lda, nrhs, d_B, d_ldb, d_info, d_logdet, batch, max_n, t.occupancy,
-                      t.ordered, g->work);
+                      t.ordered, t.split_blocks, g->work);
         DDLA_HIP(hipStreamEndCapture(cs, &graph));
         DDLA_HIP(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
         DDLA_HIP(hipGraphDestroy(graph));

@@ -50,6 +50,7 @@ struct BatchArgs {
     const int* order;  // processing order (a permutation of [0, batch))
     int* queue;        // work counter, zero on entry
     T* xinv;           // gridDim.x slots of max_blocks 64 x 64 inverses
+    const int* first;  // null, or the device count of leading order[] entries handled elsewhere
 };
 
 struct View {
@@ -179,6 +180,175 @@ __device__ inline void block_apply(const T* m, long long rs, long long ks, int r
     __syncthreads();
 }
 
+// Block column j of the order-n member A (L view v, nb blocks): factors the
+// diagonal block (potf2_tile.h), writes D_j = L(j, j)^-1 with exact zeros
+// above its diagonal to dj (64 x 64, column-major), and (below) forms the
+// blocks below as L(r, j) = A(r, j) D_j^T on the matrix cores, all in place.  Returns 0, or
+// the 1-based global index of a failed pivot (then only L(j, j) is written).
+// logpart (thread 0) receives sum log L(k, k) over the block (fixed-order
+// butterfly).  S, X: kB * kL elements of LDS each; rdiag: kB; s_flag: one int.
+template <class T>
+__device__ inline int factor_block_column(T* A, View v, int n, int nb, int j, T* dj, T* S, T* X, T* rdiag,
+                                          int* s_flag, double& logpart, bool below = true)
+{
+    const int tid = threadIdx.x;
+    const long long j0 = (long long)kB * j, lda = v.lda;
+    const int wj = min(kB, n - (int)j0);
+    T* Ajj = A + j0 + j0 * lda;
+    potf2_detail::load_block<T>(v.upper != 0, wj, Ajj, lda, S);
+    __syncthreads();
+    const int f = potf2_detail::factor_invert64<T>(S, X, rdiag, s_flag);
+    potf2_detail::store_factor<T>(v.upper != 0, wj, S, Ajj, lda);
+    if (f != 0) return (int)j0 + f;
+    // log of the diagonal: wave 0, fixed-order butterfly
+    if (tid < 64) {
+        double l = tid < wj ? log((double)S[tid * kL + tid]) : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+        if (tid == 0) logpart = l;
+    }
+    // D_j with exact zeros above the diagonal: kept in X for the panel, saved for the solves
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int idx = tid + kThreads * e, r = idx & (kB - 1), c = idx >> 6;
+        if (r < c) X[c * kL + r] = T(0);
+    }
+    __syncthreads();
+    potf2_detail::store_inverse<T>(X, dj);
+    // blocks below: L(r, j) = A(r, j) D_j^T
+    for (int r = j + 1; below && r < nb; ++r) {
+        const long long r0 = (long long)kB * r;
+        const int wr = min(kB, n - (int)r0);
+        load_view_block<T>(A, v, r0, j0, wr, wj, S);
+        __syncthreads();
+        Tile<T> l;
+        l.zero();
+        l.mma_nt(S, X, T(1));
+        __syncthreads();
+        l.store(S);
+        __syncthreads();
+        store_view_block<T>(A, v, r0, j0, wr, wj, false, S);
+        __syncthreads();
+    }
+    return 0;
+}
+
+// Forward then backward substitution for member A (L view v, order n, nb
+// blocks) with its inverted diagonal blocks D_j at dinv + j * 64 * 64, in
+// place on Bm (n x nrhs, leading dimension ldb), 16 columns at a time.
+// part / V: LDS scratch (kSlab * 4 * kB and kSlab * kB elements).
+template <class T>
+__device__ inline void solve_member(const T* A, View v, int n, int nb, T* Bm, long long ldb, int nrhs, const T* dinv,
+                                    T* part, T* V)
+{
+    const int tid = threadIdx.x;
+    const long long lda = v.lda;
+    for (int c0 = 0; c0 < nrhs; c0 += kSlab) {
+        const int nc = min(kSlab, nrhs - c0);
+        // forward: y_j = D_j (b_j - sum_{i<j} L(j, i) y_i), y written over b
+        for (int j = 0; j < nb; ++j) {
+            const long long j0 = (long long)kB * j;
+            const int wj = min(kB, n - (int)j0);
+            T acc[kSlab / 4];
+            {
+                const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) {
+                    const int c = q + 4 * s;
+                    acc[s] = (r < wj && c < nc) ? Bm[(j0 + r) + (c0 + c) * ldb] : T(0);
+                }
+            }
+            for (int i = 0; i < j; ++i) {
+                const long long i0 = (long long)kB * i;
+                // V = y_i
+                const int wi = min(kB, n - (int)i0);
+                for (int e = tid; e < kB * kSlab; e += kThreads) {
+                    const int k = e & (kB - 1), c = e >> 6;
+                    V[c * kB + k] = (c < nc && k < wi) ? Bm[(i0 + k) + (c0 + c) * ldb] : T(0);
+                }
+                __syncthreads();
+                T t[kSlab / 4];
+                // M = L(j, i): element (r, k) at (j0 + r, i0 + k) of the L view
+                const T* m = v.upper ? A + i0 + j0 * lda : A + j0 + i0 * lda;
+                block_apply<T>(m, v.upper ? lda : 1, v.upper ? 1 : lda, wj, wi, V, nc, part, t);
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) acc[s] -= t[s];
+            }
+            // V = acc, then y_j = D_j V
+            {
+                const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) {
+                    const int c = q + 4 * s;
+                    if (c < kSlab) V[c * kB + r] = acc[s];
+                }
+            }
+            __syncthreads();
+            T y[kSlab / 4];
+            block_apply<T>(dinv + j0 * kB, 1, kB, kB, kB, V, nc, part, y);
+            {
+                const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) {
+                    const int c = q + 4 * s;
+                    if (r < wj && c < nc) Bm[(j0 + r) + (c0 + c) * ldb] = y[s];
+                }
+            }
+            __syncthreads();
+        }
+        // backward: x_j = D_j^T (y_j - sum_{r>j} L(r, j)^T x_r)
+        for (int j = nb - 1; j >= 0; --j) {
+            const long long j0 = (long long)kB * j;
+            const int wj = min(kB, n - (int)j0);
+            T acc[kSlab / 4];
+            {
+                const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) {
+                    const int c = q + 4 * s;
+                    acc[s] = (r < wj && c < nc) ? Bm[(j0 + r) + (c0 + c) * ldb] : T(0);
+                }
+            }
+            for (int i = j + 1; i < nb; ++i) {
+                const long long i0 = (long long)kB * i;
+                const int wi = min(kB, n - (int)i0);
+                for (int e = tid; e < kB * kSlab; e += kThreads) {
+                    const int k = e & (kB - 1), c = e >> 6;
+                    V[c * kB + k] = (c < nc && k < wi) ? Bm[(i0 + k) + (c0 + c) * ldb] : T(0);
+                }
+                __syncthreads();
+                T t[kSlab / 4];
+                // M = L(i, j)^T: element (r, k) = L(i0 + k, j0 + r)
+                const T* m = v.upper ? A + j0 + i0 * lda : A + i0 + j0 * lda;
+                block_apply<T>(m, v.upper ? 1 : lda, v.upper ? lda : 1, wj, wi, V, nc, part, t);
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) acc[s] -= t[s];
+            }
+            {
+                const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) {
+                    const int c = q + 4 * s;
+                    if (c < kSlab) V[c * kB + r] = acc[s];
+                }
+            }
+            __syncthreads();
+            T x[kSlab / 4];
+            // D_j^T: element (r, k) = D_j(k, r) = slot[j][k + 64 r]
+            block_apply<T>(dinv + j0 * kB, kB, 1, kB, kB, V, nc, part, x);
+            {
+                const int r = tid & 63, q = tid >> 6;
+#pragma unroll
+                for (int s = 0; s < kSlab / 4; ++s) {
+                    const int c = q + 4 * s;
+                    if (r < wj && c < nc) Bm[(j0 + r) + (c0 + c) * ldb] = x[s];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // OCC = resident workgroups per CU the register allocation is sized for (1:
 // no spills; 2: two workgroups per CU share each SIMD, at the cost of a few
 // spilled registers); the autotuner picks one.
@@ -195,7 +365,7 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_pposv_kernel(BatchArgs<
 
     for (;;) {
         if (tid == 0) {
-            const int q = atomicAdd(a.queue, 1);
+            const int q = atomicAdd(a.queue, 1) + (a.first != nullptr ? *a.first : 0);
             s_item = q < a.batch ? a.order[q] : -1;
         }
         __syncthreads();
@@ -221,45 +391,14 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_pposv_kernel(BatchArgs<
         for (int j = 0; j < nb && fail == 0; ++j) {
             const long long j0 = (long long)kB * j;
             const int wj = min(kB, n - (int)j0);
-            T* Ajj = A + j0 + j0 * lda;
-            potf2_detail::load_block<T>(a.upper != 0, wj, Ajj, lda, S);
-            __syncthreads();
-            const int f = potf2_detail::factor_invert64<T>(S, X, rdiag, &s_flag);
-            potf2_detail::store_factor<T>(a.upper != 0, wj, S, Ajj, lda);
+            double l = 0.0;
+            const int f = factor_block_column<T>(A, v, n, nb, j, slot + (long long)j * kB * kB, S, X, rdiag, &s_flag,
+                                                 l);
             if (f != 0) {
-                fail = (int)j0 + f;
+                fail = f;
                 break;
             }
-            // log of the diagonal: wave 0, fixed-order butterfly, then thread 0 in block order
-            if (tid < 64) {
-                double l = tid < wj ? log((double)S[tid * kL + tid]) : 0.0;
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-                if (tid == 0) logsum += l;
-            }
-            // D_j with exact zeros above the diagonal: kept in X for the panel, saved for the solves
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const int idx = tid + kThreads * e, r = idx & (kB - 1), c = idx >> 6;
-                if (r < c) X[c * kL + r] = T(0);
-            }
-            __syncthreads();
-            potf2_detail::store_inverse<T>(X, slot + (long long)j * kB * kB);
-            // blocks below: L(r, j) = A(r, j) D_j^T
-            for (int r = j + 1; r < nb; ++r) {
-                const long long r0 = (long long)kB * r;
-                const int wr = min(kB, n - (int)r0);
-                load_view_block<T>(A, v, r0, j0, wr, wj, S);
-                __syncthreads();
-                Tile<T> l;
-                l.zero();
-                l.mma_nt(S, X, T(1));
-                __syncthreads();
-                l.store(S);
-                __syncthreads();
-                store_view_block<T>(A, v, r0, j0, wr, wj, false, S);
-                __syncthreads();
-            }
+            logsum += l;
             // trailing update: A(r, c) -= L(r, j) L(c, j)^T, j < c <= r
             for (int c = j + 1; c < nb; ++c) {
                 const long long c0 = (long long)kB * c;
@@ -294,116 +433,154 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_pposv_kernel(BatchArgs<
             continue;
         }
 
-        // solves, 16 right-hand-side columns at a time
-        T* Bm = a.B[item];
-        for (int c0 = 0; c0 < a.nrhs; c0 += kSlab) {
-            const int nc = min(kSlab, a.nrhs - c0);
-            // forward: y_j = D_j (b_j - sum_{i<j} L(j, i) y_i), y written over b
-            for (int j = 0; j < nb; ++j) {
-                const long long j0 = (long long)kB * j;
-                const int wj = min(kB, n - (int)j0);
-                T acc[kSlab / 4];
-                {
-                    const int r = tid & 63, q = tid >> 6;
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) {
-                        const int c = q + 4 * s;
-                        acc[s] = (r < wj && c < nc) ? Bm[(j0 + r) + (c0 + c) * ldb] : T(0);
-                    }
-                }
-                for (int i = 0; i < j; ++i) {
-                    const long long i0 = (long long)kB * i;
-                    // V = y_i
-                    const int wi = min(kB, n - (int)i0);
-                    for (int e = tid; e < kB * kSlab; e += kThreads) {
-                        const int k = e & (kB - 1), c = e >> 6;
-                        V[c * kB + k] = (c < nc && k < wi) ? Bm[(i0 + k) + (c0 + c) * ldb] : T(0);
-                    }
-                    __syncthreads();
-                    T t[kSlab / 4];
-                    // M = L(j, i): element (r, k) at (j0 + r, i0 + k) of the L view
-                    const T* m = a.upper ? A + i0 + j0 * lda : A + j0 + i0 * lda;
-                    block_apply<T>(m, a.upper ? lda : 1, a.upper ? 1 : lda, wj, wi, V, nc, part, t);
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) acc[s] -= t[s];
-                }
-                // V = acc, then y_j = D_j V
-                {
-                    const int r = tid & 63, q = tid >> 6;
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) {
-                        const int c = q + 4 * s;
-                        if (c < kSlab) V[c * kB + r] = acc[s];
-                    }
-                }
-                __syncthreads();
-                T y[kSlab / 4];
-                block_apply<T>(slot + j0 * kB, 1, kB, kB, kB, V, nc, part, y);
-                {
-                    const int r = tid & 63, q = tid >> 6;
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) {
-                        const int c = q + 4 * s;
-                        if (r < wj && c < nc) Bm[(j0 + r) + (c0 + c) * ldb] = y[s];
-                    }
-                }
-                __syncthreads();
-            }
-            // backward: x_j = D_j^T (y_j - sum_{r>j} L(r, j)^T x_r)
-            for (int j = nb - 1; j >= 0; --j) {
-                const long long j0 = (long long)kB * j;
-                const int wj = min(kB, n - (int)j0);
-                T acc[kSlab / 4];
-                {
-                    const int r = tid & 63, q = tid >> 6;
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) {
-                        const int c = q + 4 * s;
-                        acc[s] = (r < wj && c < nc) ? Bm[(j0 + r) + (c0 + c) * ldb] : T(0);
-                    }
-                }
-                for (int i = j + 1; i < nb; ++i) {
-                    const long long i0 = (long long)kB * i;
-                    const int wi = min(kB, n - (int)i0);
-                    for (int e = tid; e < kB * kSlab; e += kThreads) {
-                        const int k = e & (kB - 1), c = e >> 6;
-                        V[c * kB + k] = (c < nc && k < wi) ? Bm[(i0 + k) + (c0 + c) * ldb] : T(0);
-                    }
-                    __syncthreads();
-                    T t[kSlab / 4];
-                    // M = L(i, j)^T: element (r, k) = L(i0 + k, j0 + r)
-                    const T* m = a.upper ? A + j0 + i0 * lda : A + i0 + j0 * lda;
-                    block_apply<T>(m, a.upper ? 1 : lda, a.upper ? lda : 1, wj, wi, V, nc, part, t);
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) acc[s] -= t[s];
-                }
-                {
-                    const int r = tid & 63, q = tid >> 6;
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) {
-                        const int c = q + 4 * s;
-                        if (c < kSlab) V[c * kB + r] = acc[s];
-                    }
-                }
-                __syncthreads();
-                T x[kSlab / 4];
-                // D_j^T: element (r, k) = D_j(k, r) = slot[j][k + 64 r]
-                block_apply<T>(slot + j0 * kB, kB, 1, kB, kB, V, nc, part, x);
-                {
-                    const int r = tid & 63, q = tid >> 6;
-#pragma unroll
-                    for (int s = 0; s < kSlab / 4; ++s) {
-                        const int c = q + 4 * s;
-                        if (r < wj && c < nc) Bm[(j0 + r) + (c0 + c) * ldb] = x[s];
-                    }
-                }
-                __syncthreads();
-            }
-        }
+        solve_member<T>(A, v, n, nb, a.B[item], ldb, a.nrhs, slot, part, V);
         if (tid == 0) {
             a.info[item] = 0;
             a.logdet[item] = 2.0 * logsum;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked path for the larger members: the members with more than tb blocks
+// (the leading count entries of order[], which is sorted by block count) are
+// factored one 64-column block column per step: blocked_panel_kernel factors
+// every such member's diagonal block (and keeps D_j); one gemmVbatched launch
+// forms all their panels L(r, j) = A(r, j) D_j^T in place, and one more
+// applies all their trailing updates (arrays from blocked_prep_kernel); then
+// blocked_solve_kernel solves them.  Each member's D_j is kept at
+// dinv + (b * max_blocks + j) * 64 * 64 (b = its position in order[]) and its
+// per-block log sums at logparts[item * kMaxBlocks + j].
+// ---------------------------------------------------------------------------
+template <class T>
+struct BlockedArgs {
+    int batch, upper, max_blocks, nrhs;
+    const int* n;
+    T* const* A;
+    const int* lda;
+    const int* ldb;
+    int* info;
+    const int* order;
+    const int* count;
+    T* dinv;
+    double* logparts;
+};
+
+// *count = the number of valid members with more than tb blocks (one 256-thread workgroup).
+__global__ __launch_bounds__(256) void count_large_kernel(int batch, int max_blocks, const int* n, int tb, int* count)
+{
+    __shared__ int part[256];
+    int c = 0;
+    for (int i = threadIdx.x; i < batch; i += 256) {
+        const int m = n[i];
+        if (m >= 0 && m <= max_blocks * kB && (m + kB - 1) / kB > tb) ++c;
+    }
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = part[0];
+}
+
+// Diagonal block j of member order[b] for every b < *count (one workgroup each).
+// Members are taken as valid (count_large_kernel counts only valid orders);
+// the leading dimensions are checked at j = 0 (info = -1, nothing touched,
+// as on the fused path).  info = 0 is set at j = 0; a member whose info is
+// nonzero is skipped.
+template <class T>
+__global__ __launch_bounds__(kThreads) void blocked_panel_kernel(BlockedArgs<T> a, int j)
+{
+    __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
+    __shared__ int s_flag;
+    const int b = blockIdx.x;
+    if (b >= *a.count) return;
+    const int item = a.order[b];
+    const int n = a.n[item];
+    const int nb = (n + kB - 1) / kB;
+    if (j >= nb) return;
+    const long long lda = a.lda[item];
+    if (j == 0) {
+        if (lda < (n > 1 ? n : 1) || (a.nrhs > 0 && a.ldb[item] < (n > 1 ? n : 1))) {
+            if (threadIdx.x == 0) a.info[item] = -1;
+            return;
+        }
+        if (threadIdx.x == 0) a.info[item] = 0;
+    } else if (a.info[item] != 0) {
+        return;
+    }
+    double lp = 0.0;
+    const int f = factor_block_column<T>(a.A[item], View{lda, a.upper}, n, nb, j,
+                                         a.dinv + ((long long)b * a.max_blocks + j) * kB * kB, S, X, rdiag, &s_flag, lp,
+                                         false);
+    if (threadIdx.x == 0) {
+        if (f != 0) a.info[item] = f;
+        else a.logparts[(long long)item * kMaxBlocks + j] = lp;
+    }
+}
+
+// Arrays of step j's two gemmVbatched launches: for b < *count with a
+// successful member of order > 64 (j + 1): m[b] = n - 64 (j + 1) (rows below
+// the diagonal block), k[b] = 64, ld[b] = lda, ld64[b] = 64, P[b] = the stored
+// address of L-view element (64 (j + 1), 64 j) (the panel), C[b] that of
+// (64 (j + 1), 64 (j + 1)) (the trailing matrix), D[b] = D_j; the panel launch
+// is m x 64 (lower: L = A D_j^T) or 64 x m (upper: the stored panel U = D_j U)
+// -- pm[b], pn[b].  m[b] = pm[b] = pn[b] = 0 (no work) otherwise.
+template <class T>
+struct StepArrays {
+    int *m, *k, *ld, *ld64, *pm, *pn;
+    T **P, **C, **D;
+};
+
+template <class T>
+__global__ void blocked_prep_kernel(BlockedArgs<T> a, int j, StepArrays<T> s)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.batch) return;
+    int mm = 0;
+    if (b < *a.count) {
+        const int item = a.order[b];
+        const int n = a.n[item];
+        const long long r0 = (long long)kB * (j + 1), j0 = (long long)kB * j;
+        if (n > r0 && a.info[item] == 0) {
+            const long long lda = a.lda[item];
+            T* A = a.A[item];
+            mm = n - (int)r0;
+            s.ld[b] = (int)lda;
+            s.P[b] = a.upper ? A + j0 + r0 * lda : A + r0 + j0 * lda;
+            s.C[b] = A + r0 + r0 * lda;
+            s.D[b] = a.dinv + ((long long)b * a.max_blocks + j) * kB * kB;
+        }
+    }
+    s.m[b] = mm;
+    s.pm[b] = a.upper ? (mm > 0 ? kB : 0) : mm;
+    s.pn[b] = a.upper ? mm : (mm > 0 ? kB : 0);
+    s.k[b] = kB;
+    s.ld64[b] = kB;
+}
+
+// Solves and log-determinants of the members factored on the blocked path.
+template <class T>
+__global__ __launch_bounds__(kThreads) void blocked_solve_kernel(BlockedArgs<T> a, T* const* B, double* logdet)
+{
+    __shared__ T part[kSlab * 4 * kB], V[kSlab * kB];
+    const int b = blockIdx.x;
+    if (b >= *a.count) return;
+    const int item = a.order[b];
+    const int n = a.n[item];
+    const int nb = (n + kB - 1) / kB;
+    if (a.info[item] != 0) {
+        if (threadIdx.x == 0) logdet[item] = __builtin_nan("");
+        return;
+    }
+    if (a.nrhs > 0)
+        solve_member<T>(a.A[item], View{a.lda[item], a.upper}, n, nb, B[item], a.ldb[item], a.nrhs,
+                        a.dinv + (long long)b * a.max_blocks * kB * kB, part, V);
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int jj = 0; jj < nb; ++jj) sum += a.logparts[(long long)item * kMaxBlocks + jj];
+        logdet[item] = 2.0 * sum;
     }
 }
 

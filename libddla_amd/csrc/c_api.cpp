@@ -453,6 +453,32 @@ int ddla_c_graph_launch(void* graph, void* handle)
     });
 }
 
+int ddla_c_vbatched_schedule(void* handle, int element_bytes, int* tuned, int* occupancy, int* ordered,
+                             int* split_blocks)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if (element_bytes != 4 && element_bytes != 8) return DDLA_C_STATUS_INVALID_VALUE;
+        const auto& t = h->batched_tuning[element_bytes == 8 ? 1 : 0];
+        if (tuned) *tuned = t.tuned ? 1 : 0;
+        if (occupancy) *occupancy = t.occupancy;
+        if (ordered) *ordered = t.ordered ? 1 : 0;
+        if (split_blocks) *split_blocks = t.split_blocks;
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_vbatched_set_schedule(void* handle, int element_bytes, int occupancy, int ordered, int split_blocks)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if ((element_bytes != 4 && element_bytes != 8) || (occupancy != 1 && occupancy != 2) || split_blocks < 1 ||
+            split_blocks > ddla::kern::kBatchedMaxBlocks)
+            return DDLA_C_STATUS_INVALID_VALUE;
+        h->batched_tuning[element_bytes == 8 ? 1 : 0] =
+            ddla::DdlaStream::BatchedTuning{true, occupancy, ordered != 0, split_blocks};
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
 int ddla_c_graph_destroy(void* graph)
 {
     return guarded([&]() -> int {

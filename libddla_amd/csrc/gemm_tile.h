@@ -79,9 +79,16 @@ struct TileOrder {
 // Workgroup b -> tile (ti, tj).  Tile t = (b % 8) * per_xcd + b / 8, so XCD x
 // owns tiles [x * per_xcd, (x + 1) * per_xcd).  Tiles are ordered in groups of
 // `group` tile columns; within a group by tile row, then column.
+__device__ inline bool tile_of(const TileOrder& o, int t, int& ti, int& tj);
+
 __device__ inline bool map_tile(const TileOrder& o, int b, int& ti, int& tj)
 {
-    const int t = (b & 7) * o.per_xcd + (b >> 3);
+    return tile_of(o, (b & 7) * o.per_xcd + (b >> 3), ti, tj);
+}
+
+// Tile t (< o.total) of the schedule of map_tile.
+__device__ inline bool tile_of(const TileOrder& o, int t, int& ti, int& tj)
+{
     if (t >= o.total) return false;
     const int G = o.group;
     if (!o.lower_enum) {
@@ -430,6 +437,7 @@ struct VBatch {
     const int* ldb;
     T* const* C;
     const int* ldc;
+    int tri;  // triangle mask of every C (kTri*D, diagonal offset 0)
 };
 
 template <class T, int BM, int BN, int BK, bool TA, bool TB>
@@ -449,9 +457,11 @@ __global__ __launch_bounds__(256) void gemm_vbatched_kernel(VBatch<T> v, int off
     p.ldb = v.ldb[b];
     p.C = v.C[b];
     p.ldc = v.ldc[b];
-    p.tri = kTriNoneD;
+    p.tri = v.tri;
     p.diag = 0;
-    p.vec_ab = 0;
+    // 16-byte loads when this member's operands allow them
+    constexpr int VW = Mfma<T>::kVec;
+    p.vec_ab = ((uintptr_t)p.A % 16 == 0) && ((uintptr_t)p.B % 16 == 0) && p.lda % VW == 0 && p.ldb % VW == 0;
     __shared__ T smem[lds_elems<BM, BN, BK>()];
     gemm_tile<T, BM, BN, BK, TA, TB>(p, blockIdx.x * BM, blockIdx.y * BN, smem);
 }

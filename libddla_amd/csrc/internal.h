@@ -51,6 +51,7 @@ public:
         bool tuned = false;
         int occupancy = 1;
         bool ordered = true;
+        int split_blocks = 16;  // members of more blocks take the blocked path (16: none)
     };
     BatchedTuning batched_tuning[2];  // [0] float, [1] double
 

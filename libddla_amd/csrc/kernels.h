@@ -24,11 +24,13 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
           const int* abort = nullptr);
 
 // Variable-size batched GEMM over device arrays of sizes, leading dimensions
-// and pointers; max_m / max_n size the grid.
+// and pointers; max_m / max_n size the grid; tri masks every C (diagonal
+// offset 0).  A member with m <= 0 or n <= 0 does nothing; k == 0 only scales
+// its C.
 template <class T>
 void gemm_vbatched(hipStream_t s, char ta, char tb, const int* m, const int* n, const int* k, T alpha,
                    const T* const* A, const int* lda, const T* const* B, const int* ldb, T beta, T* const* C,
-                   const int* ldc, int batch, int max_m, int max_n);
+                   const int* ldc, int batch, int max_m, int max_n, Tri tri = kTriNone);
 
 // Validation + max reduction of vbatched dimensions: summary[0..2] = max m, n,
 // k; summary[3] = 1 + first batch index with a negative size or a too-small
@@ -93,15 +95,21 @@ void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
 // n[i] x nrhs with ldb[i]) in one persistent launch of `grid` workgroups, after
 // a one-workgroup ordering kernel (largest first) and a counter reset; all of
 // it is stream-ordered and graph-capturable.  info[i] / logdet[i] as in
-// batched_tile.h.  work holds batched_work_bytes(sizeof(T), batch, grid,
-// max_blocks) bytes.  ordered: largest matrices first (else index order);
+// batched_tile.h.  ordered: largest matrices first (else index order);
 // occupancy 1 or 2: the kernel variant sized for that many workgroups per CU.
+// split_blocks (with ordered, 1 <= split_blocks < max_blocks): the members of
+// more than split_blocks 64-row blocks are factored on the blocked path
+// instead -- one launch per 64-column block column, the trailing updates of all
+// of them in one gemm_vbatched launch per step, then their solves -- and the
+// persistent launch takes the rest.  work holds batched_work_bytes(sizeof(T),
+// batch, grid, max_blocks, blocked) bytes, blocked = whether split_blocks may be
+// used.
 constexpr int kBatchedMaxBlocks = 16;
-size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks);
+size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks, bool blocked);
 template <class T>
 void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
                    T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
-                   int occupancy, void* work);
+                   int occupancy, int split_blocks, void* work);
 // Batched copy of matrix i (rows n[i], columns cols or n[i] when cols < 0)
 // from src[i] (leading dimension lds[i]) to dst[i] (leading dimension ldd[i]).
 template <class T>

@@ -387,10 +387,10 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
 template <class T>
 void gemm_vbatched(hipStream_t s, char ta, char tb, const int* m, const int* n, const int* k, T alpha,
                    const T* const* A, const int* lda, const T* const* B, const int* ldb, T beta, T* const* C,
-                   const int* ldc, int batch, int max_m, int max_n)
+                   const int* ldc, int batch, int max_m, int max_n, Tri tri)
 {
     if (batch <= 0 || max_m <= 0 || max_n <= 0) return;
-    VBatch<T> v{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc};
+    VBatch<T> v{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, (int)tri};
     launch_vbatched_cfg<T, 64, 64, 16>(s, !is_n(ta), !is_n(tb), v, batch, max_m, max_n);
 }
 
@@ -522,10 +522,28 @@ void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long l
     check_launch("trtri_diag_blocks");
 }
 
-size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks)
+namespace {
+size_t pad256(size_t b) { return (b + 255) / 256 * 256; }
+
+// [count | D_j of every member (batch x max_blocks x 64 x 64) | log sums
+// (batch x 16 doubles) | m | k | ld | ld64 | pm | pn | P | C | D] of the
+// blocked path
+size_t blocked_bytes(size_t elem, int batch, int max_blocks)
 {
-    const size_t inv = (size_t)grid * max_blocks * 64 * 64 * elem;
-    return (inv + 255) / 256 * 256 + ((size_t)batch * sizeof(int) + 255) / 256 * 256 + 256;
+    return 256 + pad256((size_t)batch * max_blocks * 64 * 64 * elem) +
+           pad256((size_t)batch * batched_detail::kMaxBlocks * sizeof(double)) + 6 * pad256((size_t)batch * sizeof(int)) +
+           3 * pad256((size_t)batch * sizeof(void*));
+}
+
+size_t fused_bytes(size_t elem, int batch, int grid, int max_blocks)
+{
+    return pad256((size_t)grid * max_blocks * 64 * 64 * elem) + pad256((size_t)batch * sizeof(int)) + 256;
+}
+} // namespace
+
+size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks, bool blocked)
+{
+    return fused_bytes(elem, batch, grid, max_blocks) + (blocked ? blocked_bytes(elem, batch, max_blocks) : 0);
 }
 
 namespace {
@@ -555,7 +573,7 @@ __global__ __launch_bounds__(256) void batched_copy_kernel(int batch, const int*
 template <class T>
 void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
                    T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
-                   int occupancy, void* work)
+                   int occupancy, int split_blocks, void* work)
 {
     using namespace batched_detail;
     if (batch <= 0) return;
@@ -572,9 +590,56 @@ void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n,
         fill_iota_kernel<<<grid_for(batch, 256), 256, 0, s>>>(order, batch);
     }
     check_launch("batched_pposv");
+    // blocked path for the members of more than split_blocks blocks (needs the sorted order)
+    const int* first = nullptr;
+    if (ordered && split_blocks >= 1 && split_blocks < max_blocks) {
+        char* bw = w + fused_bytes(sizeof(T), batch, grid, max_blocks);
+        int* count = reinterpret_cast<int*>(bw);
+        bw += 256;
+        T* dinv = reinterpret_cast<T*>(bw);
+        bw += pad256((size_t)batch * max_blocks * 64 * 64 * sizeof(T));
+        double* logparts = reinterpret_cast<double*>(bw);
+        bw += pad256((size_t)batch * kMaxBlocks * sizeof(double));
+        StepArrays<T> sa;
+        int** ints[] = {&sa.m, &sa.k, &sa.ld, &sa.ld64, &sa.pm, &sa.pn};
+        for (int** q : ints) {
+            *q = reinterpret_cast<int*>(bw);
+            bw += pad256((size_t)batch * sizeof(int));
+        }
+        T*** ptrs[] = {&sa.P, &sa.C, &sa.D};
+        for (T*** q : ptrs) {
+            *q = reinterpret_cast<T**>(bw);
+            bw += pad256((size_t)batch * sizeof(void*));
+        }
+        count_large_kernel<<<1, 256, 0, s>>>(batch, max_blocks, n, split_blocks, count);
+        check_launch("batched_pposv");
+        BlockedArgs<T> ba{batch, upper ? 1 : 0, max_blocks, nrhs, n, A, lda, ldb, info, order, count, dinv, logparts};
+        for (int j = 0; j < max_blocks; ++j) {
+            blocked_panel_kernel<T><<<batch, kThreads, 0, s>>>(ba, j);
+            check_launch("batched_pposv");
+            if (j + 1 < max_blocks) {
+                blocked_prep_kernel<T><<<(batch + 255) / 256, 256, 0, s>>>(ba, j, sa);
+                check_launch("batched_pposv");
+                const int mx = (max_blocks - 1 - j) * 64;
+                // panel: L(r, j) = A(r, j) D_j^T in place (lower), U = D_j U in place (upper)
+                if (upper)
+                    gemm_vbatched<T>(s, 'N', 'N', sa.pm, sa.pn, sa.k, T(1), sa.D, sa.ld64, sa.P, sa.ld, T(0), sa.P,
+                                     sa.ld, batch, 64, mx);
+                else
+                    gemm_vbatched<T>(s, 'N', 'T', sa.pm, sa.pn, sa.k, T(1), sa.P, sa.ld, sa.D, sa.ld64, T(0), sa.P,
+                                     sa.ld, batch, mx, 64);
+                // trailing update
+                gemm_vbatched<T>(s, upper ? 'T' : 'N', upper ? 'N' : 'T', sa.m, sa.m, sa.k, T(-1), sa.P, sa.ld, sa.P,
+                                 sa.ld, T(1), sa.C, sa.ld, batch, mx, mx, upper ? kTriUpper : kTriLower);
+            }
+        }
+        blocked_solve_kernel<T><<<batch, kThreads, 0, s>>>(ba, B, logdet);
+        check_launch("batched_pposv");
+        first = count;
+    }
     solve_detail::zero_ints_kernel<<<1, 64, 0, s>>>(queue, 1);
     check_launch("batched_pposv");
-    BatchArgs<T> a{batch, nrhs, upper ? 1 : 0, max_blocks, n, A, lda, B, ldb, info, logdet, order, queue, xinv};
+    BatchArgs<T> a{batch, nrhs, upper ? 1 : 0, max_blocks, n, A, lda, B, ldb, info, logdet, order, queue, xinv, first};
     if (occupancy == 2) batched_pposv_kernel<T, 2><<<grid, kThreads, 0, s>>>(a);
     else batched_pposv_kernel<T, 1><<<grid, kThreads, 0, s>>>(a);
     check_launch("batched_pposv");
@@ -803,13 +868,13 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
                           T, T*, long long, Tri, long long, const int*);                                      \
     template void gemm_vbatched<T>(hipStream_t, char, char, const int*, const int*, const int*, T,            \
                                    const T* const*, const int*, const T* const*, const int*, T, T* const*,    \
-                                   const int*, int, int, int);                                                \
+                                   const int*, int, int, int, Tri);                                           \
     template size_t panel_work_bytes<T>(int);                                                                 \
     template void panel_reset<T>(hipStream_t, int, void*);                                                    \
     template int* panel_error_word<T>(int, void*);                                                            \
     template void panel_factor<T>(hipStream_t, bool, int, T*, long long, int, int, int*, void*);              \
     template void batched_pposv<T>(hipStream_t, bool, int, int, const int*, T* const*, const int*, T* const*,  \
-                                   const int*, int*, double*, int, int, bool, int, void*);                     \
+                                   const int*, int*, double*, int, int, bool, int, int, void*);                \
     template void batched_copy<T>(hipStream_t, int, const int*, int, T* const*, const int*, T* const*,         \
                                   const int*, int);                                                            \
     template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*);                      \

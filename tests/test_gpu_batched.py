@@ -1,5 +1,8 @@
 """GPU tests of the batched fused potrf + potrs + logdet (ddla_{s,d}_pposv_vbatched).
 
+Schedules: the persistent fused kernel for every member, or the blocked path
+(64-column steps with gemmVbatched trailing updates) for the members of more
+than split_blocks blocks; each test pins the schedule it runs.
 Every member is checked against the oracle on its own: the factor against
 oracle potrf, the solution against oracle potrs, the log-determinant against
 2 * the oracle's diag_logdet of its factor; the unused triangle (NaN-poisoned)
@@ -63,11 +66,17 @@ def check_member(vb, i, a, b, raw, dtype, uplo):
     assert abs(ld - 2 * mag) <= ltol * max(1.0, n), (i, n, ld, 2 * mag)
 
 
+# (workgroups per CU, largest first, split_blocks): fused only; blocked above 1 and 3 blocks
+SCHEDULES = [(1, True, 16), (2, False, 16), (1, True, 1), (2, True, 3)]
+
+
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("uplo", ["L", "U"])
-@pytest.mark.parametrize("nrhs", [1, 3, 17])
-def test_batched_members_match_oracle(handle, dtype, uplo, nrhs):
-    print(f"case batched {np.dtype(dtype).name} uplo={uplo} nrhs={nrhs}", flush=True)
+@pytest.mark.parametrize("nrhs", [1, 17])
+def test_batched_members_match_oracle(handle, dtype, uplo, nrhs, schedule):
+    print(f"case batched {np.dtype(dtype).name} uplo={uplo} nrhs={nrhs} schedule={schedule}", flush=True)
+    dd.set_vbatched_schedule(handle, dtype, *schedule)
     vb, mats, rhs, raws = make_batch(handle, SIZES, dtype, uplo, nrhs)
     dd.pposv_vbatched(vb, uplo)
     info = vb.get_info()
@@ -76,10 +85,14 @@ def test_batched_members_match_oracle(handle, dtype, uplo, nrhs):
         check_member(vb, i, mats[i], rhs[i], raws[i], dtype, uplo)
 
 
+@pytest.mark.parametrize("split", [16, 1])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_batched_failures_are_per_member(handle, dtype):
+def test_batched_failures_are_per_member(handle, dtype, split):
     """A non-PD member reports its pivot (B untouched, logdet NaN); an order
-    above max_n reports -1 and is not touched; the others are solved."""
+    above max_n reports -1 and is not touched; the others are solved.  With
+    split 1 the members above 64 take the blocked path (the failing pivot is in
+    the third block column)."""
+    dd.set_vbatched_schedule(handle, dtype, 1, True, split)
     vb, mats, rhs, raws = make_batch(handle, [130, 70, 5, 40], dtype, "L", 2)
     # member 0: final pivot negative
     raw0 = raws[0].copy()
@@ -105,23 +118,34 @@ def test_batched_failures_are_per_member(handle, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batched_run_tune_and_graph_agree_bitwise(handle, dtype):
-    """The workspace run, every tuned schedule and each replay of a captured
-    graph give the driver's results bit for bit (every member is computed by
-    one workgroup in a fixed order, whatever the schedule); tuning leaves the
-    inputs unchanged.  The graph is replayed three times with eager work and
-    re-uploaded inputs in between."""
+    """Tuning leaves the inputs unchanged and records its schedule; for that
+    schedule the driver, the workspace run and each replay of a captured graph
+    give the same results bit for bit (no member's arithmetic depends on timing).
+    The graph is replayed three times with eager work and re-uploaded inputs in
+    between."""
     sizes = [300, 17, 512, 64, 129, 1, 250, 400]
     vb, mats, rhs, raws = make_batch(handle, sizes, dtype, "L", 2)
-    dd.pposv_vbatched(vb, "L")
-    ref_a = [vb.get_raw(i).copy() for i in range(len(sizes))]
-    ref_x = [vb.get_rhs(i).copy() for i in range(len(sizes))]
-    ref_l = vb.get_logdet().copy()
 
     def reload():
         for i, n in enumerate(sizes):
             vb.set_raw(i, raws[i])
             vb.set_rhs(i, rhs[i])
         handle.synchronize()
+
+    occ, ordered = dd.pposv_vbatched_tune(vb, "L")
+    assert occ in (1, 2)
+    for i in range(len(sizes)):  # inputs unchanged by tuning
+        assert vb.get_raw(i).tobytes() == raws[i].tobytes()
+    sched = dd.vbatched_schedule(handle, dtype)
+    assert sched["tuned"] and sched["workgroups_per_cu"] == occ and sched["largest_first"] == ordered
+    assert 1 <= sched["split_blocks"] <= 16
+    dd.pposv_vbatched(vb, "L")
+    assert (vb.get_info() == 0).all()
+    for i in range(len(sizes)):
+        check_member(vb, i, mats[i], rhs[i], raws[i], dtype, "L")
+    ref_a = [vb.get_raw(i).copy() for i in range(len(sizes))]
+    ref_x = [vb.get_rhs(i).copy() for i in range(len(sizes))]
+    ref_l = vb.get_logdet().copy()
 
     def same():
         for i in range(len(sizes)):
@@ -130,10 +154,6 @@ def test_batched_run_tune_and_graph_agree_bitwise(handle, dtype):
         assert vb.get_logdet().tobytes() == ref_l.tobytes()
 
     reload()
-    occ, ordered = dd.pposv_vbatched_tune(vb, "L")
-    assert occ in (1, 2)
-    for i in range(len(sizes)):  # inputs unchanged by tuning
-        assert vb.get_raw(i).tobytes() == raws[i].tobytes()
     nbytes = dd.pposv_vbatched_workspace_size(vb)
     work = dd.batched._Device(handle, nbytes)
     dd.pposv_vbatched_run(vb, work.ptr.value, nbytes, "L")
@@ -152,6 +172,9 @@ def test_batched_run_tune_and_graph_agree_bitwise(handle, dtype):
 
 
 def test_batched_invalid_arguments(handle):
+    for bad in ((3, True, 4), (1, True, 0), (1, True, 17)):
+        with pytest.raises(dd.DdlaError):
+            dd.set_vbatched_schedule(handle, np.float64, *bad)
     vb = dd.VBatch(handle, [4, 4], nrhs=1)
     with pytest.raises(dd.DdlaExceptionError):
         dd.pposv_vbatched(vb, "X")
@@ -167,6 +190,7 @@ def test_config4_full_batch(handle, dtype):
     one right-hand side each: every member succeeds; a sample of 64 members has
     backward error ||b - A x||_inf / (||A||_inf ||x||_inf) <= 64 n eps and a
     log-determinant within 1e-9 n (FP64) / 1e-3 n (FP32) of numpy's slogdet."""
+    dd.set_vbatched_schedule(handle, dtype, 1, True, 4)  # blocked path above order 256
     rng = np.random.RandomState(3)
     sizes = rng.randint(32, 513, size=4096)
     vb = dd.VBatch(handle, sizes, nrhs=1, dtype=dtype)
