@@ -7,7 +7,10 @@
 //   potrf_backward: Abar = sym( L^-T Phi(L^T Lbar) L^-1 ),  sym(S) = (S + S^T)/2,
 //                   Phi(M) = tril(M) with the diagonal halved
 //                   (Murray 2016, "Differentiation of the Cholesky decomposition",
-//                   the symmetric-input convention of the blocked algorithm);
+//                   the symmetric-input convention of the blocked algorithm),
+//                   evaluated as L^-T Psi L^-1 with the symmetric
+//                   Psi = sym(Phi(M)): only tril(M) and the lower triangle
+//                   of the (symmetric) result are formed;
 //   potrs_backward: X = A^-1 B:  Bbar = A^-1 Xbar,
 //                   Lbar -= tril(Bbar (L^T X)^T + X (L^T Bbar)^T);
 //   logdet_backward: logdet A = 2 sum log L(i, i):  Lbar(i, i) += 2 g / L(i, i).
@@ -42,6 +45,65 @@ void finish(const DdlaHandle_t& h, int* error, const char* who)
     if (e != 0) throw std::runtime_error(std::string(who) + ": a triangular-solve sweep exceeded its wait limit");
 }
 
+// M = tril(L^T Lbar) for Lz, Bz with zero strict upper triangles: the terms
+// of k in [k0, k1) reach only rows and columns below k1, so the product runs
+// in four k blocks of growing order (about n^3 / 2 flops instead of 2 n^3),
+// the last block -- every row, beta 0 -- first.
+void k_block(int n, int q, int& k0, int& k1)
+{
+    k0 = q * (n / 4) / kIB * kIB;
+    k1 = q == 3 ? n : (q + 1) * (n / 4) / kIB * kIB;
+}
+
+double lower_product_flops(int n)
+{
+    double flops = 0;
+    for (int q = 0, k0, k1; q < 4; ++q) {
+        k_block(n, q, k0, k1);
+        if (k1 > k0) flops += (double)k1 * (k1 + 1) * (k1 - k0);
+    }
+    return flops;
+}
+
+template <class T>
+void lower_product(hipStream_t s, int n, const T* Lz, const T* Bz, T* M)
+{
+    for (int q = 3, k0, k1; q >= 0; --q) {
+        k_block(n, q, k0, k1);
+        if (k1 <= k0) continue;
+        kern::gemm<T>(s, 'T', 'N', k1, k1, k1 - k0, T(1), Lz + k0, n, Bz + k0, n, q == 3 ? T(0) : T(1), M, n,
+                      kern::kTriLower, 0);
+    }
+}
+
+// tril(X) of L^T X = C (order n, in place on C): rows j.. of column j of X
+// depend only on rows j.. of C(:, j) and the trailing factor from j, so with
+// L = [L11 0; L21 L22] and the columns split alike,
+//   X21 = L22^-T C21,  tril(X22) from (L22, C22),
+//   C11 -= L21^T X21 (lower triangle),  tril(X11) from (L11, C11)
+// -- n^3 / 3 flops instead of n^3.  Leaves of order <= kLowerLeaf take the
+// full solve.
+constexpr int kLowerLeaf = 512;
+
+template <class T>
+void lower_solve(hipStream_t s, int n, const T* L, long long ldl, const T* dinv, T* C, long long ldc, void* sync,
+                 int* error)
+{
+    if (n <= kLowerLeaf) {
+        kern::trsm<T>(s, false, true, n, n, L, ldl, dinv, C, ldc, sync, error);
+        return;
+    }
+    const int n1 = (n / 2 + kIB - 1) / kIB * kIB, n2 = n - n1;
+    const T* L21 = L + n1;
+    const T* L22 = L + n1 + (long long)n1 * ldl;
+    const T* dinv22 = dinv + (long long)(n1 / kIB) * kIB * kIB;
+    T* C21 = C + n1;
+    kern::trsm<T>(s, false, true, n2, n1, L22, ldl, dinv22, C21, ldc, sync, error);
+    lower_solve<T>(s, n2, L22, ldl, dinv22, C21 + (long long)n1 * ldc, ldc, sync, error);
+    kern::gemm<T>(s, 'T', 'N', n1, n1, n2, T(-1), L21, ldl, C21, ldc, T(1), C, ldc, kern::kTriLower, 0);
+    lower_solve<T>(s, n1, L, ldl, dinv, C, ldc, sync, error);
+}
+
 } // namespace
 
 template <class T>
@@ -67,21 +129,21 @@ void potrf_backward(const DdlaHandle_t& h, char uplo, int n, const T* L, int ldl
     kern::tril_copy<T>(s, n, L, ldl, Lz, n);
     kern::tril_copy<T>(s, n, Lbar, ldlb, Bz, n);
     {
-        LaunchTimer timer(h, kClassOther, 2.0 * n * (double)nn);
-        kern::gemm<T>(s, 'T', 'N', n, n, n, T(1), Lz, n, Bz, n, T(0), M, n);  // M = L^T Lbar
+        LaunchTimer timer(h, kClassOther, lower_product_flops(n));
+        lower_product<T>(s, n, Lz, Bz, M);  // M = tril(L^T Lbar)
     }
-    kern::phi<T>(s, n, M, n);
+    kern::mirror_lower<T>(s, n, M, n, T(0.5), Bz, n);  // Psi = sym(Phi(M)) = sym(tril(M)) / 2
     {
         LaunchTimer timer(h, kClassTrtri, (double)n * kIB * kIB / 3.0);
         kern::trtri_diag_blocks<T>(s, false, n, L, ldl, dinv);
     }
     {
-        LaunchTimer timer(h, kClassSolve, 2.0 * (double)nn * n, 0);
-        kern::trsm_sweep<T>(s, false, true, n, n, L, ldl, dinv, M, n, sync, error);   // W = L^-T Phi
-        kern::transpose<T>(s, n, n, M, n, Bz, n);                                      // W^T
-        kern::trsm_sweep<T>(s, false, true, n, n, L, ldl, dinv, Bz, n, sync, error);  // S^T = L^-T W^T
+        LaunchTimer timer(h, kClassSolve, 4.0 / 3.0 * (double)nn * n, 0);
+        kern::trsm<T>(s, false, true, n, n, L, ldl, dinv, Bz, n, sync, error);  // W = L^-T Psi
+        kern::transpose<T>(s, n, n, Bz, n, M, n);                              // W^T = Psi L^-1
+        lower_solve<T>(s, n, L, ldl, dinv, M, n, sync, error);                 // tril(L^-T Psi L^-1)
     }
-    kern::sym_half<T>(s, n, Bz, n, Abar, ldab);
+    kern::mirror_lower<T>(s, n, M, n, T(1), Abar, ldab);
     finish(h, error, "potrf_backward");
 }
 

@@ -123,17 +123,27 @@ template <class T>
 void trsm_sweep(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L, long long ldl, const T* dinv,
                 T* B, long long ldb, void* work, int* error);
 
+// The same solve for many right-hand sides: while trsm_splits(n, nrhs) the
+// order is split in two (the first part a multiple of 64 rows) and the
+// off-diagonal block is applied by one GEMM, so that the GEMM carries most of
+// the n^2 nrhs flops and the sweep kernel only the leaves of order <=
+// kTrsmLeaf; otherwise one sweep.  potrs_sweeps takes the same route.
+constexpr int kTrsmLeaf = 512;
+constexpr int kTrsmMinColumns = 256;
+bool trsm_splits(int n, int nrhs);
+template <class T>
+void trsm(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L, long long ldl, const T* dinv, T* B,
+          long long ldb, void* work, int* error);
+
 // Elementwise helpers of the adjoints (order n, column-major):
-//   tril_copy:   out = tril(in) (strict upper triangle zero)
-//   phi:         M = tril(M) with the diagonal halved, strict upper zero
-//   sym_half:    out = (S + S^T) / 2
+//   tril_copy:    out = tril(in) (strict upper triangle zero)
+//   mirror_lower: out(i, j) = scale * in(max(i, j), min(i, j)), the symmetric
+//                 matrix of in's lower triangle (out must not alias in)
 //   diag_adjoint: Lbar(i, i) += 2 g / L(i, i)
 template <class T>
 void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long long ldo);
 template <class T>
-void phi(hipStream_t s, int n, T* M, long long ld);
-template <class T>
-void sym_half(hipStream_t s, int n, const T* S, long long lds, T* out, long long ldo);
+void mirror_lower(hipStream_t s, int n, const T* in, long long ldi, T scale, T* out, long long ldo);
 template <class T>
 void diag_adjoint(hipStream_t s, int n, const T* L, long long ldl, double g, T* Lbar, long long ldlb);
 

@@ -671,6 +671,11 @@ void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
                   long long ldb, void* work, int* error)
 {
     if (n <= 0 || nrhs <= 0) return;
+    if (trsm_splits(n, nrhs)) {
+        trsm<T>(s, upper, false, n, nrhs, L, ldl, dinv, B, ldb, work, error);
+        trsm<T>(s, upper, true, n, nrhs, L, ldl, dinv, B, ldb, work, error);
+        return;
+    }
     using namespace solve_detail;
     const int nblk = (n + kSB - 1) / kSB, slabs = (nrhs + kSB - 1) / kSB;
     if (slabs > 65535) throw std::invalid_argument("potrs: too many right-hand sides for one launch");
@@ -703,23 +708,72 @@ void trsm_sweep(hipStream_t s, bool upper, bool backward, int n, int nrhs, const
     check_launch("trsm_sweep");
 }
 
-namespace {
-// mode 0: out = tril(in); mode 1: out = tril(in) with the diagonal halved;
-// mode 2: out = (in + in^T) / 2.  One thread per element of out.
+bool trsm_splits(int n, int nrhs) { return nrhs >= kTrsmMinColumns && n > kTrsmLeaf; }
+
 template <class T>
-__global__ void square_map_kernel(int n, const T* in, long long ldi, T* out, long long ldo, int mode)
+void trsm(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L, long long ldl, const T* dinv, T* B,
+          long long ldb, void* work, int* error)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    if (!trsm_splits(n, nrhs)) {
+        trsm_sweep<T>(s, upper, backward, n, nrhs, L, ldl, dinv, B, ldb, work, error);
+        return;
+    }
+    // L = [L11 0; L21 L22] with n1 a multiple of the 64-row diagonal blocks, so
+    // that dinv of L22 starts at block n1 / 64.  L21 (L view, n2 x n1) is
+    // stored at L + n1 as an n2 x n1 matrix (lower) or at L + n1 * ldl as its
+    // n1 x n2 transpose (upper).
+    const int n1 = (n / 2 + 63) / 64 * 64, n2 = n - n1;
+    const T* L21 = upper ? L + (long long)n1 * ldl : L + n1;
+    const T* L22 = L + n1 + (long long)n1 * ldl;
+    const T* dinv22 = dinv + (long long)(n1 / 64) * 4096;
+    T* B2 = B + n1;
+    if (!backward) {
+        // L11 Y1 = B1;  B2 -= L21 Y1;  L22 Y2 = B2
+        trsm<T>(s, upper, false, n1, nrhs, L, ldl, dinv, B, ldb, work, error);
+        gemm<T>(s, upper ? 'T' : 'N', 'N', n2, nrhs, n1, T(-1), L21, ldl, B, ldb, T(1), B2, ldb);
+        trsm<T>(s, upper, false, n2, nrhs, L22, ldl, dinv22, B2, ldb, work, error);
+    } else {
+        // L22^T X2 = B2;  B1 -= L21^T X2;  L11^T X1 = B1
+        trsm<T>(s, upper, true, n2, nrhs, L22, ldl, dinv22, B2, ldb, work, error);
+        gemm<T>(s, upper ? 'N' : 'T', 'N', n1, nrhs, n2, T(-1), L21, ldl, B2, ldb, T(1), B, ldb);
+        trsm<T>(s, upper, true, n1, nrhs, L, ldl, dinv, B, ldb, work, error);
+    }
+}
+
+namespace {
+// out = tril(in), strict upper triangle zero.  One thread per element of out.
+template <class T>
+__global__ void tril_copy_kernel(int n, const T* in, long long ldi, T* out, long long ldo)
 {
     const long long total = (long long)n * n;
     for (long long t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
         const int i = (int)(t % n), j = (int)(t / n);
-        T v;
-        if (mode == 2) {
-            v = T(0.5) * (in[i + j * ldi] + in[j + i * ldi]);
-        } else {
-            v = i >= j ? in[i + j * ldi] : T(0);
-            if (mode == 1 && i == j) v *= T(0.5);
+        out[i + j * ldo] = i >= j ? in[i + j * ldi] : T(0);
+    }
+}
+
+// out(i, j) = scale * in(max(i, j), min(i, j)).  Workgroup (bi, bj) writes
+// the 32 x 32 tile (bi, bj) of out from the lower-triangle tile (max, min) of
+// in, staged through LDS so that the mirrored read and the write are both
+// coalesced.
+template <class T>
+__global__ __launch_bounds__(256) void mirror_lower_kernel(int n, const T* in, long long ldi, T scale, T* out,
+                                                           long long ldo)
+{
+    __shared__ T tile[32][33];
+    const int bi = blockIdx.x, bj = blockIdx.y;
+    const long long si = (long long)max(bi, bj) * 32, sj = (long long)min(bi, bj) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8)
+        if (si + tx < n && sj + r < n) tile[r][tx] = in[(si + tx) + (sj + r) * ldi];  // tile[col][row]
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const long long i = (long long)bi * 32 + tx, j = (long long)bj * 32 + r;
+        if (i < n && j < n) {
+            const bool mirrored = bi < bj || (bi == bj && tx < r);
+            out[i + j * ldo] = scale * (mirrored ? tile[tx][r] : tile[r][tx]);
         }
-        out[i + j * ldo] = v;
     }
 }
 
@@ -735,25 +789,19 @@ template <class T>
 void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long long ldo)
 {
     if (n <= 0) return;
-    square_map_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, in, ldi, out, ldo, 0);
+    tril_copy_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, in, ldi, out, ldo);
     check_launch("tril_copy");
 }
 
 template <class T>
-void phi(hipStream_t s, int n, T* M, long long ld)
+void mirror_lower(hipStream_t s, int n, const T* in, long long ldi, T scale, T* out, long long ldo)
 {
     if (n <= 0) return;
-    square_map_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, M, ld, M, ld, 1);
-    check_launch("phi");
-}
-
-template <class T>
-void sym_half(hipStream_t s, int n, const T* S, long long lds, T* out, long long ldo)
-{
-    if (n <= 0) return;
-    if (S == out) throw std::invalid_argument("sym_half: output must not alias the input");
-    square_map_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, S, lds, out, ldo, 2);
-    check_launch("sym_half");
+    if (in == out) throw std::invalid_argument("mirror_lower: output must not alias the input");
+    const unsigned tiles = (unsigned)((n + 31) / 32);
+    if (tiles > 65535) throw std::invalid_argument("mirror_lower: order too large");
+    mirror_lower_kernel<T><<<dim3(tiles, tiles), 256, 0, s>>>(n, in, ldi, scale, out, ldo);
+    check_launch("mirror_lower");
 }
 
 template <class T>
@@ -884,9 +932,10 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void add_diagonal<T>(hipStream_t, int, T*, long long, T);                                        \
     template void trsm_sweep<T>(hipStream_t, bool, bool, int, int, const T*, long long, const T*, T*, long long, \
                                 void*, int*);                                                                 \
+    template void trsm<T>(hipStream_t, bool, bool, int, int, const T*, long long, const T*, T*, long long,       \
+                          void*, int*);                                                                       \
     template void tril_copy<T>(hipStream_t, int, const T*, long long, T*, long long);                         \
-    template void phi<T>(hipStream_t, int, T*, long long);                                                     \
-    template void sym_half<T>(hipStream_t, int, const T*, long long, T*, long long);                          \
+    template void mirror_lower<T>(hipStream_t, int, const T*, long long, T, T*, long long);                   \
     template void diag_adjoint<T>(hipStream_t, int, const T*, long long, double, T*, long long);              \
     template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);
 DDLA_KERN_INSTANTIATE(double)

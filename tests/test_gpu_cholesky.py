@@ -174,6 +174,27 @@ def test_ppotrs_flags_closed_form_and_readonly_factor(handle, dtype, side, uplo,
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_ppotrs_many_rhs_recursive_split(handle, dtype, side, uplo):
+    """Many right-hand sides (n = 1100, 300 columns): the solve splits the
+    order recursively (GEMM off-diagonal updates, ragged 64-row leaves) in
+    both storage orientations; closed-form solution as above."""
+    n, nrhs = 1100, 300
+    print(f"case many-rhs n={n} nrhs={nrhs} side={side} uplo={uplo} {np.dtype(dtype).name}", flush=True)
+    a = hpd_real(n)
+    x_true = x_real(n, nrhs) if side == "L" else x_real(n, nrhs).T
+    b = a @ x_true if side == "L" else x_true @ a
+    A = upload(handle, a, dtype=dtype)
+    _, info = dd.ppotrf(A, uplo)
+    assert info == 0
+    B = upload(handle, b, dtype=dtype)
+    dd.ppotrs(A, B, uplo, side=side)
+    x = B.to_global_matrix()
+    assert np.abs(x - x_true).max() < (5e-9 if dtype == np.float64 else 2e-3)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("uplo", ["L", "U"])
 def test_pposv_final_negative_pivot(handle, dtype, uplo):
     """python/tests/test_cholesky.py:79-92: info = n and B unchanged."""
