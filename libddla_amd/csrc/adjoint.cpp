@@ -157,13 +157,24 @@ void potrs_backward(const DdlaHandle_t& h, char uplo, int n, int nrhs, const T* 
     if (!X || !Xbar || !Bbar || !Lbar || ldx < n || ldxb < n || ldbb < n || ldlb < n)
         throw std::invalid_argument("potrs_backward: invalid X, Xbar, Bbar or Lbar");
     const std::size_t nn = (std::size_t)n * n, nr = (std::size_t)n * nrhs;
-    const std::size_t bytes = detail::padded_bytes<T>(nn) + 2 * detail::padded_bytes<T>(nr) +
-                              detail::padded_bytes<T>(inv_elems(n)) + kern::solve_work_bytes(nrhs) +
-                              detail::padded_bytes<int>(1);
+    // Few columns: P = [X | Bbar | X], Q = L^T [X | Bbar] = [U | V] in one pass
+    // over the lower triangle, and Lbar -= [Bbar | X] [U | V]^T in one update.
+    // Otherwise U and V are GEMMs with a zero-padded copy of L.
+    const bool few = 2 * nrhs <= kern::kTmvColumns;
+    const std::size_t bytes =
+        (few ? detail::padded_bytes<T>(3 * nr) + detail::padded_bytes<T>(2 * nr)
+             : detail::padded_bytes<T>(nn) + 2 * detail::padded_bytes<T>(nr)) +
+        detail::padded_bytes<T>(inv_elems(n)) + kern::solve_work_bytes(nrhs) + detail::padded_bytes<int>(1);
     char* w = static_cast<char*>(h->ensure_scratch(bytes));
-    T* Lz = detail::take<T>(w, nn);
-    T* U = detail::take<T>(w, nr);
-    T* V = detail::take<T>(w, nr);
+    T *P = nullptr, *Q = nullptr, *Lz = nullptr, *U = nullptr, *V = nullptr;
+    if (few) {
+        P = detail::take<T>(w, 3 * nr);
+        Q = detail::take<T>(w, 2 * nr);
+    } else {
+        Lz = detail::take<T>(w, nn);
+        U = detail::take<T>(w, nr);
+        V = detail::take<T>(w, nr);
+    }
     T* dinv = detail::take<T>(w, inv_elems(n));
     void* sync = w;
     w += kern::solve_work_bytes(nrhs);
@@ -176,12 +187,20 @@ void potrs_backward(const DdlaHandle_t& h, char uplo, int n, int nrhs, const T* 
         LaunchTimer timer(h, kClassSolve, 2.0 * (double)nn * nrhs, 0);
         kern::potrs_sweeps<T>(s, false, n, nrhs, L, ldl, dinv, Bbar, ldbb, sync, error);  // Bbar = A^-1 Xbar
     }
-    kern::tril_copy<T>(s, n, L, ldl, Lz, n);
-    kern::gemm<T>(s, 'T', 'N', n, nrhs, n, T(1), Lz, n, X, ldx, T(0), U, n);     // U = L^T X
-    kern::gemm<T>(s, 'T', 'N', n, nrhs, n, T(1), Lz, n, Bbar, ldbb, T(0), V, n);  // V = L^T Bbar
-    // Lbar (lower) -= Bbar U^T + X V^T
-    kern::gemm<T>(s, 'N', 'T', n, n, nrhs, T(-1), Bbar, ldbb, U, n, T(1), Lbar, ldlb, kern::kTriLower, 0);
-    kern::gemm<T>(s, 'N', 'T', n, n, nrhs, T(-1), X, ldx, V, n, T(1), Lbar, ldlb, kern::kTriLower, 0);
+    if (few) {
+        kern::convert<T, T>(s, n, nrhs, X, ldx, P, n);
+        kern::convert<T, T>(s, n, nrhs, Bbar, ldbb, P + nr, n);
+        kern::convert<T, T>(s, n, nrhs, X, ldx, P + 2 * nr, n);
+        kern::lower_tmv<T>(s, n, 2 * nrhs, L, ldl, P, n, Q, n);  // [U | V] = L^T [X | Bbar]
+        kern::gemm<T>(s, 'N', 'T', n, n, 2 * nrhs, T(-1), P + nr, n, Q, n, T(1), Lbar, ldlb, kern::kTriLower, 0);
+    } else {
+        kern::tril_copy<T>(s, n, L, ldl, Lz, n);
+        kern::gemm<T>(s, 'T', 'N', n, nrhs, n, T(1), Lz, n, X, ldx, T(0), U, n);     // U = L^T X
+        kern::gemm<T>(s, 'T', 'N', n, nrhs, n, T(1), Lz, n, Bbar, ldbb, T(0), V, n);  // V = L^T Bbar
+        // Lbar (lower) -= Bbar U^T + X V^T
+        kern::gemm<T>(s, 'N', 'T', n, n, nrhs, T(-1), Bbar, ldbb, U, n, T(1), Lbar, ldlb, kern::kTriLower, 0);
+        kern::gemm<T>(s, 'N', 'T', n, n, nrhs, T(-1), X, ldx, V, n, T(1), Lbar, ldlb, kern::kTriLower, 0);
+    }
     finish(h, error, "potrs_backward");
 }
 

@@ -144,6 +144,13 @@ template <class T>
 void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long long ldo);
 template <class T>
 void mirror_lower(hipStream_t s, int n, const T* in, long long ldi, T scale, T* out, long long ldo);
+// Y = tril(L)^T W for W with c <= kTmvColumns columns (L lower storage, its
+// strict upper triangle never read): the few-column products of the solve
+// adjoint, one wavefront per row of Y.
+constexpr int kTmvColumns = 16;
+template <class T>
+void lower_tmv(hipStream_t s, int n, int c, const T* L, long long ldl, const T* W, long long ldw, T* Y,
+               long long ldy);
 template <class T>
 void diag_adjoint(hipStream_t s, int n, const T* L, long long ldl, double g, T* Lbar, long long ldlb);
 

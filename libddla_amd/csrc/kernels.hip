@@ -777,6 +777,38 @@ __global__ __launch_bounds__(256) void mirror_lower_kernel(int n, const T* in, l
     }
 }
 
+// Y(i, j) = sum_{k >= i} L(k, i) W(k, j), i < n, j < c <= kTmvColumns: the
+// transposed product with the lower triangle of L (lower storage; the strict
+// upper triangle is never read).  One wavefront per row i reads column i of L
+// from the diagonal down, coalesced; fixed lane partition and butterfly, so
+// the summation order is deterministic.
+template <class T>
+__global__ __launch_bounds__(256) void lower_tmv_kernel(int n, int c, const T* L, long long ldl, const T* W,
+                                                        long long ldw, T* Y, long long ldy)
+{
+    const int lane = threadIdx.x & 63;
+    const long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    T acc[kTmvColumns];
+#pragma unroll
+    for (int j = 0; j < kTmvColumns; ++j) acc[j] = T(0);
+    const T* col = L + i * ldl;
+    for (long long k = i + lane; k < n; k += 64) {
+        const T l = col[k];
+#pragma unroll
+        for (int j = 0; j < kTmvColumns; ++j)
+            if (j < c) acc[j] = fma(l, W[k + j * ldw], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kTmvColumns; ++j) {
+        if (j >= c) break;
+        T v = acc[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) Y[i + j * ldy] = v;
+    }
+}
+
 template <class T>
 __global__ void diag_adjoint_kernel(int n, const T* L, long long ldl, double g, T* Lbar, long long ldlb)
 {
@@ -802,6 +834,16 @@ void mirror_lower(hipStream_t s, int n, const T* in, long long ldi, T scale, T* 
     if (tiles > 65535) throw std::invalid_argument("mirror_lower: order too large");
     mirror_lower_kernel<T><<<dim3(tiles, tiles), 256, 0, s>>>(n, in, ldi, scale, out, ldo);
     check_launch("mirror_lower");
+}
+
+template <class T>
+void lower_tmv(hipStream_t s, int n, int c, const T* L, long long ldl, const T* W, long long ldw, T* Y,
+               long long ldy)
+{
+    if (n <= 0 || c <= 0) return;
+    if (c > kTmvColumns) throw std::invalid_argument("lower_tmv: too many columns");
+    lower_tmv_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, c, L, ldl, W, ldw, Y, ldy);
+    check_launch("lower_tmv");
 }
 
 template <class T>
@@ -936,6 +978,7 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
                           void*, int*);                                                                       \
     template void tril_copy<T>(hipStream_t, int, const T*, long long, T*, long long);                         \
     template void mirror_lower<T>(hipStream_t, int, const T*, long long, T, T*, long long);                   \
+    template void lower_tmv<T>(hipStream_t, int, int, const T*, long long, const T*, long long, T*, long long); \
     template void diag_adjoint<T>(hipStream_t, int, const T*, long long, double, T*, long long);              \
     template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);
 DDLA_KERN_INSTANTIATE(double)

@@ -13,6 +13,17 @@ launch duration and HBM traffic per launch.  FETCH_SIZE and WRITE_SIZE (KiB
 per dispatch) are scaled by what they read for the copy kernel of known size
 in the same run (__amd_rocclr_copyBuffer: the largest one moves n^2 doubles
 each way), so traffic is reported in bytes.
+
+    python -m libddla_amd.prof_summary --trace DIR --after NAME [--per K] ...
+    python -m libddla_amd.prof_summary --trace DIR --span START END ...
+
+summarize one window of the kernel trace instead: --after keeps the
+dispatches after the last one whose name contains NAME (through the last one
+containing --until, if given); --span keeps those
+from the last dispatch containing START that precedes the last one
+containing END, through that END dispatch.  The window's wall time (first
+start to last end), busy time and per-kernel totals are listed, divided by
+--per (e.g. the number of problems in the window).
 """
 import argparse
 import csv
@@ -44,6 +55,50 @@ def kernel_stats(trace_dir):
         out.append({"name": r["Name"], "calls": int(r["Calls"]), "total_ms": float(r["TotalDurationNs"]) / 1e6,
                     "avg_us": float(r["AverageNs"]) / 1e3, "pct": float(r["Percentage"])})
     return sorted(out, key=lambda x: -x["total_ms"])
+
+
+def trace_window(trace_dir, after=None, span=None, until=None):
+    """Dispatches of the kernel trace in the window selected by after (and until) / span, in start order."""
+    rows = _read(os.path.join(trace_dir, "*kernel_trace.csv"))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    if after:
+        hits = [i for i, r in enumerate(rows) if after in r["Kernel_Name"]]
+        if not hits:
+            raise ValueError(f"no dispatch contains {after!r}")
+        last = len(rows)
+        if until:
+            ends = [i for i, r in enumerate(rows) if until in r["Kernel_Name"] and i > hits[-1]]
+            if not ends:
+                raise ValueError(f"no dispatch containing {until!r} follows the last {after!r}")
+            last = ends[-1] + 1
+        return rows[hits[-1] + 1:last]
+    start, end = span
+    ends = [i for i, r in enumerate(rows) if end in r["Kernel_Name"]]
+    if not ends:
+        raise ValueError(f"no dispatch contains {end!r}")
+    starts = [i for i in range(ends[-1] + 1) if start in rows[i]["Kernel_Name"]]
+    if not starts:
+        raise ValueError(f"no dispatch containing {start!r} precedes the last {end!r}")
+    return rows[starts[-1]:ends[-1] + 1]
+
+
+def window_lines(rows, per):
+    t0 = min(int(r["Start_Timestamp"]) for r in rows)
+    t1 = max(int(r["End_Timestamp"]) for r in rows)
+    agg = {}
+    for r in rows:
+        a = agg.setdefault(short_name(r["Kernel_Name"]), [0, 0.0])
+        a[0] += 1
+        a[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+    busy = sum(v[1] for v in agg.values())
+    lines = [f"Window: {len(rows)} dispatches, wall {(t1 - t0) / 1e6:.3f} ms, kernel busy {busy:.3f} ms"
+             + (f"; per unit (/{per}): wall {(t1 - t0) / 1e6 / per:.3f} ms, busy {busy / per:.3f} ms" if per > 1 else ""),
+             "", "| kernel | calls | total ms | avg us | % of busy |" + (" ms per unit |" if per > 1 else ""),
+             "|---|---:|---:|---:|---:|" + ("---:|" if per > 1 else "")]
+    for name, (calls, ms) in sorted(agg.items(), key=lambda x: -x[1][1])[:20]:
+        lines.append(f"| `{name}` | {calls} | {ms:.3f} | {ms / calls * 1e3:.1f} | {100 * ms / busy:.1f} |"
+                     + (f" {ms / per:.3f} |" if per > 1 else ""))
+    return lines
 
 
 def counters(pmc_dir, counter):
@@ -113,12 +168,27 @@ def main():
                     help="name substrings of the trailing-update kernels to report")
     ap.add_argument("--md")
     ap.add_argument("--traffic-json")
+    ap.add_argument("--after", help="summarize the dispatches after the last one whose name contains this")
+    ap.add_argument("--until", help="with --after: end the window at the last dispatch whose name contains this")
+    ap.add_argument("--span", nargs=2, metavar=("START", "END"),
+                    help="summarize the last START .. END window of dispatches")
+    ap.add_argument("--per", type=int, default=1, help="units (problems, calls) in the window")
+    ap.add_argument("--title", default="rocprofv3 kernel summary")
     args = ap.parse_args()
 
-    stats = kernel_stats(args.trace)
-    lines = ["# rocprofv3 kernel summary", ""]
+    lines = [f"# {args.title}", ""]
     if args.command:
         lines += ["Command:", "", "```", args.command, "```", ""]
+    if args.after or args.span:
+        lines += window_lines(trace_window(args.trace, args.after, args.span, args.until), args.per)
+        text = "\n".join(lines) + "\n"
+        if args.md:
+            with open(args.md, "w") as f:
+                f.write(text)
+        else:
+            print(text)
+        return
+    stats = kernel_stats(args.trace)
     lines += ["| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
     for s in stats[:16]:
         lines.append(f"| `{short_name(s['name'])}` | {s['calls']} | {s['total_ms']:.3f} | {s['avg_us']:.1f} | "
