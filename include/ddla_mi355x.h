@@ -249,8 +249,10 @@ int ddla_c_spd_generate_batch(void* handle, int batch, const int* n, unsigned lo
  * the handle (and clears earlier records); ddla_c_profile_read synchronizes
  * and returns, for kernel class cls, the number of launches, their summed
  * duration in ms and their summed algorithmic flops and bytes.  Classes: 0
- * trailing update, 1 panel factorization, 2 triangular-solve sweeps, 3
- * diagonal-block inverses, 4 refinement residual, 5 other. */
+ * trailing update (the columns beyond the look-ahead panels), 1 panel
+ * factorization, 2 triangular-solve sweeps, 3 diagonal-block inverses, 4
+ * refinement residual, 5 other, 6 look-ahead updates of the next two panels'
+ * columns. */
 int ddla_c_profile(void* handle, int enable);
 int ddla_c_profile_read(void* handle, int cls, int* count, double* ms, double* flops, double* bytes);
 

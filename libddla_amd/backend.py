@@ -79,7 +79,7 @@ class Handle:
         check_status(lib.ddla_c_set_stream(self._handle, ctypes.c_void_p(stream)), "ddla_c_set_stream")
 
     # -- per-launch event profiler -------------------------------------------
-    KERNEL_CLASSES = ("trailing_update", "panel", "solve", "trtri", "residual", "other")
+    KERNEL_CLASSES = ("trailing_update", "panel", "solve", "trtri", "residual", "other", "lookahead_update")
 
     def profile(self, enable: bool):
         check_status(lib.ddla_c_profile(self._handle, 1 if enable else 0), "ddla_c_profile")

@@ -58,13 +58,14 @@ void update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& v, int
 {
     if (M <= 0 || N <= 0) return;
     LaunchTimer timer(h, cls, flops, 0, st);
+    const bool lookahead = cls == kClassLookahead;
     if (!v.upper) {
         kern::gemm<T>(st, 'N', 'T', M, N, K, alpha, v.at(ai, ak), v.lda, v.at(bj, bk), v.lda, beta, v.at(ci, cj),
-                      v.lda, mask ? kTriLower : kTriNone, ci - cj, abort);
+                      v.lda, mask ? kTriLower : kTriNone, ci - cj, abort, lookahead);
     } else {
         // Stored block at (cj, ci) is C_L^T = B_L * A_L^T.
         kern::gemm<T>(st, 'T', 'N', N, M, K, alpha, v.at(bk, bj), v.lda, v.at(ak, ai), v.lda, beta, v.at(cj, ci),
-                      v.lda, mask ? kTriUpper : kTriNone, cj - ci, abort);
+                      v.lda, mask ? kTriUpper : kTriNone, cj - ci, abort, lookahead);
     }
 }
 
@@ -80,24 +81,25 @@ void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k
 }
 
 // BF16 split-product trailing updates (FP32 factorization only): after panel
-// k is factored, its rows below the diagonal block are split into bfloat16
-// planes in buf[(k / NB) % 2] (split_buffer_bytes(n, products) bytes each), and
-// both trailing updates of step k read that buffer.  Two buffers: the split of
-// panel k + 1 (side stream) overlaps the update of step k (main stream).
+// p is factored, its rows below the diagonal block are split into bfloat16
+// planes in buf[p % 3] (split_buffer_bytes(n, products) bytes each), and every
+// trailing update of step p reads that buffer.  Three buffers: the split of
+// panel p + 3 waits (through the look-ahead chain) for step p's update of the
+// rest, the last reader of buf[p % 3].
 struct SplitPlan {
     int products;  // 3 (bf16x3) or 6 (bf16x6)
-    void* buf[2];
+    void* buf[3];
 };
 
 // C_L(r + a_row.., r + b_row..) -= P P^T over the split panel of step k (rows
 // r.. of the L view split into buf).
 template <class T>
-void split_update_lower(DdlaStream* h, hipStream_t st, const LView<T>& v, const SplitPlan& sp, const void* buf,
-                        int m_split, int M, int N, int kb, long long r, int a_row, int b_row, const int* abort,
-                        double flops)
+void split_update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& v, const SplitPlan& sp,
+                        const void* buf, int m_split, int M, int N, int kb, long long r, int a_row, int b_row,
+                        const int* abort, double flops)
 {
     if (M <= 0 || N <= 0) return;
-    LaunchTimer timer(h, kClassTrailingUpdate, flops, 0, st);
+    LaunchTimer timer(h, cls, flops, 0, st);
     if constexpr (std::is_same<T, float>::value) {
         T* C = v.upper ? v.at(r + b_row, r + a_row) : v.at(r + a_row, r + b_row);
         kern::split_update(st, sp.products, v.upper, M, N, kb, buf, m_split, a_row, b_row, C, v.lda, true,
@@ -107,64 +109,77 @@ void split_update_lower(DdlaStream* h, hipStream_t st, const LView<T>& v, const 
     }
 }
 
-// Blocked right-looking Cholesky of the order-n L view with look-ahead; *info
-// (device) must be zero on entry and receives the 1-based index of the first
-// failed pivot; work holds kern::panel_work_bytes<T>(n) bytes.  After panel k is factored, the trailing update is split: the
-// columns of panel k+1 are updated first on the handle's stream, panel k+1 is
-// then factored on the side stream while the rest of the trailing matrix is
-// updated on the handle's stream (the two touch disjoint columns).  Returns
-// with all work ordered on the handle's stream.  split (FP32 only, optional):
-// the trailing updates run as BF16 split products.
+// Blocked right-looking Cholesky of the order-n L view with a look-ahead of
+// two panels; *info (device) must be zero on entry and receives the 1-based
+// index of the first failed pivot; work holds kern::panel_work_bytes<T>(n)
+// bytes.  Step p (panel p factored) applies panel p to
+//   * the columns of panel p + 1, then panel p + 2, on the `look` stream;
+//     panel p + 1 is factored on the `side` stream as soon as its columns are
+//     done, and panel p + 2's update waits for the previous step's update of
+//     the rest (which also wrote those columns);
+//   * every column beyond panel p + 2 on the handle's stream, right after the
+//     previous step's.
+// So the handle's stream runs the large updates back to back while the next
+// panels are updated and factored beside them; concurrent launches write
+// disjoint columns and read only finished panels.  Returns with all work
+// ordered on the handle's stream.  split (FP32 only, optional): the trailing
+// updates run as BF16 split products.
 template <class T>
 void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work,
                    const SplitPlan* split = nullptr)
 {
     const LView<T> v{A, lda, upper};
     const int NB = outer_block<T>();
-    hipStream_t main = h->stream, side = h->side;
-    // splits rows r.. of the panel at columns [k, k + kb) into the buffer of step k
-    auto split_panel = [&](hipStream_t st, int k, int kb) {
+    const int np = (n + NB - 1) / NB;
+    hipStream_t main = h->stream, side = h->side, look = h->look;
+    auto col = [&](int q) { return std::min(n, q * NB); };  // first column of panel q (n past the end)
+    // factors panel q and splits its rows below the diagonal block into buf[q % 3]
+    auto factor = [&](hipStream_t st, int q) {
+        const int k = col(q), kb = col(q + 1) - k;
+        factor_panel<T>(h, st, v, n, k, kb, info, work);
         const long long r = (long long)k + kb;
-        const int m = n - (int)r;
-        if (!split || m <= 0) return;
+        if (!split || r >= n) return;
         const T* X = upper ? v.at(k, r) : v.at(r, k);
         if constexpr (std::is_same<T, float>::value)
-            kern::split_panel(st, split->products, upper, m, kb, X, lda, split->buf[(k / NB) % 2]);
+            kern::split_panel(st, split->products, upper, n - (int)r, kb, X, lda, split->buf[q % 3]);
+    };
+    // C_L(i, j) -= sum over panel p's columns of L(i, .) L(j, .) for columns
+    // c_lo <= j < c_hi and rows i >= j (lower trapezoid), on stream st
+    auto update = [&](hipStream_t st, int p, int c_lo, int c_hi) {
+        const int k = col(p), kb = col(p + 1) - k;
+        const long long r = (long long)k + kb;
+        const int M = n - c_lo, N = c_hi - c_lo;
+        if (M <= 0 || N <= 0) return;
+        const double flops = (double)N * (N + 1) * kb + 2.0 * (M - N) * N * kb;
+        const int cls = st == main ? kClassTrailingUpdate : kClassLookahead;
+        if (split)
+            split_update_lower<T>(h, st, cls, v, *split, split->buf[p % 3], n - (int)r, M, N, kb, r, c_lo - (int)r,
+                                  c_lo - (int)r, info, flops);
+        else
+            update_lower<T>(h, st, cls, v, M, N, kb, T(-1), c_lo, k, c_lo, k, T(1), c_lo, c_lo, true, info, flops);
     };
     kern::panel_reset<T>(main, n, work);
-    factor_panel<T>(h, main, v, n, 0, std::min(NB, n), info, work);
-    split_panel(main, 0, std::min(NB, n));
-    for (int k = 0; k < n; k += NB) {
-        const int kb = std::min(NB, n - k);
-        const int m = n - k - kb;
-        if (m <= 0) break;
-        const int kb2 = std::min(NB, m);  // width of the next panel
-        const long long r = (long long)k + kb;
-        const double next_flops = (double)kb2 * (kb2 + 1) / 2 * 2 * kb + 2.0 * (m - kb2) * kb2 * kb;
-        // next panel's columns: rows r.., columns [r, r + kb2)
-        if (split)
-            split_update_lower<T>(h, main, v, *split, split->buf[(k / NB) % 2], m, m, kb2, kb, r, 0, 0, info,
-                                  next_flops);
-        else
-            update_lower<T>(h, main, kClassTrailingUpdate, v, m, kb2, kb, T(-1), r, k, r, k, T(1), r, r, true, info,
-                            next_flops);
-        DDLA_HIP(hipEventRecord(h->ev_update, main));
-        DDLA_HIP(hipStreamWaitEvent(side, h->ev_update, 0));
-        factor_panel<T>(h, side, v, n, (int)r, kb2, info, work);
-        split_panel(side, (int)r, kb2);
-        DDLA_HIP(hipEventRecord(h->ev_panel, side));
-        // the rest of the trailing matrix: rows and columns [r + kb2, n)
-        const int mr = m - kb2;
-        if (mr > 0) {
-            if (split)
-                split_update_lower<T>(h, main, v, *split, split->buf[(k / NB) % 2], m, mr, mr, kb, r, kb2, kb2, info,
-                                      (double)mr * (mr + 1) * kb);
-            else
-                update_lower<T>(h, main, kClassTrailingUpdate, v, mr, mr, kb, T(-1), r + kb2, k, r + kb2, k, T(1),
-                                r + kb2, r + kb2, true, info, (double)mr * (mr + 1) * kb);
-        }
+    factor(main, 0);
+    DDLA_HIP(hipEventRecord(h->ev_panel, main));
+    for (int p = 0; p + 1 < np; ++p) {
+        // panel p is factored
+        DDLA_HIP(hipStreamWaitEvent(look, h->ev_panel, 0));
         DDLA_HIP(hipStreamWaitEvent(main, h->ev_panel, 0));
+        update(look, p, col(p + 1), col(p + 2));
+        DDLA_HIP(hipEventRecord(h->ev_look, look));
+        DDLA_HIP(hipStreamWaitEvent(side, h->ev_look, 0));
+        factor(side, p + 1);
+        DDLA_HIP(hipEventRecord(h->ev_panel, side));
+        if (p + 2 < np) {
+            if (p > 0) DDLA_HIP(hipStreamWaitEvent(look, h->ev_rest, 0));
+            update(look, p, col(p + 2), col(p + 3));
+        }
+        update(main, p, col(p + 3), n);
+        DDLA_HIP(hipEventRecord(h->ev_rest, main));
     }
+    DDLA_HIP(hipEventRecord(h->ev_look, look));
+    DDLA_HIP(hipStreamWaitEvent(main, h->ev_look, 0));
+    DDLA_HIP(hipStreamWaitEvent(main, h->ev_panel, 0));
 }
 
 // Left solve A X = B with A = L L^T given by its L view; B is n x nrhs: the
@@ -542,9 +557,9 @@ template void potrf_blocked_dispatch<float>(DdlaStream*, bool, int, float*, long
 template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, long long, int*, void*);
 
 void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long lda, int* info, void* work,
-                          int products, void* buf0, void* buf1)
+                          int products, void* const* bufs)
 {
-    const SplitPlan plan{products, {buf0, buf1}};
+    const SplitPlan plan{products, {bufs[0], bufs[1], bufs[2]}};
     potrf_blocked<float>(h, upper, n, A, lda, info, work, &plan);
 }
 

@@ -427,6 +427,20 @@ __global__ __launch_bounds__(256, WAVES_PER_EU) void syrk_update_kernel(GemmProb
     gemm_tile<T, BM, BN, BK, TA, TB, NBUF>(p, ti * BM, tj * BN, smem);
 }
 
+// The same tile loop once more, for the factorization's look-ahead updates of
+// the next panels' columns, so that kernel traces keep them apart from the
+// trailing updates above.
+template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
+__global__ __launch_bounds__(256, WAVES_PER_EU) void panel_update_kernel(GemmProblem<T> p, TileOrder o,
+                                                                          const int* abort)
+{
+    if (abort != nullptr && *abort != 0) return;
+    int ti, tj;
+    if (!map_tile(o, blockIdx.x, ti, tj)) return;
+    __shared__ T smem[lds_elems<BM, BN, BK>()];
+    gemm_tile<T, BM, BN, BK, TA, TB>(p, ti * BM, tj * BN, smem);
+}
+
 template <class T>
 struct VBatch {
     const int *m, *n, *k;

@@ -27,7 +27,8 @@ enum KernelClass {
     kClassTrtri = 3,           // inversion of the diagonal blocks for potrs
     kClassResidual = 4,        // FP64 residual R = B - A X of the refinement (MFMA)
     kClassOther = 5,
-    kClassCount = 6
+    kClassLookahead = 6,       // look-ahead updates of the next panels' columns (MFMA, beside the trailing update)
+    kClassCount = 7
 };
 
 class DdlaStream {
@@ -37,10 +38,11 @@ public:
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
-    // Side stream of the look-ahead factorization (non-blocking, default
-    // priority) and the two events that order it against `stream`.
-    hipStream_t side = nullptr;
-    hipEvent_t ev_update = nullptr, ev_panel = nullptr;
+    // Streams of the look-ahead factorization (non-blocking, default
+    // priority): `side` factors panels, `look` updates the next two panels'
+    // columns; the three events order them against `stream`.
+    hipStream_t side = nullptr, look = nullptr;
+    hipEvent_t ev_look = nullptr, ev_panel = nullptr, ev_rest = nullptr;
     int nprows_ = 0, npcols_ = 0, myprow_ = 0, mypcol_ = 0, nprocs = 0;
     char major = 'R';
     int num_cus = 0;  // compute units of the device (set by ddlaSet)

@@ -17,11 +17,13 @@ enum Tri { kTriNone = 0, kTriLower = 1, kTriUpper = 2 };
 // non-null and *abort != 0 when a workgroup starts, the workgroup does nothing
 // (the factorization stops at the first failed pivot).  In-place use with C
 // aliasing op(B) is safe when m <= 64 (one workgroup row covers all of m),
-// and with C aliasing op(A) when n <= 64.
+// and with C aliasing op(A) when n <= 64.  Triangle-masked launches run under
+// the symbol syrk_update_kernel, or panel_update_kernel when lookahead (the
+// same code; kernel traces tell the factorization's two update kinds apart).
 template <class T>
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, long long lda,
           const T* B, long long ldb, T beta, T* C, long long ldc, Tri tri = kTriNone, long long diag = 0,
-          const int* abort = nullptr);
+          const int* abort = nullptr, bool lookahead = false);
 
 // Variable-size batched GEMM over device arrays of sizes, leading dimensions
 // and pointers; max_m / max_n size the grid; tri masks every C (diagonal

@@ -324,7 +324,7 @@ bool is_n(char op) { return op == 'N' || op == 'n'; }
 
 // Builds the tile schedule of one launch (see map_tile) and launches it.
 template <class T, int BM, int BN, int BK, int WPE>
-void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, const int* abort)
+void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, const int* abort, bool lookahead)
 {
     TileOrder o;
     o.ntm = (p.m + BM - 1) / BM;
@@ -336,10 +336,15 @@ void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, c
     o.group = 8;
     o.per_xcd = (o.total + 7) / 8;
     const dim3 grid((unsigned)(o.per_xcd * 8)), block(256);
-    if (p.tri != kTriNoneD) {
+    if (p.tri != kTriNoneD && !lookahead) {
         // the trailing updates: C (N T) or its mirror (T N)
         if (!ta && tb) syrk_update_kernel<T, BM, BN, BK, false, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
         else if (ta && !tb) syrk_update_kernel<T, BM, BN, BK, true, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
+        else throw std::invalid_argument("gemm: triangle-masked products are N T or T N");
+    } else if (p.tri != kTriNoneD) {
+        // the look-ahead updates of the next panels' columns
+        if (!ta && tb) panel_update_kernel<T, BM, BN, BK, false, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
+        else if (ta && !tb) panel_update_kernel<T, BM, BN, BK, true, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
         else throw std::invalid_argument("gemm: triangle-masked products are N T or T N");
     } else if (!ta && !tb) {
         gemm_kernel<T, BM, BN, BK, false, false, WPE><<<grid, block, 0, s>>>(p, o, abort);
@@ -373,15 +378,15 @@ void launch_vbatched_cfg(hipStream_t s, bool ta, bool tb, const VBatch<T>& v, in
 
 template <class T>
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, long long lda, const T* B,
-          long long ldb, T beta, T* C, long long ldc, Tri tri, long long diag, const int* abort)
+          long long ldb, T beta, T* C, long long ldc, Tri tri, long long diag, const int* abort, bool lookahead)
 {
     if (m <= 0 || n <= 0) return;
     constexpr int VW = 16 / sizeof(T);
     const int vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % VW == 0 && ldb % VW == 0;
     GemmProblem<T> p{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, (int)tri, diag, vec};
     const bool a_t = !is_n(ta), b_t = !is_n(tb);
-    if (m >= 128 && n >= 128) launch_gemm_cfg<T, 128, 128, 16, 2>(s, a_t, b_t, p, abort);
-    else launch_gemm_cfg<T, 64, 64, 16, 2>(s, a_t, b_t, p, abort);
+    if (m >= 128 && n >= 128) launch_gemm_cfg<T, 128, 128, 16, 2>(s, a_t, b_t, p, abort, lookahead);
+    else launch_gemm_cfg<T, 64, 64, 16, 2>(s, a_t, b_t, p, abort, lookahead);
 }
 
 template <class T>
@@ -955,7 +960,7 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
 
 #define DDLA_KERN_INSTANTIATE(T)                                                                              \
     template void gemm<T>(hipStream_t, char, char, int, int, int, T, const T*, long long, const T*, long long, \
-                          T, T*, long long, Tri, long long, const int*);                                      \
+                          T, T*, long long, Tri, long long, const int*, bool);                                \
     template void gemm_vbatched<T>(hipStream_t, char, char, const int*, const int*, const int*, T,            \
                                    const T* const*, const int*, const T* const*, const int*, T, T* const*,    \
                                    const int*, int, int, int, Tri);                                           \

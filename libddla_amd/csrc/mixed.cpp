@@ -22,7 +22,7 @@ namespace ddla {
 template <class T>
 void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work);
 void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long lda, int* info, void* work,
-                          int products, void* buf0, void* buf1);
+                          int products, void* const* bufs);
 
 namespace {
 
@@ -31,7 +31,7 @@ constexpr int kIB = 64;
 std::size_t diag_inverse_elems(int n) { return (std::size_t)((n + kIB - 1) / kIB) * kIB * kIB; }
 
 struct MixedLayout {
-    std::size_t a32, norm, scalars, x, r, w, info, panel, dinv, sync, error, split0, split1, total;
+    std::size_t a32, norm, scalars, x, r, w, info, panel, dinv, sync, error, split[3], total;
 };
 
 MixedLayout mixed_layout(int n, int nrhs)
@@ -55,8 +55,7 @@ MixedLayout mixed_layout(int n, int nrhs)
     l.dinv = put(diag_inverse_elems(n) * sizeof(float));
     l.sync = put(kern::solve_work_bytes(nrhs));
     l.error = put(sizeof(int));
-    l.split0 = put(kern::split_buffer_bytes(n, 6));
-    l.split1 = put(kern::split_buffer_bytes(n, 6));
+    for (std::size_t& b : l.split) b = put(kern::split_buffer_bytes(n, 6));
     l.total = off;
     return l;
 }
@@ -124,11 +123,12 @@ void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, co
         kern::convert_sym_with_norm<float, double>(s, n, !upper, A, lda, a32, n, base + l.norm, scalars);
     }
     fill_int(s, info32, 0);
-    if (factor == 0)
+    if (factor == 0) {
         potrf_blocked_dispatch<float>(h, upper, n, a32, n, info32, base + l.panel);
-    else
-        potrf_split_dispatch(h, upper, n, a32, n, info32, base + l.panel, factor == 1 ? 3 : 6, base + l.split0,
-                             base + l.split1);
+    } else {
+        void* const bufs[3] = {base + l.split[0], base + l.split[1], base + l.split[2]};
+        potrf_split_dispatch(h, upper, n, a32, n, info32, base + l.panel, factor == 1 ? 3 : 6, bufs);
+    }
     int host_info = 0, panel_error = 0;
     DDLA_HIP(hipMemcpyAsync(&host_info, info32, sizeof(int), hipMemcpyDeviceToHost, s));
     DDLA_HIP(hipMemcpyAsync(&panel_error, kern::panel_error_word<float>(n, base + l.panel), sizeof(int),

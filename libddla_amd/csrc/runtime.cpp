@@ -114,8 +114,10 @@ ddlaStatus_t ddlaSet(DdlaHandle_t handle, int nprows, int npcols, char major)
         DDLA_HIP(hipStreamCreate(&handle->stream));
         handle->owns_stream = true;
         DDLA_HIP(hipStreamCreateWithFlags(&handle->side, hipStreamNonBlocking));
-        DDLA_HIP(hipEventCreateWithFlags(&handle->ev_update, hipEventDisableTiming));
+        DDLA_HIP(hipStreamCreateWithFlags(&handle->look, hipStreamNonBlocking));
+        DDLA_HIP(hipEventCreateWithFlags(&handle->ev_look, hipEventDisableTiming));
         DDLA_HIP(hipEventCreateWithFlags(&handle->ev_panel, hipEventDisableTiming));
+        DDLA_HIP(hipEventCreateWithFlags(&handle->ev_rest, hipEventDisableTiming));
     }
     handle->nprows_ = handle->npcols_ = handle->nprocs = 1;
     handle->myprow_ = handle->mypcol_ = 0;
@@ -131,12 +133,13 @@ ddlaStatus_t ddlaDestroy(DdlaHandle_t& handle)
         (void)hipStreamSynchronize(handle->stream);
         if (handle->scratch != nullptr) (void)hipFree(handle->scratch);
         for (hipEvent_t e : handle->event_pool) (void)hipEventDestroy(e);
-        if (handle->side != nullptr) {
-            (void)hipStreamSynchronize(handle->side);
-            (void)hipStreamDestroy(handle->side);
+        for (hipStream_t st : {handle->side, handle->look}) {
+            if (st == nullptr) continue;
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
         }
-        if (handle->ev_update != nullptr) (void)hipEventDestroy(handle->ev_update);
-        if (handle->ev_panel != nullptr) (void)hipEventDestroy(handle->ev_panel);
+        for (hipEvent_t e : {handle->ev_look, handle->ev_panel, handle->ev_rest})
+            if (e != nullptr) (void)hipEventDestroy(e);
         if (handle->owns_stream && handle->stream != nullptr) (void)hipStreamDestroy(handle->stream);
     }
     delete handle;

@@ -120,40 +120,43 @@ def traffic(fetch_dir, write_dir, n):
     return f_scale, w_scale, sum(fs) * 1024.0 * f_scale / len(fs), sum(ws) * 1024.0 * w_scale / len(ws), len(fs)
 
 
-def trailing_flops_per_launch(n, nb):
-    """Mean algorithmic flops over the trailing-update launches of one factorization: U_next
-    kb2 (kb2 + 1) kb + 2 (m - kb2) kb2 kb, U_rest m_r (m_r + 1) kb."""
-    total, launches = 0.0, 0
-    for k in range(0, n, nb):
-        m = n - k - nb
-        if m <= 0:
+def rest_updates(n, nb):
+    """(M, kb) of the square trailing updates of one factorization: step p applies panel p to every
+    column beyond panel p + 2 (the next two panels' columns are look-ahead updates, another kernel)."""
+    np_ = (n + nb - 1) // nb
+    out = []
+    for p in range(np_):
+        c = min(n, (p + 3) * nb)
+        if c >= n:
             break
-        kb2 = min(nb, m)
-        total += kb2 * (kb2 + 1) * nb + 2.0 * (m - kb2) * kb2 * nb
-        launches += 1
-        mr = m - kb2
-        if mr > 0:
-            total += mr * (mr + 1.0) * nb
-            launches += 1
-    return total / launches, launches
+        out.append((n - c, min(nb, n - p * nb)))
+    return out
+
+
+def trailing_flops_per_launch(n, nb):
+    """Mean algorithmic flops over the square trailing-update launches of one factorization: M (M + 1) kb."""
+    ups = rest_updates(n, nb)
+    return sum(m * (m + 1.0) * kb for m, kb in ups) / len(ups), len(ups)
+
+
+def all_updates(n, nb):
+    """(flops, launches) of every update of one factorization -- the look-ahead updates of the next two
+    panels' columns and the square trailing updates: panel p reaches the lower triangle of every column
+    beyond it, M (M + 1) kb flops with M = n - (p + 1) nb."""
+    np_ = (n + nb - 1) // nb
+    flops, launches = 0.0, 0
+    for p in range(np_ - 1):
+        m = n - min(n, (p + 1) * nb)
+        flops += m * (m + 1.0) * min(nb, n - p * nb)
+        launches += 1 + (p + 2 < np_) + (p + 3 < np_)
+    return flops, launches
 
 
 def algorithmic_bytes_per_launch(n, nb):
-    """Mean over the trailing-update launches of one factorization: the panel (m x nb) read once and the
-    lower triangle of C read and written (U_next: m x kb2 block, U_rest: the rest)."""
-    total, launches = 0.0, 0
-    for k in range(0, n, nb):
-        m = n - k - nb
-        if m <= 0:
-            break
-        kb2 = min(nb, m)
-        total += 8.0 * (m * nb + kb2 * (kb2 + 1) + 2.0 * (m - kb2) * kb2)  # U_next
-        launches += 1
-        mr = m - kb2
-        if mr > 0:
-            total += 8.0 * (mr * nb + mr * (mr + 1))  # U_rest
-            launches += 1
-    return total / launches, launches
+    """Mean over the square trailing-update launches: the panel rows read once (M x kb) and the lower
+    triangle of C read and written."""
+    ups = rest_updates(n, nb)
+    return sum(8.0 * (m * kb + m * (m + 1.0)) for m, kb in ups) / len(ups), len(ups)
 
 
 def main():
@@ -166,6 +169,9 @@ def main():
     ap.add_argument("--command", default="")
     ap.add_argument("--focus", nargs="+", default=["syrk_update_kernel"],
                     help="name substrings of the trailing-update kernels to report")
+    ap.add_argument("--focus-all", nargs="+", default=[],
+                    help="name substrings of kernels that run every update of the factorization (look-ahead and "
+                         "trailing): reported as all update flops over their summed time per factorization")
     ap.add_argument("--md")
     ap.add_argument("--traffic-json")
     ap.add_argument("--after", help="summarize the dispatches after the last one whose name contains this")
@@ -200,6 +206,17 @@ def main():
                           f"{s['avg_us'] / 1e3:.4f} ms per launch; {per_fact} launches per factorization at "
                           f"n = {args.n}, nb = {args.nb}, mean {flops / 1e9:.1f} GFLOP (FP32- or FP64-equivalent) "
                           f"per launch -> {flops / (s['avg_us'] * 1e-6) / 1e12:.1f} TFLOP/s."]
+    all_flops, all_launches = all_updates(args.n, args.nb)
+    for focus in args.focus_all:
+        for s in [s for s in stats if focus in s["name"]]:
+            fact = s["calls"] / all_launches
+            per_fact_ms = s["total_ms"] / fact
+            lines += ["", f"Updates `{short_name(s['name'])}`: {s['calls']} launches = {fact:.2f} factorizations of "
+                          f"{all_launches} launches (look-ahead and trailing) at n = {args.n}, nb = {args.nb}; "
+                          f"{per_fact_ms:.2f} ms of summed launch time per factorization for "
+                          f"{all_flops / 1e12:.3f} TFLOP (FP32-equivalent) -> at least "
+                          f"{all_flops / (per_fact_ms * 1e-3) / 1e12:.1f} TFLOP/s (the look-ahead launches overlap "
+                          f"the trailing ones, so summed durations exceed the time they occupy)."]
     result = {}
     if args.fetch and args.write:
         f_scale, w_scale, fb, wb, nl = traffic(args.fetch, args.write, args.n)
