@@ -187,7 +187,7 @@ __device__ inline void block_apply(const T* m, long long rs, long long ks, int r
 // the 1-based global index of a failed pivot (then only L(j, j) is written).
 // logpart (thread 0) receives sum log L(k, k) over the block (fixed-order
 // butterfly).  S, X: kB * kL elements of LDS each; rdiag: kB; s_flag: one int.
-template <class T>
+template <class T, bool OUT_OF_LINE = false>
 __device__ inline int factor_block_column(T* A, View v, int n, int nb, int j, T* dj, T* S, T* X, T* rdiag,
                                           int* s_flag, double& logpart, bool below = true)
 {
@@ -197,7 +197,8 @@ __device__ inline int factor_block_column(T* A, View v, int n, int nb, int j, T*
     T* Ajj = A + j0 + j0 * lda;
     potf2_detail::load_block<T>(v.upper != 0, wj, Ajj, lda, S);
     __syncthreads();
-    const int f = potf2_detail::factor_invert64<T>(S, X, rdiag, s_flag);
+    const int f = OUT_OF_LINE ? potf2_detail::factor_invert64_call<T>(S, X, rdiag, s_flag)
+                              : potf2_detail::factor_invert64<T>(S, X, rdiag, s_flag);
     potf2_detail::store_factor<T>(v.upper != 0, wj, S, Ajj, lda);
     if (f != 0) return (int)j0 + f;
     // log of the diagonal: wave 0, fixed-order butterfly
@@ -392,8 +393,9 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_pposv_kernel(BatchArgs<
             const long long j0 = (long long)kB * j;
             const int wj = min(kB, n - (int)j0);
             double l = 0.0;
-            const int f = factor_block_column<T>(A, v, n, nb, j, slot + (long long)j * kB * kB, S, X, rdiag, &s_flag,
-                                                 l);
+            // two workgroups per CU leave too few registers to inline the factorization
+            const int f = factor_block_column<T, (OCC > 1)>(A, v, n, nb, j, slot + (long long)j * kB * kB, S, X,
+                                                            rdiag, &s_flag, l);
             if (f != 0) {
                 fail = f;
                 break;
