@@ -385,11 +385,11 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
     const int vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % VW == 0 && ldb % VW == 0;
     GemmProblem<T> p{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, (int)tri, diag, vec};
     const bool a_t = !is_n(ta), b_t = !is_n(tb);
-    // 128 x 128 tiles unless they would leave most of the 256 CUs idle (fewer
-    // than 256 tiles) where 64 x 64 tiles give at least twice as many
+    // 128 x 128 tiles unless there are fewer than 512 of them (two per CU)
+    // and 64 x 64 tiles give at least twice as many
     const long long t128 = (long long)((m + 127) / 128) * ((n + 127) / 128);
     const long long t64 = (long long)((m + 63) / 64) * ((n + 63) / 64);
-    if (m >= 128 && n >= 128 && (t128 >= 256 || t64 < 2 * t128))
+    if (m >= 128 && n >= 128 && (t128 >= 512 || t64 < 2 * t128))
         launch_gemm_cfg<T, 128, 128, 16, 2>(s, a_t, b_t, p, abort, lookahead);
     else
         launch_gemm_cfg<T, 64, 64, 16, 2>(s, a_t, b_t, p, abort, lookahead);
