@@ -83,7 +83,7 @@ void lower_product(hipStream_t s, int n, const T* Lz, const T* Bz, T* M)
 //   C11 -= L21^T X21 (lower triangle),  tril(X11) from (L11, C11)
 // -- n^3 / 3 flops instead of n^3.  Leaves of order <= kLowerLeaf take the
 // full solve.
-constexpr int kLowerLeaf = 512;
+constexpr int kLowerLeaf = 1024;
 
 template <class T>
 void lower_solve(hipStream_t s, int n, const T* L, long long ldl, const T* dinv, T* C, long long ldc, void* sync,

@@ -130,7 +130,7 @@ void trsm_sweep(hipStream_t s, bool upper, bool backward, int n, int nrhs, const
 // off-diagonal block is applied by one GEMM, so that the GEMM carries most of
 // the n^2 nrhs flops and the sweep kernel only the leaves of order <=
 // kTrsmLeaf; otherwise one sweep.  potrs_sweeps takes the same route.
-constexpr int kTrsmLeaf = 512;
+constexpr int kTrsmLeaf = 1024;
 constexpr int kTrsmMinColumns = 256;
 bool trsm_splits(int n, int nrhs);
 template <class T>

@@ -31,10 +31,11 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(1e-300, np.abs(b).max()))
 
 
-@pytest.mark.parametrize("n", [1, 64, 100, 257, 600, 1100])
+@pytest.mark.parametrize("n", [1, 64, 100, 257, 600, 1100, 2100])
 def test_cholesky_backward_matches_torch(handle, n):
-    """Orders above 512 take the recursive solves (split orders, ragged
-    64-row leaves) and the k-blocked product."""
+    """Orders above 1024 take the recursive solves (split orders, ragged
+    64-row leaves; 2100 splits twice) and every order the k-blocked
+    product."""
     print(f"case cholesky backward n={n}", flush=True)
     a = spd(n, n)
     lbar = np.tril(np.random.RandomState(n + 1).standard_normal((n, n)))

@@ -177,10 +177,10 @@ def test_ppotrs_flags_closed_form_and_readonly_factor(handle, dtype, side, uplo,
 @pytest.mark.parametrize("side", ["L", "R"])
 @pytest.mark.parametrize("uplo", ["L", "U"])
 def test_ppotrs_many_rhs_recursive_split(handle, dtype, side, uplo):
-    """Many right-hand sides (n = 1100, 300 columns): the solve splits the
-    order recursively (GEMM off-diagonal updates, ragged 64-row leaves) in
-    both storage orientations; closed-form solution as above."""
-    n, nrhs = 1100, 300
+    """Many right-hand sides (n = 2100, 300 columns): the solve splits the
+    order recursively (GEMM off-diagonal updates, ragged 64-row leaves, two
+    levels) in both storage orientations; closed-form solution as above."""
+    n, nrhs = 2100, 300
     print(f"case many-rhs n={n} nrhs={nrhs} side={side} uplo={uplo} {np.dtype(dtype).name}", flush=True)
     a = hpd_real(n)
     x_true = x_real(n, nrhs) if side == "L" else x_real(n, nrhs).T
