@@ -43,7 +43,7 @@ void vbatched_validate(hipStream_t s, int na, int nb, const int* m, const int* n
 // Panel factorization of the blocked Cholesky (one persistent launch per
 // panel).  The factor is addressed as an "L view": element (i, j), i >= j, at
 // A[i + j*lda], or A[j + i*lda] when upper.  panel_factor factors columns
-// [k, k + kb) (kb <= 512, every earlier trailing update applied): the
+// [k, k + kb) (kb <= 768, every earlier trailing update applied): the
 // diagonal blocks, their inverses and the rows below.  On a non-positive or
 // NaN pivot in global column p (0-based) it sets *info = p + 1; it does
 // nothing if *info != 0 on entry.  work holds panel_work_bytes<T>(n) bytes,

@@ -516,7 +516,7 @@ void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, 
 {
     using namespace panel_detail;
     if (kb <= 0 || k < 0 || k + kb > n) return;
-    if (kb > kMaxCols * kB) throw std::invalid_argument("panel_factor: panel wider than 512 columns");
+    if (kb > kMaxCols * kB) throw std::invalid_argument("panel_factor: panel wider than 768 columns");
     if (k % kB != 0) throw std::invalid_argument("panel_factor: panel must start on a 64-column boundary");
     PanelArgs<T> a{A, lda, upper ? 1 : 0, n, k, kb, info, static_cast<T*>(work),
                    panel_sync<T>(work) + (size_t)(k / kB) * kSyncInts, panel_error_word<T>(n, work), 1 << 23};

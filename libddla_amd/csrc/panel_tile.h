@@ -1,7 +1,7 @@
 // One block column (panel) of the blocked Cholesky factorization in a single
 // persistent launch (device code, included by kernels.hip).
 //
-// The panel is columns [k, k + kb) (kb <= 512) of the L view, rows k .. n-1,
+// The panel is columns [k, k + kb) (kb <= 768) of the L view, rows k .. n-1,
 // cut into 64 x 64 blocks: P = ceil(kb / 64) block columns and R = ceil((n -
 // k) / 64) block rows.  Workgroup t (t from a ticket counter, so tickets are
 // handed out in dispatch order) owns block row t and factors it left-looking:
@@ -40,7 +40,7 @@ using gemm_detail::Mfma;
 using potf2_detail::kB;
 using potf2_detail::kL;
 
-constexpr int kMaxCols = 8;   // block columns per panel (panel width <= 512)
+constexpr int kMaxCols = 12;  // block columns per panel (panel width <= 768)
 constexpr int kSyncInts = 16; // per panel: [0] ticket, [1 + t] progress of block row t, [1 + kMaxCols] failure
 constexpr int kPer = kB * kB / 256;
 

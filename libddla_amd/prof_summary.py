@@ -201,11 +201,16 @@ def main():
                      f"{s['pct']:.2f} |")
     flops, per_fact = trailing_flops_per_launch(args.n, args.nb)
     for focus in args.focus:
-        for s in [s for s in stats if focus in s["name"]]:
-            lines += ["", f"Trailing update `{short_name(s['name'])}`: {s['calls']} launches, average "
-                          f"{s['avg_us'] / 1e3:.4f} ms per launch; {per_fact} launches per factorization at "
-                          f"n = {args.n}, nb = {args.nb}, mean {flops / 1e9:.1f} GFLOP (FP32- or FP64-equivalent) "
-                          f"per launch -> {flops / (s['avg_us'] * 1e-6) / 1e12:.1f} TFLOP/s."]
+        rows = [s for s in stats if focus in s["name"]]
+        if not rows:
+            continue
+        calls = sum(s["calls"] for s in rows)
+        avg_ms = sum(s["total_ms"] for s in rows) / calls
+        names = ", ".join(f"`{short_name(s['name'])}` ({s['calls']})" for s in rows)
+        lines += ["", f"Trailing update {names}: {calls} launches, average {avg_ms:.4f} ms per launch; "
+                      f"{per_fact} launches per factorization at n = {args.n}, nb = {args.nb}, mean "
+                      f"{flops / 1e9:.1f} GFLOP (FP32- or FP64-equivalent) per launch -> "
+                      f"{flops / (avg_ms * 1e-3) / 1e12:.1f} TFLOP/s."]
     all_flops, all_launches = all_updates(args.n, args.nb)
     for focus in args.focus_all:
         for s in [s for s in stats if focus in s["name"]]:

@@ -125,7 +125,11 @@ def test_gp_likelihood_gradient_config5(handle):
             return float(ag.gp_negative_log_likelihood(Kp, yt, handle))
 
     fd_l = (f_of_l(0.2 + h) - f_of_l(0.2 - h)) / (2 * h)
-    assert abs(fd_l - dl) <= 1e-8 * abs(dl)
+    # The loss carries rounding noise of about cond(K) eps |y^T alpha| ~ 1e-8
+    # (cond(K) ~ 1e5 with the 1e-2 jitter), so a step of 1e-6 leaves ~1e-2 of
+    # noise in the difference quotient, ~1e-8 of dl; the 1e-10 checks above are
+    # the exact ones.
+    assert abs(fd_l - dl) <= 1e-7 * abs(dl)
     rng = np.random.RandomState(5)
     Kd = K.detach()
     scale = np.abs(kbar).max()
