@@ -205,6 +205,10 @@ void potrs_backward(const DdlaHandle_t& handle, char uplo, int n, int nrhs, cons
                     const T* Xbar, int ldxb, T* Bbar, int ldbb, T* Lbar, int ldlb);
 template <typename T>
 void logdet_backward(const DdlaHandle_t& handle, int n, const T* L, int ldl, double g, T* Lbar, int ldlb);
+// A^-1 (full symmetric, n x n at Ainv) from the factor of A = L L^T in the
+// lower triangle (LAPACK ?potri, which returns one triangle); synchronized.
+template <typename T>
+void potri(const DdlaHandle_t& handle, char uplo, int n, const T* L, int ldl, T* Ainv, int lda);
 
 namespace cholesky_impl {
 // Three components per routine: <r>_check (validation), <r>_workspace_size

@@ -227,6 +227,14 @@ int ddla_d_potrs_backward(void* handle, char uplo, int n, int nrhs, const void* 
                           const void* Xbar, int ldxb, void* Bbar, int ldbb, void* Lbar, int ldlb);
 int ddla_s_logdet_backward(void* handle, int n, const void* L, int ldl, double g, void* Lbar, int ldlb);
 int ddla_d_logdet_backward(void* handle, int n, const void* L, int ldl, double g, void* Lbar, int ldlb);
+/* A^-1 of A = L L^T from its factor (uplo 'L'), written as the full
+ * symmetric matrix at Ainv (LAPACK ?potri returns one triangle; no
+ * counterpart in the reference).  Formed as tril(L^-1) by recursive
+ * triangular inversion and then tril(L^-T L^-1), about n^3 flops; it gives
+ * the closed-form Gaussian-process gradient (K^-1 - alpha alpha^T) / 2.
+ * Returns synchronized. */
+int ddla_s_potri(void* handle, char uplo, int n, const void* L, int ldl, void* Ainv, int lda);
+int ddla_d_potri(void* handle, char uplo, int n, const void* L, int ldl, void* Ainv, int lda);
 
 /* ---- harness helpers (no counterpart in the reference) ------------------
  * Philox4x64-10 + Box-Muller normals of the project's seed convention

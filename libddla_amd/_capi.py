@@ -131,6 +131,7 @@ for _p in ("s", "d"):
         f"ddla_{_p}_potrf_backward": (_i, [_vp, _c, _i, _vp, _i, _vp, _i, _vp, _i]),
         f"ddla_{_p}_potrs_backward": (_i, [_vp, _c, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i]),
         f"ddla_{_p}_logdet_backward": (_i, [_vp, _i, _vp, _i, ctypes.c_double, _vp, _i]),
+        f"ddla_{_p}_potri": (_i, [_vp, _c, _i, _vp, _i, _vp, _i]),
         f"ddla_{_p}_pposv_vbatched": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i]),
         f"ddla_{_p}_pposv_vbatched_workspace_size": (_i, [_vp, _i, _i, _llp]),
         f"ddla_{_p}_pposv_vbatched_run": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _ll]),

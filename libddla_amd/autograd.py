@@ -10,6 +10,10 @@ Backward passes (ddla_{s,d}_*_backward):
                     gradient, as torch.linalg.cholesky's
     cholesky_solve: Bbar = A^-1 Xbar, Lbar = -tril(Bbar (L^T X)^T + X (L^T Bbar)^T)
     cholesky_logdet: Lbar = 2 g diag(1 / L_ii)
+gp_negative_log_likelihood(K, y, h) is one Function whose backward is the
+closed form (K^-1 - alpha alpha^T) / 2 with K^-1 from ddla_{s,d}_potri;
+gp_negative_log_likelihood_composed builds the same loss from the three
+Functions above.
 The library works on column-major storage; tensors are passed through
 column-major copies (or views, when they already are), and torch's current
 stream is synchronized before each call (every call returns synchronized).
@@ -172,8 +176,71 @@ def cholesky_logdet(L, h):
     return CholeskyLogdet.apply(L, h)
 
 
+def potri(L, h):
+    """A^-1 (full symmetric, column-major) of A = L L^T from the lower factor L."""
+    p = _pfx(L)
+    n = L.shape[0]
+    Lc = _colmajor(L.detach())
+    Ainv = _empty_colmajor(n, n, L)
+    _sync()
+    if n:
+        check_status(getattr(lib, f"ddla_{p}_potri")(h.ptr, b"L", n, _ptr(Lc), n, _ptr(Ainv), n), f"ddla_{p}_potri")
+    return Ainv
+
+
+class GPNegativeLogLikelihood(torch.autograd.Function):
+    """loss = y^T K^-1 y / 2 + log det K / 2 in one Function: forward runs
+    ppotrf, ppotrs and diag_logdet on the factor; backward uses the closed
+    form dloss/dK = (K^-1 - alpha alpha^T) / 2, alpha = K^-1 y (Rasmussen &
+    Williams 2006, eq. 5.9, the symmetric-input convention), with K^-1 from
+    potri (about n^3 flops, against ~1.8 n^3 for the composed adjoints), and
+    dloss/dy = alpha."""
+
+    @staticmethod
+    def forward(ctx, K, y, h):
+        p = _pfx(K)
+        n = K.shape[0]
+        Lc = K.detach().t().contiguous().t().clone() if n else K.detach().clone()
+        alpha = _colmajor(y.detach()).clone()
+        res = np.zeros(2, dtype=np.float64 if K.dtype == torch.float64 else np.float32)
+        _sync()
+        info = ctypes.c_int(0)
+        if n:
+            d, dp = _desc(h, n, n, n)
+            check_status(getattr(lib, f"ddla_{p}_ppotrf")(h.ptr, b"L", n, _ptr(Lc), 1, 1, dp, ctypes.byref(info)),
+                         f"ddla_{p}_ppotrf")
+        if info.value != 0:
+            raise RuntimeError(f"gp likelihood: the leading minor of order {info.value} is not positive definite")
+        if n:
+            db, dbp = _desc(h, n, 1, n)
+            check_status(getattr(lib, f"ddla_{p}_ppotrs")(h.ptr, b"L", b"L", b"N", n, 1, _ptr(Lc), dp, _ptr(alpha),
+                                                          dbp), f"ddla_{p}_ppotrs")
+            check_status(getattr(lib, f"ddla_{p}_diag_logdet")(h.ptr, n, _ptr(Lc), n,
+                                                               ctypes.c_void_p(res.ctypes.data)),
+                         f"ddla_{p}_diag_logdet")
+        a = alpha[:, 0]
+        ctx.h = h
+        ctx.save_for_backward(Lc, a)
+        return 0.5 * torch.dot(y.detach(), a) + float(res[0])  # log det K = 2 sum log L_ii
+
+    @staticmethod
+    def backward(ctx, g):
+        Lc, a = ctx.saved_tensors
+        Kbar = potri(Lc, ctx.h)
+        Kbar.addr_(a, a, alpha=-1.0)
+        Kbar.mul_(0.5 * g)
+        return Kbar, g * a, None
+
+
 def gp_negative_log_likelihood(K, y, h):
-    """0.5 y^T K^-1 y + 0.5 log det K (the constant n/2 log 2 pi omitted)."""
+    """0.5 y^T K^-1 y + 0.5 log det K (the constant n/2 log 2 pi omitted),
+    with the fused closed-form backward."""
+    return GPNegativeLogLikelihood.apply(K, y, h)
+
+
+def gp_negative_log_likelihood_composed(K, y, h):
+    """The same loss through the separate Cholesky, solve and log-determinant
+    Functions and their adjoints."""
     L = cholesky(K, h)
     alpha = cholesky_solve(y, L, h)
     return 0.5 * torch.dot(y, alpha) + 0.5 * cholesky_logdet(L, h)

@@ -104,7 +104,64 @@ void lower_solve(hipStream_t s, int n, const T* L, long long ldl, const T* dinv,
     lower_solve<T>(s, n1, L, ldl, dinv, C, ldc, sync, error);
 }
 
+// X = L^-1 (lower triangular, order n, in place on X, which holds the
+// identity on entry): with L = [L11 0; L21 L22] split at a multiple of 64,
+//   X11 = L11^-1,  X21 = -L22^-1 (L21 X11),  X22 = L22^-1,
+// X12 staying zero; leaves of order <= kLowerLeaf solve L X = I directly
+// (forward substitution keeps the zeros above the diagonal).  About n^3 / 2
+// flops.
+template <class T>
+void lower_inverse(hipStream_t s, int n, const T* L, long long ldl, const T* dinv, T* X, long long ldx, void* sync,
+                   int* error)
+{
+    if (n <= kLowerLeaf) {
+        kern::trsm<T>(s, false, false, n, n, L, ldl, dinv, X, ldx, sync, error);
+        return;
+    }
+    const int n1 = (n / 2 + kIB - 1) / kIB * kIB, n2 = n - n1;
+    const T* L21 = L + n1;
+    const T* L22 = L + n1 + (long long)n1 * ldl;
+    const T* dinv22 = dinv + (long long)(n1 / kIB) * kIB * kIB;
+    T* X21 = X + n1;
+    lower_inverse<T>(s, n1, L, ldl, dinv, X, ldx, sync, error);
+    kern::gemm<T>(s, 'N', 'N', n2, n1, n1, T(-1), L21, ldl, X, ldx, T(0), X21, ldx);  // -L21 X11
+    kern::trsm<T>(s, false, false, n2, n1, L22, ldl, dinv22, X21, ldx, sync, error);   // X21 = L22^-1 (-L21 X11)
+    lower_inverse<T>(s, n2, L22, ldl, dinv22, X21 + (long long)n1 * ldx, ldx, sync, error);
+}
+
 } // namespace
+
+template <class T>
+void potri(const DdlaHandle_t& h, char uplo, int n, const T* L, int ldl, T* Ainv, int lda)
+{
+    check_common(h, "potri", uplo, n, L, ldl);
+    if (n == 0) return;
+    if (!Ainv || lda < n) throw std::invalid_argument("potri: invalid Ainv");
+    const std::size_t nn = (std::size_t)n * n;
+    const std::size_t bytes = 2 * detail::padded_bytes<T>(nn) + detail::padded_bytes<T>(inv_elems(n)) +
+                              kern::solve_work_bytes(n) + detail::padded_bytes<int>(1);
+    char* w = static_cast<char*>(h->ensure_scratch(bytes));
+    T* X = detail::take<T>(w, nn);
+    T* M = detail::take<T>(w, nn);
+    T* dinv = detail::take<T>(w, inv_elems(n));
+    void* sync = w;
+    w += kern::solve_work_bytes(n);
+    int* error = detail::take<int>(w, 1);
+    hipStream_t s = h->stream;
+    fill_int(s, error, 0);
+    kern::trtri_diag_blocks<T>(s, false, n, L, ldl, dinv);
+    kern::set_identity<T>(s, n, X, n);
+    {
+        LaunchTimer timer(h, kClassSolve, (double)nn * n / 2.0, 0);
+        lower_inverse<T>(s, n, L, ldl, dinv, X, n, sync, error);  // X = L^-1
+    }
+    {
+        LaunchTimer timer(h, kClassOther, lower_product_flops(n));
+        lower_product<T>(s, n, X, X, M);  // M = tril(X^T X) = tril(A^-1)
+    }
+    kern::mirror_lower<T>(s, n, M, n, T(1), Ainv, lda);
+    finish(h, error, "potri");
+}
 
 template <class T>
 void potrf_backward(const DdlaHandle_t& h, char uplo, int n, const T* L, int ldl, const T* Lbar, int ldlb, T* Abar,
@@ -218,7 +275,8 @@ void logdet_backward(const DdlaHandle_t& h, int n, const T* L, int ldl, double g
     template void potrf_backward<T>(const DdlaHandle_t&, char, int, const T*, int, const T*, int, T*, int);    \
     template void potrs_backward<T>(const DdlaHandle_t&, char, int, int, const T*, int, const T*, int,         \
                                     const T*, int, T*, int, T*, int);                                           \
-    template void logdet_backward<T>(const DdlaHandle_t&, int, const T*, int, double, T*, int);
+    template void logdet_backward<T>(const DdlaHandle_t&, int, const T*, int, double, T*, int);                 \
+    template void potri<T>(const DdlaHandle_t&, char, int, const T*, int, T*, int);
 DDLA_ADJOINT_INSTANTIATE(float)
 DDLA_ADJOINT_INSTANTIATE(double)
 #undef DDLA_ADJOINT_INSTANTIATE

@@ -366,6 +366,13 @@ DDLA_C_DRIVERS(d, double)
             ddla::logdet_backward<T>(h, n, static_cast<const T*>(L), ldl, g, static_cast<T*>(Lbar), ldlb);     \
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_potri(void* handle, char uplo, int n, const void* L, int ldl, void* Ainv, int lda)          \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::potri<T>(h, uplo, n, static_cast<const T*>(L), ldl, static_cast<T*>(Ainv), lda);             \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
     }
 DDLA_C_ADJOINTS(s, float)
 DDLA_C_ADJOINTS(d, double)

@@ -146,6 +146,9 @@ template <class T>
 void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long long ldo);
 template <class T>
 void mirror_lower(hipStream_t s, int n, const T* in, long long ldi, T scale, T* out, long long ldo);
+// out = the order-n identity
+template <class T>
+void set_identity(hipStream_t s, int n, T* out, long long ldo);
 // Y = tril(L)^T W for W with c <= kTmvColumns columns (L lower storage, its
 // strict upper triangle never read): the few-column products of the solve
 // adjoint, one wavefront per row of Y.

@@ -753,6 +753,17 @@ void trsm(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L,
 }
 
 namespace {
+// out = the order-n identity.  One thread per element of out.
+template <class T>
+__global__ void identity_kernel(int n, T* out, long long ldo)
+{
+    const long long total = (long long)n * n;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
+        const int i = (int)(t % n), j = (int)(t / n);
+        out[i + j * ldo] = i == j ? T(1) : T(0);
+    }
+}
+
 // out = tril(in), strict upper triangle zero.  One thread per element of out.
 template <class T>
 __global__ void tril_copy_kernel(int n, const T* in, long long ldi, T* out, long long ldo)
@@ -834,6 +845,14 @@ void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long lo
     if (n <= 0) return;
     tril_copy_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, in, ldi, out, ldo);
     check_launch("tril_copy");
+}
+
+template <class T>
+void set_identity(hipStream_t s, int n, T* out, long long ldo)
+{
+    if (n <= 0) return;
+    identity_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, out, ldo);
+    check_launch("set_identity");
 }
 
 template <class T>
@@ -989,6 +1008,7 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
                           void*, int*);                                                                       \
     template void tril_copy<T>(hipStream_t, int, const T*, long long, T*, long long);                         \
     template void mirror_lower<T>(hipStream_t, int, const T*, long long, T, T*, long long);                   \
+    template void set_identity<T>(hipStream_t, int, T*, long long);                                           \
     template void lower_tmv<T>(hipStream_t, int, int, const T*, long long, const T*, long long, T*, long long); \
     template void diag_adjoint<T>(hipStream_t, int, const T*, long long, double, T*, long long);              \
     template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);

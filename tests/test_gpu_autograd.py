@@ -91,7 +91,29 @@ def gp_kernel(xt, l, s=1.0, jitter=1e-2):
     return s * s * torch.exp(-d2 / (2 * l * l)) + jitter * torch.eye(n, dtype=xt.dtype, device=xt.device), d2
 
 
-def test_gp_likelihood_gradient_config5(handle):
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [1, 64, 100, 257, 1100, 2100])
+def test_potri_matches_inverse(handle, n, dtype):
+    """A^-1 from the factor (recursive triangular inverse, then tril(X^T X)
+    mirrored) against numpy's inverse in FP64; exactly symmetric."""
+    print(f"case potri n={n} {dtype}", flush=True)
+    a = spd(n, 3 * n + 1)
+    L = torch.linalg.cholesky(torch.tensor(a)).to(dtype).to(DEV)
+    Ainv = ag.potri(L, handle).cpu().double().numpy()
+    ref = np.linalg.inv(a)
+    assert rel(Ainv, ref) < (1e-11 if dtype == torch.float64 else 1e-4)
+    assert np.array_equal(Ainv, Ainv.T)
+
+
+GP_LOSSES = {"fused": ag.gp_negative_log_likelihood, "composed": ag.gp_negative_log_likelihood_composed}
+
+
+@pytest.mark.parametrize("which", sorted(GP_LOSSES))
+def test_gp_likelihood_gradient_config5(handle, which):
+    """Both the fused Function (closed-form backward through potri) and the
+    composed Cholesky / solve / logdet Functions."""
+    print(f"case gp likelihood {which}", flush=True)
+    gp_loss = GP_LOSSES[which]
     n = 4096
     x, y = gp_problem(n)
     xt = torch.tensor(x, device=DEV)
@@ -99,7 +121,7 @@ def test_gp_likelihood_gradient_config5(handle):
     l = torch.tensor(0.2, dtype=torch.float64, device=DEV, requires_grad=True)
     K, d2 = gp_kernel(xt, l)
     K.retain_grad()
-    loss = ag.gp_negative_log_likelihood(K, yt, handle)
+    loss = gp_loss(K, yt, handle)
     loss.backward()
     kbar = K.grad.detach().cpu().numpy()
     dl = float(l.grad)
@@ -122,7 +144,7 @@ def test_gp_likelihood_gradient_config5(handle):
     def f_of_l(lv):
         with torch.no_grad():
             Kp, _ = gp_kernel(xt, torch.tensor(lv, dtype=torch.float64, device=DEV))
-            return float(ag.gp_negative_log_likelihood(Kp, yt, handle))
+            return float(gp_loss(Kp, yt, handle))
 
     fd_l = (f_of_l(0.2 + h) - f_of_l(0.2 - h)) / (2 * h)
     # The loss carries rounding noise of about cond(K) eps |y^T alpha| ~ 1e-8
@@ -140,11 +162,11 @@ def test_gp_likelihood_gradient_config5(handle):
             Kp[i, j] += h
             if i != j:
                 Kp[j, i] += h
-            fp = float(ag.gp_negative_log_likelihood(Kp, yt, handle))
+            fp = float(gp_loss(Kp, yt, handle))
             Kp[i, j] -= 2 * h
             if i != j:
                 Kp[j, i] -= 2 * h
-            fm = float(ag.gp_negative_log_likelihood(Kp, yt, handle))
+            fm = float(gp_loss(Kp, yt, handle))
         fd = (fp - fm) / (2 * h)
         want = kbar[i, j] * (1 if i == j else 2)
         assert abs(fd - want) <= 1e-7 * scale, (i, j, fd, want)
