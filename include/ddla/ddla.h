@@ -128,6 +128,18 @@ template <typename T>
 void pposv(const DdlaHandle_t& handle, char side, char uplo, char trans, int n, int nrhs, T* A, int ia, int ja,
            const int* descA, T* B, int ib, int jb, const int* descB, int& info);
 
+// Triangular solves: B := op(A)^-1 B (side 'L') or B op(A)^-1 (side 'R'),
+// A triangular (uplo), trans 'N' / 'T' / 'C', diag 'N' / 'U' (unit, the
+// stored diagonal not read); ptrtrs on descriptors at a 1 x 1 grid (upper-case
+// flags), trsm on leading dimensions with alpha (case-insensitive flags;
+// alpha == 0 sets B to zero without reading A).  Both return synchronized.
+template <typename T>
+void ptrtrs(const DdlaHandle_t& handle, char side, char uplo, char trans, char diag, int m, int n, T* A,
+            const int* descA, T* B, const int* descB);
+template <typename T>
+void trsm(const DdlaHandle_t& handle, char side, char uplo, char trans, char diag, int m, int n, const T& alpha,
+          T* A, int lda, T* B, int ldb);
+
 // Local factorization with a device-resident info word; asynchronous.
 template <typename T>
 void potrf(const DdlaHandle_t& handle, char uplo, int n, T* A, int lda, int* info);

@@ -67,6 +67,22 @@ int ddla_s_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs
 int ddla_d_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
                  const int* descA, void* B, int ib, int jb, const int* descB, int* info);            /* :1308 */
 
+/* ---- triangular solves: replace fortran/ddla_c_api.cpp:351-367 (ptrtrs)
+ * and :1451-1468 (trsm).  B := op(A)^-1 B (side 'L', A m x m) or B op(A)^-1
+ * (side 'R', A n x n); uplo 'L' / 'U', trans 'N' / 'T' / 'C', diag 'N' /
+ * 'U' (the stored diagonal is then not read).  ptrtrs: descriptors, 1 x 1
+ * grid, upper-case flags (include/ddla/ddla.h:96 of the reference); trsm:
+ * leading dimensions and *alpha, case-insensitive flags, alpha == 0 zeroes B
+ * without reading A (include/ddla/ddla.h:1620).  Both return synchronized. */
+int ddla_s_ptrtrs(void* handle, char side, char uplo, char trans, char diag, int m, int n, void* A,
+                  const int* descA, void* B, const int* descB);                                      /* :351 */
+int ddla_d_ptrtrs(void* handle, char side, char uplo, char trans, char diag, int m, int n, void* A,
+                  const int* descA, void* B, const int* descB);                                      /* :360 */
+int ddla_s_trsm(void* handle, char side, char uplo, char trans, char diag, int m, int n, const float* alpha,
+                void* A, int lda, void* B, int ldb);                                                 /* :1451 */
+int ddla_d_trsm(void* handle, char side, char uplo, char trans, char diag, int m, int n, const double* alpha,
+                void* A, int lda, void* B, int ldb);                                                 /* :1460 */
+
 /* ---- three-phase API (no C entry in the reference): the C++ components
  * cholesky_impl::{ppotrf,ppotrs,pposv}_{workspace_size,run} of
  * src/Cholesky/cholesky_workspace.h:133-249.  The size query allocates

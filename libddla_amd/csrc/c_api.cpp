@@ -337,6 +337,31 @@ DDLA_C_DRIVERS(s, float)
 DDLA_C_DRIVERS(d, double)
 #undef DDLA_C_DRIVERS
 
+// ---- triangular solves -----------------------------------------------------------
+#define DDLA_C_TRIANGULAR(P, T)                                                                                \
+    int ddla_##P##_ptrtrs(void* handle, char side, char uplo, char trans, char diag, int m, int n, void* A,   \
+                          const int* descA, void* B, const int* descB)                                         \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            ddla::ptrtrs<T>(h, side, uplo, trans, diag, m, n, static_cast<T*>(A), descA, static_cast<T*>(B),   \
+                            descB);                                                                            \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_trsm(void* handle, char side, char uplo, char trans, char diag, int m, int n,              \
+                        const T* alpha, void* A, int lda, void* B, int ldb)                                    \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (alpha == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                          \
+            ddla::trsm<T>(h, side, uplo, trans, diag, m, n, *alpha, static_cast<T*>(A), lda, static_cast<T*>(B), \
+                          ldb);                                                                                \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }
+DDLA_C_TRIANGULAR(s, float)
+DDLA_C_TRIANGULAR(d, double)
+#undef DDLA_C_TRIANGULAR
+
 // ---- adjoints ------------------------------------------------------------------
 #define DDLA_C_ADJOINTS(P, T)                                                                                  \
     int ddla_##P##_potrf_backward(void* handle, char uplo, int n, const void* L, int ldl, const void* Lbar,     \

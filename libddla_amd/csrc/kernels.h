@@ -77,9 +77,14 @@ void split_update(hipStream_t s, int products, bool upper, int m, int n2, int kb
                   int a_row, int b_row, float* C, long long ldc, bool tri, long long diag, const int* abort);
 
 // Inverts every 64 x 64 diagonal block of the order-n factor (L view as for
-// panel_factor) into inv[b * 4096 ...], identity-padded for a ragged last block.
+// panel_factor) into inv[b * 4096 ...], identity-padded for a ragged last
+// block; unit: the diagonal is taken as ones (the stored one is not used).
 template <class T>
-void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv);
+void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv, bool unit = false);
+// M := alpha M (m x n, leading dimension ld); alpha == 0 writes zeros
+// without reading M.
+template <class T>
+void scale_matrix(hipStream_t s, int m, int n, T alpha, T* M, long long ld);
 
 // Forward (L Y = B) then backward (L^T X = Y) substitution with the order-n
 // factor (L view as for panel_factor) and its inverted 64 x 64 diagonal blocks

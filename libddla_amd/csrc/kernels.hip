@@ -73,9 +73,10 @@ __global__ __launch_bounds__(256) void vbatched_validate_kernel(int na, int nb, 
 // ---------------------------------------------------------------------------
 constexpr int kIB = 64;
 
-// inv(block b) = L(b, b)^-1 for every 64 x 64 diagonal block of a factor.
+// inv(block b) = L(b, b)^-1 for every 64 x 64 diagonal block of a factor
+// (unit: the diagonal taken as ones, the stored one never used).
 template <class T>
-__global__ __launch_bounds__(256) void trtri_kernel(bool upper, int n, const T* A, long long lda, T* inv)
+__global__ __launch_bounds__(256) void trtri_kernel(bool upper, bool unit, int n, const T* A, long long lda, T* inv)
 {
     using namespace potf2_detail;
     __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
@@ -83,7 +84,10 @@ __global__ __launch_bounds__(256) void trtri_kernel(bool upper, int n, const T* 
     const int w = (int)min((long long)kIB, n - j0);
     load_block<T>(upper, w, A + j0 + j0 * lda, lda, S);
     __syncthreads();
-    if (threadIdx.x < kB) rdiag[threadIdx.x] = T(1) / S[threadIdx.x * kL + threadIdx.x];
+    if (threadIdx.x < kB) {
+        if (unit) S[threadIdx.x * kL + threadIdx.x] = T(1);
+        rdiag[threadIdx.x] = T(1) / S[threadIdx.x * kL + threadIdx.x];
+    }
     __syncthreads();
     invert_lower<T>(S, rdiag, X);
     store_inverse<T>(X, inv + (long long)blockIdx.x * kIB * kIB);
@@ -526,10 +530,10 @@ void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, 
 }
 
 template <class T>
-void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv)
+void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv, bool unit)
 {
     if (n <= 0) return;
-    trtri_kernel<T><<<(n + kIB - 1) / kIB, 256, 0, s>>>(upper, n, A, lda, inv);
+    trtri_kernel<T><<<(n + kIB - 1) / kIB, 256, 0, s>>>(upper, unit, n, A, lda, inv);
     check_launch("trtri_diag_blocks");
 }
 
@@ -753,6 +757,17 @@ void trsm(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L,
 }
 
 namespace {
+// M := alpha M (m x n); alpha == 0 writes zeros without reading M.
+template <class T>
+__global__ void scale_kernel(int m, int n, T alpha, T* M, long long ld)
+{
+    const long long total = (long long)m * n;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
+        const long long i = t % m, j = t / m;
+        M[i + j * ld] = alpha == T(0) ? T(0) : alpha * M[i + j * ld];
+    }
+}
+
 // out = the order-n identity.  One thread per element of out.
 template <class T>
 __global__ void identity_kernel(int n, T* out, long long ldo)
@@ -845,6 +860,14 @@ void tril_copy(hipStream_t s, int n, const T* in, long long ldi, T* out, long lo
     if (n <= 0) return;
     tril_copy_kernel<T><<<grid_for((long long)n * n, 256, 1 << 16), 256, 0, s>>>(n, in, ldi, out, ldo);
     check_launch("tril_copy");
+}
+
+template <class T>
+void scale_matrix(hipStream_t s, int m, int n, T alpha, T* M, long long ld)
+{
+    if (m <= 0 || n <= 0) return;
+    scale_kernel<T><<<grid_for((long long)m * n, 256, 1 << 16), 256, 0, s>>>(m, n, alpha, M, ld);
+    check_launch("scale_matrix");
 }
 
 template <class T>
@@ -997,7 +1020,8 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
                                    const int*, int*, double*, int, int, bool, int, int, void*);                \
     template void batched_copy<T>(hipStream_t, int, const int*, int, T* const*, const int*, T* const*,         \
                                   const int*, int);                                                            \
-    template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*);                      \
+    template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*, bool);                \
+    template void scale_matrix<T>(hipStream_t, int, int, T, T*, long long);                                   \
     template void diag_logdet<T>(hipStream_t, int, const T*, long long, void*, LogdetResult*);                \
     template void potrs_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, const T*, T*, long long,   \
                                   void*, int*);                                                               \

@@ -44,6 +44,25 @@ def pposv(A, B, uplo: str = "L", side: str = "L", trans: str = "N"):
     return B, info.value
 
 
+def ptrtrs(A, B, side: str = "L", uplo: str = "L", trans: str = "N", diag: str = "N"):
+    """B := op(A)^-1 B (side L) or B op(A)^-1 (side R) for a triangular A (the reference's ptrtrs); B overwritten."""
+    m, n = B.m, B.n
+    st = _fn(A, "ptrtrs")(A.handle.ptr, char(side), char(uplo), char(trans), char(diag), m, n,
+                          ctypes.c_void_p(A.data_ptr), A.desc_ptr, ctypes.c_void_p(B.data_ptr), B.desc_ptr)
+    check_status(st, f"ddla_{A.type_char}_ptrtrs")
+    return B
+
+
+def trsm(A, B, side: str = "L", uplo: str = "L", trans: str = "N", diag: str = "N", alpha: float = 1.0):
+    """B := alpha op(A)^-1 B (side L) or alpha B op(A)^-1 (side R) on the local matrices (the reference's trsm)."""
+    m, n = B.m, B.n
+    a = (ctypes.c_double if A.type_char == "d" else ctypes.c_float)(alpha)
+    st = _fn(A, "trsm")(A.handle.ptr, char(side), char(uplo), char(trans), char(diag), m, n, ctypes.byref(a),
+                        ctypes.c_void_p(A.data_ptr), A.lld, ctypes.c_void_p(B.data_ptr), B.lld)
+    check_status(st, f"ddla_{A.type_char}_trsm")
+    return B
+
+
 def ppotrf_workspace_size(A, uplo: str = "L"):
     bb, hb = ctypes.c_longlong(), ctypes.c_longlong()
     check_status(_fn(A, "ppotrf_workspace_size")(A.handle.ptr, char(uplo), A.n, A.desc_ptr, ctypes.byref(bb),
