@@ -146,6 +146,11 @@ void potrf(const DdlaHandle_t& handle, char uplo, int n, T* A, int lda, int* inf
 
 template <typename T>
 void diag_logdet(const DdlaHandle_t& h, int n, const T* A, int lda, std::complex<typename real_type<T>::type>& result);
+// pdiag_logdet (the reference's include/ddla/ddla.h:2914): diag_logdet of the
+// leading n x n block of a descriptor-described matrix, at a 1 x 1 grid.
+template <typename T>
+void pdiag_logdet(const DdlaHandle_t& h, int n, const T* A, const int* descA,
+                  std::complex<typename real_type<T>::type>& result);
 
 template <typename T>
 void gemmVbatched(const DdlaHandle_t& handle, char transA, char transB, int* d_m, int* d_n, int* d_k, T alpha,

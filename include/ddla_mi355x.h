@@ -144,6 +144,10 @@ int ddla_d_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
  * entry in the reference): result[0] = real part, result[1] = phase. ----- */
 int ddla_s_diag_logdet(void* handle, int n, const void* A, int lda, float* result);
 int ddla_d_diag_logdet(void* handle, int n, const void* A, int lda, double* result);
+/* pdiag_logdet (C++ ddla::pdiag_logdet, include/ddla/ddla.h:2914; no C entry
+ * in the reference): the same on the leading n x n block of a descriptor. */
+int ddla_s_pdiag_logdet(void* handle, int n, const void* A, const int* descA, float* result);
+int ddla_d_pdiag_logdet(void* handle, int n, const void* A, const int* descA, double* result);
 
 /* ---- gemmVbatched (C++ ddla::gemmVbatched, include/ddla/ddla.h:1836; the
  * reference binds no C entry, README.md:967-970).  All arrays are device

@@ -33,6 +33,7 @@ from .ops import (
     FACTOR_BF16X6,
     FACTOR_FP32,
     diag_logdet,
+    pdiag_logdet,
     ppotrf,
     ppotrf_run,
     ppotrf_workspace_size,
@@ -53,7 +54,7 @@ __all__ = [
     "ppotrf", "ppotrs", "pposv", "ptrtrs", "trsm", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
     "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
     "FACTOR_FP32", "FACTOR_BF16X3", "FACTOR_BF16X6",
-    "diag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",
+    "diag_logdet", "pdiag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",
     "pposv_vbatched_workspace_size", "vbatched_schedule", "set_vbatched_schedule", "DdlaError", "DdlaInvalidHandleError",
     "DdlaInvalidValueError", "DdlaAllocFailedError", "DdlaBackendUnavailableError", "DdlaInternalError",
     "DdlaExceptionError",

@@ -141,6 +141,7 @@ for _p in ("s", "d"):
         f"ddla_{_p}_pposv_vbatched_graph": (_i, [_vp, _c, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i,
                                                  ctypes.POINTER(_vp)]),
         f"ddla_{_p}_diag_logdet": (_i, [_vp, _i, _vp, _i, _vp]),
+        f"ddla_{_p}_pdiag_logdet": (_i, [_vp, _i, _vp, _ip, _vp]),
         f"ddla_{_p}_gemm_vbatched": (_i, [_vp, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
         f"ddla_{_p}_gemm_vbatched2s": (_i, [_vp, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                             _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _ip,

@@ -301,6 +301,17 @@ void* ddla_c_get_stream(void* handle)
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
     }                                                                                                          \
+    int ddla_##P##_pdiag_logdet(void* handle, int n, const void* A, const int* descA, T* result)               \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (result == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                         \
+            std::complex<T> z;                                                                                 \
+            ddla::pdiag_logdet<T>(h, n, static_cast<const T*>(A), descA, z);                                   \
+            result[0] = z.real();                                                                              \
+            result[1] = z.imag();                                                                              \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
     int ddla_##P##_gemm_vbatched(void* handle, char ta, char tb, int* d_m, int* d_n, int* d_k, const T* alpha, \
                                  const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb,       \
                                  const T* beta, void** d_C, int* d_ldc, int batch_count)                       \

@@ -485,6 +485,21 @@ void vbatched_maxima(const DdlaHandle_t& h, const char* who, char ta, char tb, c
 
 } // namespace
 
+
+// pdiag_logdet at a 1 x 1 grid (src/PBLAS/pdiag_logdet.cpp:11-48): the
+// descriptor checks, then diag_logdet on the local matrix with its LLD.
+template <class T>
+void pdiag_logdet(const DdlaHandle_t& h, int n, const T* a, const int* desc,
+                  std::complex<typename real_type<T>::type>& result)
+{
+    require_gpu(h, "pdiag_logdet");
+    check_desc(desc, h);
+    if (n < 0 || n > desc[DDLA_M_] || n > desc[DDLA_N_])
+        throw std::invalid_argument("pdiag_logdet: order exceeds descriptor");
+    if (n > 0 && !a) throw std::invalid_argument("pdiag_logdet: null local matrix");
+    diag_logdet<T>(h, n, a, desc[DDLA_LLD_], result);
+}
+
 template <class T>
 void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* d_m, int* d_n, int* d_k, T alpha,
                   const T* const* d_A, int* d_lda, const T* const* d_B, int* d_ldb, T beta, T** d_C, int* d_ldc,
@@ -575,6 +590,8 @@ void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long 
                            int, const int*, int&);                                                             \
     template void potrf<T>(const DdlaHandle_t&, char, int, T*, int, int*);                                     \
     template void diag_logdet<T>(const DdlaHandle_t&, int, const T*, int, std::complex<real_type<T>::type>&);  \
+    template void pdiag_logdet<T>(const DdlaHandle_t&, int, const T*, const int*,                             \
+                                  std::complex<real_type<T>::type>&);                                         \
     template void gemmVbatched<T>(const DdlaHandle_t&, char, char, int*, int*, int*, T, const T* const*, int*, \
                                   const T* const*, int*, T, T**, int*, int);                                   \
     template void gemmVbatched2s<T>(const DdlaHandle_t&, char, char, int*, int*, int*, T, const T* const*,     \

@@ -171,3 +171,13 @@ def diag_logdet(A, n: int = None):
                                ctypes.c_void_p(res.ctypes.data))
     check_status(st, f"ddla_{A.type_char}_diag_logdet")
     return complex(float(res[0]), float(res[1]))
+
+
+def pdiag_logdet(A, n: int = None):
+    """diag_logdet of the leading n x n block, through A's descriptor (the reference's pdiag_logdet)."""
+    n = A.m if n is None else n
+    res = np.zeros(2, dtype=A.dtype)
+    st = _fn(A, "pdiag_logdet")(A.handle.ptr, n, ctypes.c_void_p(A.data_ptr), A.desc_ptr,
+                                ctypes.c_void_p(res.ctypes.data))
+    check_status(st, f"ddla_{A.type_char}_pdiag_logdet")
+    return complex(float(res[0]), float(res[1]))

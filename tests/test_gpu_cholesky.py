@@ -300,15 +300,24 @@ def test_diag_logdet_cases(handle, d, dtype):
     for i, x in enumerate(d):
         full[i, i] = x
     M = dd.DistributedMatrix(handle, max(n, 1), max(n, 1), 4, 4, dtype, lld=lda).upload_raw(full)
-    got = dd.diag_logdet(M, n)
     want = orc.finish_logdet(*orc.diag_logdet(full.astype(np.float64), n))
-    if math.isinf(want.real):
-        assert math.isinf(got.real) and got.real < 0 and got.imag == 0
-    else:
-        tol = 3e-12 if dtype == np.float64 else 2e-5
-        assert abs(got.real - want.real) < tol * (1 + abs(want.real))
-        assert abs(math.remainder(got.imag - want.imag, 2 * math.pi)) < tol
+    # diag_logdet on the leading dimension, pdiag_logdet through the descriptor
+    for got in (dd.diag_logdet(M, n), dd.pdiag_logdet(M, n)):
+        if math.isinf(want.real):
+            assert math.isinf(got.real) and got.real < 0 and got.imag == 0
+        else:
+            tol = 3e-12 if dtype == np.float64 else 2e-5
+            assert abs(got.real - want.real) < tol * (1 + abs(want.real))
+            assert abs(math.remainder(got.imag - want.imag, 2 * math.pi)) < tol
     assert M.to_host().tobytes(order="F") == full.tobytes(order="F")
+
+
+def test_pdiag_logdet_descriptor_checks(handle):
+    """src/PBLAS/pdiag_logdet.cpp:17-29: an order beyond the descriptor is
+    rejected."""
+    M = upload(handle, np.diag([2.0, 3.0, 4.0]))
+    with pytest.raises(RuntimeError):
+        dd.pdiag_logdet(M, 4)
 
 
 @pytest.mark.parametrize("bad", [np.inf, np.nan])
