@@ -4,7 +4,9 @@
 // argument meaning and error behaviour), for a 1x1 process grid on one GPU:
 //   handle lifecycle        include/ddla/ddla_handle_t.h:51-67
 //   memory helpers          include/ddla/ddla_handle_t.h:130-284
-//   descriptor              include/ddla/ddla_desc.h:10-135
+//   descriptor, index maps  include/ddla/ddla_desc.h:10-135
+//   ddla::vector            include/ddla/ddla_vector.h:27-110 (ddla/ddla_vector.h,
+//                           included at the end of this header)
 //   ppotrf / ppotrs / pposv include/ddla/ddla.h:1065, 1149, 1184
 //   gemmVbatched(2s)        include/ddla/ddla.h:1836, 1857
 //   potrf                   include/ddla/ddla.h:2160
@@ -102,6 +104,18 @@ inline int num_loc(int n, int nb, int iproc, int srcproc, int nprocs)
     if (mydist < extra) count += nb;
     else if (mydist == extra) count += n % nb;
     return count;
+}
+
+// Global <-> local index maps of the block-cyclic distribution (0-based; the
+// ScaLAPACK tools INDXG2P, INDXG2L and INDXL2G; the reference's
+// include/ddla/ddla_desc.h:10-24).  g: global index, l: local index, nb:
+// block size, src: process holding the first block, p: process count.
+inline int indxg2p(int g, int nb, int src, int p) { return (src + g / nb) % p; }
+inline int indxg2l(int g, int nb, int p) { return (g / (nb * p)) * nb + g % nb; }
+inline int indxl2g(int l, int nb, int iproc, int src, int p)
+{
+    const int dist = (p + iproc - src) % p;  // position of iproc in the cycle
+    return ((l / nb) * p + dist) * nb + l % nb;
 }
 
 // DESCINIT semantics: lld is raised to max(1, LOCr(m)); lld < 0 derives it.
@@ -280,5 +294,7 @@ void pposv_vbatched_run(const DdlaHandle_t& h, char uplo, const int* d_n, T* con
 } // namespace cholesky_impl
 
 } // namespace ddla
+
+#include "ddla_vector.h"
 
 #endif
