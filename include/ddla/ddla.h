@@ -10,6 +10,7 @@
 //   ppotrf / ppotrs / pposv include/ddla/ddla.h:1065, 1149, 1184
 //   gemmVbatched(2s)        include/ddla/ddla.h:1836, 1857
 //   potrf                   include/ddla/ddla.h:2160
+//   potrf_bottom_right(p)   include/ddla/ddla.h:1090, 1115
 //   diag_logdet             include/ddla/ddla.h:2901
 //   check / size / run      src/Cholesky/cholesky_workspace.h:133-249
 // Differences: there is no MPI (ddlaSet takes the grid shape only and
@@ -158,6 +159,27 @@ void trsm(const DdlaHandle_t& handle, char side, char uplo, char trans, char dia
 template <typename T>
 void potrf(const DdlaHandle_t& handle, char uplo, int n, T* A, int lda, int* info);
 
+// Cholesky from the bottom-right corner (the reference's include/ddla/ddla.h:1090,
+// 1115): uplo 'U' gives A = U U^T in the upper triangle, 'L' gives A = L^T L in
+// the lower one; the other triangle is neither read nor written.  info = i > 0
+// when the trailing principal submatrix from row i (1-based) is not positive
+// definite (A then holds a partial result).  Synchronized.
+template <typename T>
+void potrf_bottom_right(char uplo, int n, T* A, int lda, int& info, const DdlaHandle_t& handle);
+template <typename T>
+void ppotrf_bottom_right(const DdlaHandle_t& handle, char uplo, int n, T* A, const int* descA, int& info);
+
+namespace lapack_impl {
+// potrf_bottom_right in three components (src/lapack_impl.h:137-155 of the reference).
+void potrf_bottom_right_check(char uplo, int n, const void* A, int lda, const DdlaHandle_t& handle);
+template <class T>
+void potrf_bottom_right_workspace_size(const DdlaHandle_t& handle, char uplo, int n, T* A,
+                                       std::size_t* backend_bytes);
+template <class T>
+void potrf_bottom_right_run(const DdlaHandle_t& handle, char uplo, int n, T* A, int lda, int& info,
+                            void* backend_work, std::size_t backend_bytes);
+} // namespace lapack_impl
+
 template <typename T>
 void diag_logdet(const DdlaHandle_t& h, int n, const T* A, int lda, std::complex<typename real_type<T>::type>& result);
 // pdiag_logdet (the reference's include/ddla/ddla.h:2914): diag_logdet of the
@@ -291,6 +313,13 @@ template <class T>
 void pposv_vbatched_run(const DdlaHandle_t& h, char uplo, const int* d_n, T* const* d_A, const int* d_lda, int nrhs,
                         T* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch, int max_n,
                         void* backend_work, std::size_t backend_bytes);
+void ppotrf_bottom_right_check(const DdlaHandle_t& h, char uplo, int n, const void* A, const int* descA);
+template <class T>
+void ppotrf_bottom_right_workspace_size(const DdlaHandle_t& h, char uplo, int n, T* A, const int* descA,
+                                        std::size_t* backend_bytes, std::size_t* host_bytes);
+template <class T>
+void ppotrf_bottom_right_run(const DdlaHandle_t& h, char uplo, int n, T* A, const int* descA, int& info,
+                             void* backend_work, std::size_t backend_bytes, void* host_work, std::size_t host_bytes);
 } // namespace cholesky_impl
 
 } // namespace ddla

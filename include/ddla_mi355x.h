@@ -140,6 +140,15 @@ int ddla_d_pposv_mixed_run(void* handle, char uplo, int n, int nrhs, void* A, co
 int ddla_s_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
 int ddla_d_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
 
+/* ---- Cholesky from the bottom-right corner, host info (C++
+ * ddla::potrf_bottom_right / ppotrf_bottom_right, include/ddla/ddla.h:1090,
+ * 1115; C entries fortran/ddla_c_api.cpp:1191-1258): uplo 'U' A = U U^T,
+ * 'L' A = L^T L; info = first row of the failing trailing submatrix. ---- */
+int ddla_s_potrf_bottom_right(void* handle, char uplo, int n, void* A, int lda, int* info); /* :1191 */
+int ddla_d_potrf_bottom_right(void* handle, char uplo, int n, void* A, int lda, int* info); /* :1200 */
+int ddla_s_ppotrf_bottom_right(void* handle, char uplo, int n, void* A, const int* descA, int* info); /* :1227 */
+int ddla_d_ppotrf_bottom_right(void* handle, char uplo, int n, void* A, const int* descA, int* info); /* :1236 */
+
 /* ---- diag_logdet (C++ ddla::diag_logdet, include/ddla/ddla.h:2901; no C
  * entry in the reference): result[0] = real part, result[1] = phase. ----- */
 int ddla_s_diag_logdet(void* handle, int n, const void* A, int lda, float* result);

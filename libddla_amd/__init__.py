@@ -34,7 +34,9 @@ from .ops import (
     FACTOR_FP32,
     diag_logdet,
     pdiag_logdet,
+    potrf_bottom_right,
     ppotrf,
+    ppotrf_bottom_right,
     ppotrf_run,
     ppotrf_workspace_size,
     pposv,
@@ -51,7 +53,7 @@ from .ops import (
 
 __all__ = [
     "LIBRARY_PATH", "lib", "check_status", "Backend", "Handle", "DistributedMatrix", "descinit",
-    "ppotrf", "ppotrs", "pposv", "ptrtrs", "trsm", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
+    "ppotrf", "ppotrs", "pposv", "ptrtrs", "trsm", "potrf_bottom_right", "ppotrf_bottom_right", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
     "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
     "FACTOR_FP32", "FACTOR_BF16X3", "FACTOR_BF16X6",
     "diag_logdet", "pdiag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",

@@ -128,6 +128,8 @@ for _p in ("s", "d"):
         f"ddla_{_p}_pposv_run": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _i, _i, _ip, _vp, _i, _i, _ip, _ip, _vp, _ll,
                                       _vp, _ll]),
         f"ddla_{_p}_potrf": (_i, [_vp, _c, _i, _vp, _i, _vp]),
+        f"ddla_{_p}_potrf_bottom_right": (_i, [_vp, _c, _i, _vp, _i, _ip]),
+        f"ddla_{_p}_ppotrf_bottom_right": (_i, [_vp, _c, _i, _vp, _ip, _ip]),
         f"ddla_{_p}_ptrtrs": (_i, [_vp, _c, _c, _c, _c, _i, _i, _vp, _ip, _vp, _ip]),
         f"ddla_{_p}_trsm": (_i, [_vp, _c, _c, _c, _c, _i, _i, _vp, _vp, _i, _vp, _i]),
         f"ddla_{_p}_potrf_backward": (_i, [_vp, _c, _i, _vp, _i, _vp, _i, _vp, _i]),

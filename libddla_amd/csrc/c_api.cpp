@@ -290,6 +290,22 @@ void* ddla_c_get_stream(void* handle)
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
     }                                                                                                          \
+    int ddla_##P##_potrf_bottom_right(void* handle, char uplo, int n, void* A, int lda, int* info)             \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (info == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                           \
+            ddla::potrf_bottom_right<T>(uplo, n, static_cast<T*>(A), lda, *info, h);                           \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_ppotrf_bottom_right(void* handle, char uplo, int n, void* A, const int* descA, int* info)   \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (info == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                           \
+            ddla::ppotrf_bottom_right<T>(h, uplo, n, static_cast<T*>(A), descA, *info);                        \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
     int ddla_##P##_diag_logdet(void* handle, int n, const void* A, int lda, T* result)                         \
     {                                                                                                          \
         return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \

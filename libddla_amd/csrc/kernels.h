@@ -200,5 +200,14 @@ void transpose(hipStream_t s, int m, int n, const T* in, long long ldi, T* out, 
 template <class D, class S>
 void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, long long ldo);
 
+// out(i, j) = in(n-1-j, n-1-i) over the stored triangle of order n (lower: i
+// >= j, upper: i <= j): J M J, J the order reversal, for the symmetric M read
+// through that triangle, written to the same triangle of out.  zero_other:
+// out's other triangle is set to zero, else it is not written; in's other
+// triangle is never read.  in and out must not overlap.
+template <class T>
+void reverse_triangle(hipStream_t s, bool upper, int n, const T* in, long long ldi, T* out, long long ldo,
+                      bool zero_other);
+
 } // namespace kern
 } // namespace ddla

@@ -305,6 +305,41 @@ __global__ __launch_bounds__(256) void transpose_kernel(int m, int n, const T* i
     }
 }
 
+// One 64 x 64 tile of out per workgroup (64 x 4 threads).  tile[b][a] holds
+// in(n - j0 - 64 + a, n - i0 - 64 + b), so out(i0 + di, j0 + dj) is
+// tile[63 - di][63 - dj]; reads run down columns of in, writes down columns of
+// out.
+template <class T>
+__global__ __launch_bounds__(256) void reverse_triangle_kernel(bool upper, int n, const T* in, long long ldi,
+                                                               T* out, long long ldo, bool zero_other)
+{
+    __shared__ T tile[64][65];
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+    const bool outside = upper ? (i0 > j0 + 63) : (i0 + 63 < j0);  // no element of the triangle
+    if (outside && !zero_other) return;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    if (!outside) {
+        const int r0 = n - j0 - 64, c0 = n - i0 - 64;
+        for (int b = ty; b < 64; b += 4) {
+            const int r = r0 + tx, c = c0 + b;  // r, c < n always
+            const bool stored = upper ? (r <= c) : (r >= c);
+            if (r >= 0 && c >= 0 && stored) tile[b][tx] = in[r + (long long)c * ldi];
+        }
+    }
+    __syncthreads();
+    const int i = i0 + tx;
+    if (i >= n) return;
+    for (int dj = ty; dj < 64; dj += 4) {
+        const int j = j0 + dj;
+        if (j >= n) break;
+        const bool stored = upper ? (i <= j) : (i >= j);
+        if (stored)
+            out[i + (long long)j * ldo] = tile[63 - tx][63 - dj];
+        else if (zero_other)
+            out[i + (long long)j * ldo] = T(0);
+    }
+}
+
 template <class D, class S>
 __global__ void convert_kernel(int m, int n, const S* in, long long ldi, D* out, long long ldo)
 {
@@ -989,6 +1024,18 @@ void add_diagonal(hipStream_t s, int n, T* A, long long lda, T delta)
 }
 
 template <class T>
+void reverse_triangle(hipStream_t s, bool upper, int n, const T* in, long long ldi, T* out, long long ldo,
+                      bool zero_other)
+{
+    if (n <= 0) return;
+    if (ldi < n || ldo < n) throw std::invalid_argument("reverse_triangle: leading dimension below the order");
+    const unsigned tiles = (unsigned)((n + 63) / 64);
+    if (tiles > 65535) throw std::invalid_argument("reverse_triangle: order too large");
+    reverse_triangle_kernel<T><<<dim3(tiles, tiles), 256, 0, s>>>(upper, n, in, ldi, out, ldo, zero_other);
+    check_launch("reverse_triangle");
+}
+
+template <class T>
 void transpose(hipStream_t s, int m, int n, const T* in, long long ldi, T* out, long long ldo)
 {
     if (m <= 0 || n <= 0) return;
@@ -1035,7 +1082,8 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void set_identity<T>(hipStream_t, int, T*, long long);                                           \
     template void lower_tmv<T>(hipStream_t, int, int, const T*, long long, const T*, long long, T*, long long); \
     template void diag_adjoint<T>(hipStream_t, int, const T*, long long, double, T*, long long);              \
-    template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);
+    template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);                     \
+    template void reverse_triangle<T>(hipStream_t, bool, int, const T*, long long, T*, long long, bool);
 DDLA_KERN_INSTANTIATE(double)
 DDLA_KERN_INSTANTIATE(float)
 #undef DDLA_KERN_INSTANTIATE

@@ -23,6 +23,28 @@ def ppotrf(A, uplo: str = "L"):
     return A, info.value
 
 
+def ppotrf_bottom_right(A, uplo: str = "U"):
+    """Cholesky from the bottom-right corner in place (the reference's
+    ppotrf_bottom_right): uplo U gives A = U U^T, uplo L gives A = L^T L;
+    returns (A, info), info = first row (1-based) of the trailing submatrix
+    that is not positive definite."""
+    info = ctypes.c_int(0)
+    st = _fn(A, "ppotrf_bottom_right")(A.handle.ptr, char(uplo), A.n, ctypes.c_void_p(A.data_ptr), A.desc_ptr,
+                                       ctypes.byref(info))
+    check_status(st, f"ddla_{A.type_char}_ppotrf_bottom_right")
+    return A, info.value
+
+
+def potrf_bottom_right(A, uplo: str = "U"):
+    """The local routine (the reference's potrf_bottom_right) on A's local
+    matrix and leading dimension; returns (A, info) as ppotrf_bottom_right."""
+    info = ctypes.c_int(0)
+    st = _fn(A, "potrf_bottom_right")(A.handle.ptr, char(uplo), A.n, ctypes.c_void_p(A.data_ptr), A.lld,
+                                      ctypes.byref(info))
+    check_status(st, f"ddla_{A.type_char}_potrf_bottom_right")
+    return A, info.value
+
+
 def ppotrs(A, B, uplo: str = "L", side: str = "L", trans: str = "N"):
     """Solve with the factor from ppotrf: A X = B (side L) or X A = B (side R), B overwritten."""
     n = A.n

@@ -102,6 +102,50 @@ DEFINE_POTRF(oracle_dpotrf, double, sqrt)
 DEFINE_POTRF(oracle_spotrf, float, sqrtf)
 
 /* ------------------------------------------------------------------------ */
+/* potrf_bottom_right: the Cholesky recurrence run from the last column to   */
+/* the first (src/Cholesky/potrf_bottom_right.cpp:221-273 does it in blocks  */
+/* of 128 on reversed copies).  uplo 'U': A = U U^T,                         */
+/*   U(j,j)^2 = A(j,j) - sum_{k>j} U(j,k)^2,                                 */
+/*   U(i,j)   = (A(i,j) - sum_{k>j} U(i,k) U(j,k)) / U(j,j),  i < j;         */
+/* uplo 'L': A = L^T L,                                                      */
+/*   L(j,j)^2 = A(j,j) - sum_{k>j} L(k,j)^2,                                 */
+/*   L(j,i)   = (A(j,i) - sum_{k>j} L(k,j) L(k,i)) / L(j,j),  i < j.         */
+/* info = j+1 for the first non-positive (or NaN) pivot met, j descending.   */
+/* ------------------------------------------------------------------------ */
+#define DEFINE_POTRF_BR(NAME, T, SQRT)                                                  \
+void NAME(char uplo, int n, T* A, int lda, int* info)                                   \
+{                                                                                       \
+    *info = 0;                                                                          \
+    const int lower = (uplo == 'L' || uplo == 'l');                                     \
+    for (int j = n - 1; j >= 0; --j) {                                                  \
+        T ajj = A[j + (size_t)j * lda];                                                 \
+        for (int k = j + 1; k < n; ++k) {                                               \
+            const T v = lower ? A[k + (size_t)j * lda] : A[j + (size_t)k * lda];        \
+            ajj -= v * v;                                                               \
+        }                                                                               \
+        if (!(ajj > 0)) { A[j + (size_t)j * lda] = ajj; *info = j + 1; return; }        \
+        ajj = SQRT(ajj);                                                                \
+        A[j + (size_t)j * lda] = ajj;                                                   \
+        for (int i = 0; i < j; ++i) {                                                   \
+            T s;                                                                        \
+            if (lower) {                                                                \
+                s = A[j + (size_t)i * lda];                                             \
+                for (int k = j + 1; k < n; ++k)                                         \
+                    s -= A[k + (size_t)j * lda] * A[k + (size_t)i * lda];               \
+                A[j + (size_t)i * lda] = s / ajj;                                       \
+            } else {                                                                    \
+                s = A[i + (size_t)j * lda];                                             \
+                for (int k = j + 1; k < n; ++k)                                         \
+                    s -= A[i + (size_t)k * lda] * A[j + (size_t)k * lda];               \
+                A[i + (size_t)j * lda] = s / ajj;                                       \
+            }                                                                           \
+        }                                                                               \
+    }                                                                                   \
+}
+DEFINE_POTRF_BR(oracle_dpotrf_bottom_right, double, sqrt)
+DEFINE_POTRF_BR(oracle_spotrf_bottom_right, float, sqrtf)
+
+/* ------------------------------------------------------------------------ */
 /* ?potrs: two triangular solves (src/Cholesky/ppotrs.cpp:44-61).  For side  */
 /* 'R' the system X A = B is A X^T = B^T (A is symmetric), solved row by row.*/
 /* ------------------------------------------------------------------------ */

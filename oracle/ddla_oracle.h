@@ -18,6 +18,11 @@
  *                      n=1 sqrt (tests/test_api_grid_ppotrf.cpp:17-30),
  *                      info = n for a final -1 pivot
  *                      (python/tests/test_cholesky.py:79-92), reconstruction
+ *   potrf_bottom_right the analytic factor of
+ *                      tests/test_potrf_bottom_right.cpp:96-147 (U U^T /
+ *                      L^T L formed from a known factor), scipy's Cholesky
+ *                      of the order-reversed matrix, the failing row of a
+ *                      planted negative pivot, reconstruction
  *   potrs              scipy LAPACK dpotrs fixtures, closed-form X_true
  *                      (tests/api_grid_test_common.h:490-512)
  *   diag_logdet        long-double sum of logs, the special cases of
@@ -50,6 +55,13 @@ void oracle_fill_normal(uint64_t seed, uint64_t stream, uint64_t first, int64_t 
  * written; info = j+1 for the first non-positive (or NaN) pivot. */
 void oracle_dpotrf(char uplo, int n, double* A, int lda, int* info);
 void oracle_spotrf(char uplo, int n, float* A, int lda, int* info);
+
+/* potrf_bottom_right (include/ddla/ddla.h:1072-1093, src/Cholesky/
+ * potrf_bottom_right.cpp:221-273): A = U U^T (uplo 'U') or L^T L ('L'),
+ * columns from the last to the first; only the uplo triangle is read and
+ * written; info = j+1 for the first non-positive pivot, j descending. */
+void oracle_dpotrf_bottom_right(char uplo, int n, double* A, int lda, int* info);
+void oracle_spotrf_bottom_right(char uplo, int n, float* A, int lda, int* info);
 
 /* ppotrs at 1x1 (src/Cholesky/ppotrs.cpp:31-62 -> ptrtrs.cpp:150-155 ->
  * BLAS ?trsm twice).  side 'L': B (n x nrhs) := A^-1 B; side 'R':

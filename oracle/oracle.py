@@ -29,6 +29,7 @@ def _load():
     lib.oracle_fill_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64, dp]
     for p in ("d", "s"):
         getattr(lib, f"oracle_{p}potrf").argtypes = [ctypes.c_char, ctypes.c_int, dp, ctypes.c_int, ip]
+        getattr(lib, f"oracle_{p}potrf_bottom_right").argtypes = [ctypes.c_char, ctypes.c_int, dp, ctypes.c_int, ip]
         getattr(lib, f"oracle_{p}potrs").argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, dp,
                                                      ctypes.c_int, dp, ctypes.c_int]
         getattr(lib, f"oracle_{p}diag_logdet").argtypes = [ctypes.c_int, dp, ctypes.c_int,
@@ -82,6 +83,17 @@ def potrf(uplo, a, lda=None):
     n = f.shape[1]
     info = ctypes.c_int(0)
     getattr(lib, f"oracle_{_pfx(f)}potrf")(uplo.encode(), n, f.ctypes.data, f.shape[0], ctypes.byref(info))
+    return f, info.value
+
+
+def potrf_bottom_right(uplo, a):
+    """Bottom-right factorization of a copy of a (A = U U^T for uplo U, L^T L
+    for L); returns (factor, info)."""
+    f = np.array(a, dtype=a.dtype, order="F", copy=True)
+    n = f.shape[1]
+    info = ctypes.c_int(0)
+    getattr(lib, f"oracle_{_pfx(f)}potrf_bottom_right")(uplo.encode(), n, f.ctypes.data, f.shape[0],
+                                                         ctypes.byref(info))
     return f, info.value
 
 

@@ -251,3 +251,68 @@ def test_gemm_special_cases():
     b = np.ones((2, 4))
     c = np.full((3, 4), np.nan)
     assert (orc.gemm("N", "N", 3, 4, 2, 1.0, a, b, 0.0, c) == 2.0).all()  # beta 0: C unread
+
+
+def bottom_right_factor(uplo, n):
+    """The factor of tests/test_potrf_bottom_right.cpp:96-118 (real part):
+    diagonal 1.5 + 0.002 (i % 23), off-diagonal 0.004 (((3i + 5j) % 11) - 5),
+    on the uplo triangle."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    f = 0.004 * (((3 * i + 5 * j) % 11) - 5)
+    f = np.where(i == j, 1.5 + 0.002 * (i % 23), f)
+    return np.triu(f) if uplo == "U" else np.tril(f)
+
+
+def bottom_right_product(uplo, f):
+    """A = U U^T (uplo U) or L^T L (uplo L)."""
+    return f @ f.T if uplo == "U" else f.T @ f
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [1, 2, 70, 129])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_potrf_bottom_right_recovers_known_factor(uplo, n, dtype):
+    """The reference test's acceptance: factor error <= 128 eps n, other
+    triangle untouched (here: a sentinel)."""
+    f = bottom_right_factor(uplo, n)
+    a = bottom_right_product(uplo, f).astype(dtype)
+    other = np.tril(np.ones((n, n), bool), -1) if uplo == "U" else np.triu(np.ones((n, n), bool), 1)
+    a[other] = -73.0
+    got, info = orc.potrf_bottom_right(uplo, a)
+    assert info == 0
+    keep = ~other
+    assert np.abs(got[keep] - f[keep]).max() <= 128 * np.finfo(dtype).eps * n
+    assert (got[other] == -73.0).all()
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_potrf_bottom_right_is_cholesky_of_the_reversed_matrix(uplo):
+    """With J the order reversal, J A J = V^T V (scipy, upper) gives U = J V^T J,
+    and J A J = W W^T (lower) gives L = J W^T J."""
+    import scipy.linalg
+
+    a = orc.spd_matrix(97, seed=5, delta=0.1)
+    rev = a[::-1, ::-1]
+    if uplo == "U":
+        want = scipy.linalg.cholesky(rev, lower=False).T[::-1, ::-1]
+    else:
+        want = scipy.linalg.cholesky(rev, lower=True).T[::-1, ::-1]
+    got, info = orc.potrf_bottom_right(uplo, a)
+    assert info == 0
+    tri = np.triu if uplo == "U" else np.tril
+    assert np.abs(tri(got) - want).max() < 1e-13 * np.abs(want).max()
+    assert np.allclose(bottom_right_product(uplo, tri(got)), a, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_potrf_bottom_right_reports_the_highest_failing_row(uplo):
+    """Two planted negative pivots (rows 3 and 11 of 16): the bottom-right
+    order meets row 11 first, so info = 12 (the forward potrf reports 4)."""
+    a = np.eye(16) * 4.0
+    a[3, 3] = a[11, 11] = -1.0
+    _, info = orc.potrf_bottom_right(uplo, a)
+    assert info == 12
+    _, finfo = orc.potrf(uplo, a)
+    assert finfo == 4
+    g, info = orc.potrf_bottom_right(uplo, np.array([[9.0]]))
+    assert info == 0 and g[0, 0] == 3.0
