@@ -130,7 +130,10 @@ template <class T> struct real_type { using type = T; };
 template <class R> struct real_type<std::complex<R>> { using type = R; };
 
 // ---------------------------------------------------------------------------
-// Cholesky path.  float and double are instantiated.
+// Cholesky path.  float and double are instantiated.  The factorizations
+// (ppotrf, pposv, potrf) return the strict triangle opposite uplo as they found
+// it (the reference's docs/hip-dtk.md:3-7 leaves it unspecified); with uplo 'U'
+// it is working storage during the call.
 // ---------------------------------------------------------------------------
 template <typename T>
 void ppotrf(const DdlaHandle_t& handle, char uplo, int n, T* A, int ia, int ja, const int* descA, int& info);

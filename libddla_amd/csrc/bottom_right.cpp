@@ -12,15 +12,18 @@
 //                        ppotrf_bottom_right.cpp:61-85): descriptor checks throw
 //                        std::invalid_argument; at 1 x 1 the local routine on
 //                        the descriptor's LLD.
-// The arithmetic is this project's.  With J the order reversal, J A J = V^T V
-// (uplo 'U') or W W^T (uplo 'L') is an ordinary Cholesky factorization of the
-// same triangle, and U = J V^T J, L = J W^T J; element-wise both are the
-// anti-transpose M(i, j) -> M(n-1-j, n-1-i), which maps each triangle onto
-// itself.  So the stored triangle is anti-transposed into a scratch copy, the
-// blocked factorization (MFMA trailing updates, look-ahead) runs on the copy,
-// and the factor is anti-transposed back: two passes over the triangle in
-// HBM beside the n^3/3 flops, instead of the reference's sequence of 128-wide
-// blocks.  A failed pivot k of the copy is pivot n - k + 1 of A.
+// The arithmetic is this project's.  With J the order reversal, S = J A J has
+// the ordinary factorization S = W W^T (W lower), and
+//   uplo 'U': U = J W J, i.e. U(i, j) = W(n-1-i, n-1-j), and S's lower triangle
+//             is A's upper one rotated the same way;
+//   uplo 'L': L = J W^T J, i.e. L(i, j) = W(n-1-j, n-1-i), and S's lower
+//             triangle is A's lower one under the same anti-transpose.
+// So A's stored triangle is reversed into the lower triangle of a scratch copy
+// (its upper triangle zeroed), the blocked lower factorization (MFMA trailing
+// updates, look-ahead) runs on the copy, and the factor is reversed back into
+// A's triangle: two passes over the triangle in HBM beside the n^3/3 flops,
+// instead of the reference's sequence of 128-wide blocks.  A failed pivot k of
+// the copy is pivot n - k + 1 of A.
 #include "internal.h"
 #include "kernels.h"
 
@@ -83,12 +86,13 @@ void potrf_bottom_right_run(const DdlaHandle_t& h, char uplo, int n, T* A, int l
     fill_int(s, d_info, 0);
     {
         LaunchTimer timer(h, kClassOther, 0, ((double)n * (n + 1) / 2 + (double)n * n) * sizeof(T));
-        kern::reverse_triangle<T>(s, upper, n, A, lda, rev, ldw, true);
+        // uplo 'U' rotates, uplo 'L' anti-transposes, into the copy's lower triangle
+        kern::reverse_triangle<T>(s, false, !upper, n, A, lda, rev, ldw, true);
     }
-    potrf_blocked_dispatch<T>(h, upper, n, rev, ldw, d_info, panel);
+    potrf_blocked_dispatch<T>(h, false, n, rev, ldw, d_info, panel);
     {
         LaunchTimer timer(h, kClassOther, 0, (double)n * (n + 1) * sizeof(T));
-        kern::reverse_triangle<T>(s, upper, n, rev, ldw, A, lda, false);
+        kern::reverse_triangle<T>(s, upper, !upper, n, rev, ldw, A, lda, false);
     }
     int rinfo = 0, error = 0;
     DDLA_HIP(hipMemcpyAsync(&rinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -132,6 +132,24 @@ template <class T>
 void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work,
                    const SplitPlan* split = nullptr)
 {
+    if (upper) {
+        // In the upper-storage view every trailing update reads its operands
+        // along k (transposed products), which the matrix-core kernels run at
+        // about half the rate of the lower view.  So the two strict triangles
+        // are exchanged in place, the lower one (now A's upper triangle
+        // transposed) is factored, and the exchange is undone: the factor
+        // lands in the upper triangle and the caller's strict lower triangle
+        // is restored bit for bit (two HBM passes over the matrix).
+        const double bytes = 2.0 * n * (n - 1) * sizeof(T);
+        {
+            LaunchTimer timer(h, kClassOther, 0, bytes);
+            kern::swap_triangles<T>(h->stream, n, A, lda);
+        }
+        potrf_blocked<T>(h, false, n, A, lda, info, work, split);
+        LaunchTimer timer(h, kClassOther, 0, bytes);
+        kern::swap_triangles<T>(h->stream, n, A, lda);
+        return;
+    }
     const LView<T> v{A, lda, upper};
     const int NB = outer_block<T>(n);
     const int np = (n + NB - 1) / NB;

@@ -200,14 +200,19 @@ void transpose(hipStream_t s, int m, int n, const T* in, long long ldi, T* out, 
 template <class D, class S>
 void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, long long ldo);
 
-// out(i, j) = in(n-1-j, n-1-i) over the stored triangle of order n (lower: i
-// >= j, upper: i <= j): J M J, J the order reversal, for the symmetric M read
-// through that triangle, written to the same triangle of out.  zero_other:
-// out's other triangle is set to zero, else it is not written; in's other
-// triangle is never read.  in and out must not overlap.
+// Order reversal of a symmetric matrix given by one triangle (order n, J the
+// order reversal): out's triangle out_upper receives J M J,
+//   transpose: out(i, j) = in(n-1-j, n-1-i), in's stored triangle = out's;
+//   rotate:    out(i, j) = in(n-1-i, n-1-j), in's stored triangle = the other.
+// zero_other: out's other triangle is set to zero, else it is not written;
+// in's unstored triangle is never read.  in and out must not overlap.
 template <class T>
-void reverse_triangle(hipStream_t s, bool upper, int n, const T* in, long long ldi, T* out, long long ldo,
-                      bool zero_other);
+void reverse_triangle(hipStream_t s, bool out_upper, bool transpose, int n, const T* in, long long ldi, T* out,
+                      long long ldo, bool zero_other);
+// In place: exchanges the strict lower and strict upper triangles of the
+// order-n A (A(i, j) <-> A(j, i), i > j); applied twice it is the identity.
+template <class T>
+void swap_triangles(hipStream_t s, int n, T* A, long long lda);
 
 } // namespace kern
 } // namespace ddla

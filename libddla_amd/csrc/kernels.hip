@@ -305,24 +305,28 @@ __global__ __launch_bounds__(256) void transpose_kernel(int m, int n, const T* i
     }
 }
 
-// One 64 x 64 tile of out per workgroup (64 x 4 threads).  tile[b][a] holds
-// in(n - j0 - 64 + a, n - i0 - 64 + b), so out(i0 + di, j0 + dj) is
-// tile[63 - di][63 - dj]; reads run down columns of in, writes down columns of
-// out.
+// One 64 x 64 tile of out per workgroup (64 x 4 threads), staged through LDS:
+// tile[b][a] = in(r0 + a, c0 + b) for the source block (r0, c0) of the tile,
+// loaded down columns of in and stored down columns of out.
+//   transpose (out(i, j) = in(n-1-j, n-1-i)): r0 = n - j0 - 64, c0 = n - i0 - 64,
+//     out(i0 + di, j0 + dj) = tile[63 - di][63 - dj]; in's triangle = out's;
+//   rotate (out(i, j) = in(n-1-i, n-1-j)):    r0 = n - i0 - 64, c0 = n - j0 - 64,
+//     out(i0 + di, j0 + dj) = tile[63 - dj][63 - di]; in's triangle = the other.
 template <class T>
-__global__ __launch_bounds__(256) void reverse_triangle_kernel(bool upper, int n, const T* in, long long ldi,
-                                                               T* out, long long ldo, bool zero_other)
+__global__ __launch_bounds__(256) void reverse_triangle_kernel(bool out_upper, bool transpose, int n, const T* in,
+                                                               long long ldi, T* out, long long ldo, bool zero_other)
 {
     __shared__ T tile[64][65];
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
-    const bool outside = upper ? (i0 > j0 + 63) : (i0 + 63 < j0);  // no element of the triangle
+    const bool outside = out_upper ? (i0 > j0 + 63) : (i0 + 63 < j0);  // no element of out's triangle
     if (outside && !zero_other) return;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     if (!outside) {
-        const int r0 = n - j0 - 64, c0 = n - i0 - 64;
+        const bool in_upper = transpose ? out_upper : !out_upper;
+        const int r0 = n - (transpose ? j0 : i0) - 64, c0 = n - (transpose ? i0 : j0) - 64;
         for (int b = ty; b < 64; b += 4) {
             const int r = r0 + tx, c = c0 + b;  // r, c < n always
-            const bool stored = upper ? (r <= c) : (r >= c);
+            const bool stored = in_upper ? (r <= c) : (r >= c);
             if (r >= 0 && c >= 0 && stored) tile[b][tx] = in[r + (long long)c * ldi];
         }
     }
@@ -332,11 +336,39 @@ __global__ __launch_bounds__(256) void reverse_triangle_kernel(bool upper, int n
     for (int dj = ty; dj < 64; dj += 4) {
         const int j = j0 + dj;
         if (j >= n) break;
-        const bool stored = upper ? (i <= j) : (i >= j);
+        const bool stored = out_upper ? (i <= j) : (i >= j);
         if (stored)
-            out[i + (long long)j * ldo] = tile[63 - tx][63 - dj];
+            out[i + (long long)j * ldo] = transpose ? tile[63 - tx][63 - dj] : tile[63 - dj][63 - tx];
         else if (zero_other)
             out[i + (long long)j * ldo] = T(0);
+    }
+}
+
+// In-place exchange of the strict lower and strict upper triangles of the
+// order-n A: A(i, j) <-> A(j, i) for i > j.  The workgroup of the 32 x 32 tile
+// pair (bi, bj), bi >= bj, loads the strict-lower part of tile (bi, bj) and
+// the strict-upper part of tile (bj, bi) into LDS and writes each into the
+// other's place: every element is read and written by one workgroup only.
+template <class T>
+__global__ __launch_bounds__(256) void swap_triangles_kernel(int n, T* A, long long lda)
+{
+    __shared__ T lo[32][33], up[32][33];  // lo[c][r] = A(i0 + r, j0 + c), up[c][r] = A(j0 + r, i0 + c)
+    const int bi = blockIdx.x, bj = blockIdx.y;
+    if (bi < bj) return;
+    const int i0 = bi * 32, j0 = bj * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int c = ty; c < 32; c += 8) {
+        const int li = i0 + tx, lj = j0 + c;  // lower element
+        if (li < n && lj < n && li > lj) lo[c][tx] = A[li + (long long)lj * lda];
+        const int ui = j0 + tx, uj = i0 + c;  // upper element
+        if (ui < n && uj < n && ui < uj) up[c][tx] = A[ui + (long long)uj * lda];
+    }
+    __syncthreads();
+    for (int c = ty; c < 32; c += 8) {
+        const int li = i0 + tx, lj = j0 + c;  // A(li, lj) <- A(lj, li) = up[tx][c]
+        if (li < n && lj < n && li > lj) A[li + (long long)lj * lda] = up[tx][c];
+        const int ui = j0 + tx, uj = i0 + c;  // A(ui, uj) <- A(uj, ui) = lo[tx][c]
+        if (ui < n && uj < n && ui < uj) A[ui + (long long)uj * lda] = lo[tx][c];
     }
 }
 
@@ -1024,15 +1056,27 @@ void add_diagonal(hipStream_t s, int n, T* A, long long lda, T delta)
 }
 
 template <class T>
-void reverse_triangle(hipStream_t s, bool upper, int n, const T* in, long long ldi, T* out, long long ldo,
-                      bool zero_other)
+void reverse_triangle(hipStream_t s, bool out_upper, bool transpose, int n, const T* in, long long ldi, T* out,
+                      long long ldo, bool zero_other)
 {
     if (n <= 0) return;
     if (ldi < n || ldo < n) throw std::invalid_argument("reverse_triangle: leading dimension below the order");
     const unsigned tiles = (unsigned)((n + 63) / 64);
     if (tiles > 65535) throw std::invalid_argument("reverse_triangle: order too large");
-    reverse_triangle_kernel<T><<<dim3(tiles, tiles), 256, 0, s>>>(upper, n, in, ldi, out, ldo, zero_other);
+    reverse_triangle_kernel<T><<<dim3(tiles, tiles), 256, 0, s>>>(out_upper, transpose, n, in, ldi, out, ldo,
+                                                                  zero_other);
     check_launch("reverse_triangle");
+}
+
+template <class T>
+void swap_triangles(hipStream_t s, int n, T* A, long long lda)
+{
+    if (n <= 1) return;
+    if (lda < n) throw std::invalid_argument("swap_triangles: leading dimension below the order");
+    const unsigned tiles = (unsigned)((n + 31) / 32);
+    if (tiles > 65535) throw std::invalid_argument("swap_triangles: order too large");
+    swap_triangles_kernel<T><<<dim3(tiles, tiles), 256, 0, s>>>(n, A, lda);
+    check_launch("swap_triangles");
 }
 
 template <class T>
@@ -1083,7 +1127,8 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void lower_tmv<T>(hipStream_t, int, int, const T*, long long, const T*, long long, T*, long long); \
     template void diag_adjoint<T>(hipStream_t, int, const T*, long long, double, T*, long long);              \
     template void transpose<T>(hipStream_t, int, int, const T*, long long, T*, long long);                     \
-    template void reverse_triangle<T>(hipStream_t, bool, int, const T*, long long, T*, long long, bool);
+    template void reverse_triangle<T>(hipStream_t, bool, bool, int, const T*, long long, T*, long long, bool); \
+    template void swap_triangles<T>(hipStream_t, int, T*, long long);
 DDLA_KERN_INSTANTIATE(double)
 DDLA_KERN_INSTANTIATE(float)
 #undef DDLA_KERN_INSTANTIATE
