@@ -94,6 +94,37 @@ def test_split_factor_modes_three_panels(handle, uplo):
     assert its[dd.FACTOR_BF16X3] >= its[dd.FACTOR_BF16X6], its
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("factor", FACTORS)
+def test_full_order(handle, factor, uplo):
+    """n = 16384 (32 panels, trailing updates over 15872 rows): A = G G^T / n
+    + 4e-4 I generated on the GPU (seed 2), four columns of B; the backward
+    error, in FP64 on the GPU (the residual's own rounding is ~1e-17 against
+    the 1e-14 target), meets the tolerance, and A is left untouched."""
+    import torch
+
+    print(f"case mixed full order n=16384 factor={factor} uplo={uplo}", flush=True)
+    n, nrhs, tol = 16384, 4, 1e-14
+    a0 = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    g = torch.empty_like(a0)
+    dd.check_status(dd.lib.ddla_d_spd_generate(handle.ptr, n, 2, 4e-4, ctypes.c_void_p(a0.data_ptr()), n,
+                                               ctypes.c_void_p(g.data_ptr())))
+    handle.synchronize()
+    del g
+    gen = torch.Generator(device="cuda").manual_seed(2)
+    b0 = torch.randn((nrhs, n), generator=gen, dtype=torch.float64, device="cuda")  # column-major n x nrhs
+    a, b = a0.clone(), b0.clone()
+    torch.cuda.synchronize()
+    A = dd.DistributedMatrix.from_torch(handle, a, n, n, 512, 512)
+    B = dd.DistributedMatrix.from_torch(handle, b, n, nrhs, 512, 512)
+    _, it, info = dd.pposv_mixed(A, B, uplo, factor=factor, tol=tol)
+    assert info == 0 and 1 <= it <= 30
+    assert torch.equal(a, a0)  # A untouched
+    r = b0 - b @ a0
+    berr = (r.abs().amax(1) / (a0.abs().sum(1).max() * b.abs().amax(1))).max().item()
+    assert berr <= tol, berr
+
+
 def test_iterations_grow_with_condition(handle):
     n, nrhs = 512, 3
     its = []
