@@ -27,7 +27,6 @@ namespace {
 
 using kern::kTriLower;
 using kern::kTriNone;
-using kern::kTriUpper;
 
 constexpr int kIB = 64;  // diagonal block of potf2 and of the solves
 
@@ -41,15 +40,14 @@ constexpr int kIB = 64;  // diagonal block of potf2 and of the solves
 // panel buffers hold 512 columns).
 template <class T> int outer_block(int n) { return std::is_same<T, double>::value && n >= 16384 ? 768 : 512; }
 
-// The factor is addressed through an "L view": element (i, j), i >= j, of the
-// lower factor is A[i + j*lda] for uplo 'L' and A[j + i*lda] for uplo 'U'
-// (U = L^T).  The helpers below translate L-view operations into GEMM calls
-// on the stored layout.
+// The blocked factorization always runs on the lower triangle (uplo 'U' is
+// brought there by potrf_blocked): element (i, j), i >= j, of L is
+// A[i + j*lda].  The helpers below translate operations on L into kernel
+// calls.
 template <class T>
 struct LView {
     T* A;
     long long lda;
-    bool upper;
     T* at(long long r, long long c) const { return A + r + c * lda; }  // stored (r, c)
 };
 
@@ -63,14 +61,8 @@ void update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& v, int
     if (M <= 0 || N <= 0) return;
     LaunchTimer timer(h, cls, flops, 0, st);
     const bool lookahead = cls == kClassLookahead;
-    if (!v.upper) {
-        kern::gemm<T>(st, 'N', 'T', M, N, K, alpha, v.at(ai, ak), v.lda, v.at(bj, bk), v.lda, beta, v.at(ci, cj),
-                      v.lda, mask ? kTriLower : kTriNone, ci - cj, abort, lookahead);
-    } else {
-        // Stored block at (cj, ci) is C_L^T = B_L * A_L^T.
-        kern::gemm<T>(st, 'T', 'N', N, M, K, alpha, v.at(bk, bj), v.lda, v.at(ak, ai), v.lda, beta, v.at(cj, ci),
-                      v.lda, mask ? kTriUpper : kTriNone, cj - ci, abort, lookahead);
-    }
+    kern::gemm<T>(st, 'N', 'T', M, N, K, alpha, v.at(ai, ak), v.lda, v.at(bj, bk), v.lda, beta, v.at(ci, cj), v.lda,
+                  mask ? kTriLower : kTriNone, ci - cj, abort, lookahead);
 }
 
 // Factors the panel of columns [k, k + kb) of the order-n L view (its
@@ -81,7 +73,7 @@ void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k
 {
     const double below = (double)n - k - kb;
     LaunchTimer timer(h, kClassPanel, (double)kb * kb * kb / 3.0 + below * kb * kb, 0, st);
-    kern::panel_factor<T>(st, v.upper, n, v.A, v.lda, k, kb, info, work);
+    kern::panel_factor<T>(st, false, n, v.A, v.lda, k, kb, info, work);
 }
 
 // BF16 split-product trailing updates (FP32 factorization only): after panel
@@ -105,9 +97,8 @@ void split_update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& 
     if (M <= 0 || N <= 0) return;
     LaunchTimer timer(h, cls, flops, 0, st);
     if constexpr (std::is_same<T, float>::value) {
-        T* C = v.upper ? v.at(r + b_row, r + a_row) : v.at(r + a_row, r + b_row);
-        kern::split_update(st, sp.products, v.upper, M, N, kb, buf, m_split, a_row, b_row, C, v.lda, true,
-                           (long long)a_row - b_row, abort);
+        kern::split_update(st, sp.products, false, M, N, kb, buf, m_split, a_row, b_row, v.at(r + a_row, r + b_row),
+                           v.lda, true, (long long)a_row - b_row, abort);
     } else {
         throw std::logic_error("split-product updates apply to the FP32 factorization");
     }
@@ -150,7 +141,7 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
         kern::swap_triangles<T>(h->stream, n, A, lda);
         return;
     }
-    const LView<T> v{A, lda, upper};
+    const LView<T> v{A, lda};
     const int NB = outer_block<T>(n);
     const int np = (n + NB - 1) / NB;
     hipStream_t main = h->stream, side = h->side, look = h->look;
@@ -161,9 +152,8 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
         factor_panel<T>(h, st, v, n, k, kb, info, work);
         const long long r = (long long)k + kb;
         if (!split || r >= n) return;
-        const T* X = upper ? v.at(k, r) : v.at(r, k);
         if constexpr (std::is_same<T, float>::value)
-            kern::split_panel(st, split->products, upper, n - (int)r, kb, X, lda, split->buf[q % 3]);
+            kern::split_panel(st, split->products, false, n - (int)r, kb, v.at(r, k), lda, split->buf[q % 3]);
     };
     // C_L(i, j) -= sum over panel p's columns of L(i, .) L(j, .) for columns
     // c_lo <= j < c_hi and rows i >= j (lower trapezoid), on stream st
