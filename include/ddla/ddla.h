@@ -29,6 +29,10 @@ namespace ddla {
 
 enum class DdlaBackend { CPU, GPU };
 
+// Communication mode of a handle (include/ddla/ddla_handle_t.h:22-26).  A 1 x 1
+// grid exchanges nothing, so every mode is accepted and only recorded.
+enum class DdlaCommunication { MPI = 0, HostStagedMPI = 1, CCL = 2 };
+
 enum class ddlaStatus_t {
     DDLA_STATUS_SUCCESS = 0,
     DDLA_STATUS_INVALID_HANDLE,
@@ -64,6 +68,21 @@ ddlaStatus_t ddlaSynchronize(const DdlaHandle_t& handle);
 const char* ddlaGetErrorString(ddlaStatus_t status);
 // Replace the compute stream (the handle owns a non-null replacement; null
 // selects the default stream, which is never destroyed).
+// Rectangle copy of height rows of width bytes between pitched buffers
+// (include/ddla/ddla_handle_t.h:180-197): direction from where each rectangle
+// starts, host-to-host copies finish before returning; an empty rectangle
+// succeeds without touching either pointer; null pointers or a pitch below
+// width are DDLA_STATUS_INVALID_VALUE.
+ddlaStatus_t ddlaMemcpy2D(const DdlaHandle_t& handle, void* dst, std::size_t dpitch, const void* src,
+                          std::size_t spitch, std::size_t width, std::size_t height);
+// Communication selection (include/ddla/ddla_handle_t.h:80-99): a valid mode
+// is recorded (default CCL, the reference's GPU default); an invalid one is
+// DDLA_STATUS_INVALID_VALUE and keeps the previous mode; null handles are
+// DDLA_STATUS_INVALID_HANDLE for the setter, std::runtime_error for the
+// getter and false for the query.
+ddlaStatus_t ddlaSetCommunication(const DdlaHandle_t& handle, DdlaCommunication mode);
+DdlaCommunication ddlaGetCommunication(const DdlaHandle_t& handle);
+bool ddlaCommunicationAvailable(const DdlaHandle_t& handle, DdlaCommunication mode);
 ddlaStatus_t ddlaSetStream(const DdlaHandle_t& handle, void* stream);
 void* ddlaGetStream(const DdlaHandle_t& handle);
 

@@ -48,7 +48,21 @@ int ddla_c_numloc(int n, int nb, int iproc, int srcproc, int nprocs); /* :224-22
 int ddla_c_malloc(void* handle, void** ptr, long long bytes);       /* :229-234 */
 int ddla_c_free(void* handle, void* ptr);                           /* :236-240 */
 int ddla_c_memcpy(void* handle, void* dst, const void* src, long long bytes); /* :242-247 */
+int ddla_c_memcpy2d(void* handle, void* dst, long long dpitch, const void* src, long long spitch,
+                    long long width, long long height);             /* :249-258 */
 int ddla_c_memset(void* handle, void* ptr, int value, long long bytes); /* :260-265 */
+/* Communication mode 0 MPI, 1 host-staged MPI, 2 CCL: every valid mode is
+ * accepted and recorded (a 1 x 1 grid exchanges nothing). */
+int ddla_c_set_communication(void* handle, int communication);      /* :134-142 */
+int ddla_c_get_communication(void* handle, int* communication);     /* :144-154 */
+int ddla_c_communication_available(void* handle, int communication); /* :156-163 */
+/* Communicator helpers of a single process: default communicator 0, rank 0,
+ * size 1; the dimension broadcast leaves its arguments as they are. */
+int ddla_c_default_comm(void);                                      /* :304-308 */
+int ddla_c_comm_rank(int f_comm);                                   /* :310-317 */
+int ddla_c_comm_size(int f_comm);                                   /* :319-326 */
+void ddla_c_finalize(void);                                         /* :328-337 */
+int ddla_c_bcast_dims(int f_comm, int* m, int* n, int* type_char, int root); /* :339-349 */
 int ddla_c_synchronize(void* handle);                               /* :267-271 */
 /* ddlaSetStream / ddlaGetStream (include/ddla/ddla_handle_t.h:300, :337);
  * no C entry in the reference. */

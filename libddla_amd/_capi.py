@@ -95,6 +95,14 @@ _SIGNATURES = {
     "ddla_c_free": (_i, [_vp, _vp]),
     "ddla_c_memcpy": (_i, [_vp, _vp, _vp, _ll]),
     "ddla_c_memset": (_i, [_vp, _vp, _i, _ll]),
+    "ddla_c_memcpy2d": (_i, [_vp, _vp, _ll, _vp, _ll, _ll, _ll]),
+    "ddla_c_set_communication": (_i, [_vp, _i]),
+    "ddla_c_get_communication": (_i, [_vp, ctypes.POINTER(_i)]),
+    "ddla_c_communication_available": (_i, [_vp, _i]),
+    "ddla_c_default_comm": (_i, []),
+    "ddla_c_comm_rank": (_i, [_i]),
+    "ddla_c_comm_size": (_i, [_i]),
+    "ddla_c_bcast_dims": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), _i]),
     "ddla_c_synchronize": (_i, [_vp]),
     "ddla_c_set_stream": (_i, [_vp, _vp]),
     "ddla_c_get_stream": (_vp, [_vp]),

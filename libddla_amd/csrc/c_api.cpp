@@ -157,6 +157,44 @@ int ddla_c_memcpy(void* handle, void* dst, const void* src, long long bytes)
     return S(ddla::ddlaMemcpy(H(handle), dst, src, static_cast<std::size_t>(bytes)));
 }
 
+int ddla_c_memcpy2d(void* handle, void* dst, long long dpitch, const void* src, long long spitch, long long width,
+                    long long height)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (dpitch < 0 || spitch < 0 || width < 0 || height < 0) return DDLA_C_STATUS_INVALID_VALUE;
+    return S(ddla::ddlaMemcpy2D(H(handle), dst, (std::size_t)dpitch, src, (std::size_t)spitch, (std::size_t)width,
+                                (std::size_t)height));
+}
+
+int ddla_c_set_communication(void* handle, int communication)
+{
+    return guarded([&]() -> int {
+        return S(ddla::ddlaSetCommunication(H(handle), static_cast<ddla::DdlaCommunication>(communication)));
+    });
+}
+
+int ddla_c_get_communication(void* handle, int* communication)
+{
+    if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;
+    if (communication == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+    return guarded([&]() -> int {
+        *communication = static_cast<int>(ddla::ddlaGetCommunication(H(handle)));
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_communication_available(void* handle, int communication)
+{
+    return ddla::ddlaCommunicationAvailable(H(handle), static_cast<ddla::DdlaCommunication>(communication)) ? 1 : 0;
+}
+
+// One process: the communicator helpers describe a world of one rank.
+int ddla_c_default_comm() { return 0; }
+int ddla_c_comm_rank(int) { return 0; }
+int ddla_c_comm_size(int) { return 1; }
+void ddla_c_finalize() {}
+int ddla_c_bcast_dims(int, int*, int*, int*, int) { return 0; }
+
 int ddla_c_memset(void* handle, void* ptr, int value, long long bytes)
 {
     if (handle == nullptr) return DDLA_C_STATUS_INVALID_HANDLE;

@@ -38,6 +38,7 @@ public:
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
+    DdlaCommunication communication = DdlaCommunication::CCL;  // recorded only: a 1 x 1 grid exchanges nothing
     // Streams of the look-ahead factorization (non-blocking, default
     // priority): `side` factors panels, `look` updates the next two panels'
     // columns; the three events order them against `stream`.

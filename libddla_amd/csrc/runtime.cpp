@@ -203,6 +203,52 @@ ddlaStatus_t ddlaMemcpy(const DdlaHandle_t& handle, void* dst, const void* src, 
     return ddlaStatus_t::DDLA_STATUS_SUCCESS;
 }
 
+ddlaStatus_t ddlaMemcpy2D(const DdlaHandle_t& handle, void* dst, std::size_t dpitch, const void* src,
+                          std::size_t spitch, std::size_t width, std::size_t height)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (width == 0 || height == 0) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    if (dst == nullptr || src == nullptr || dpitch < width || spitch < width)
+        return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const bool dd = is_device_pointer(dst), sd = is_device_pointer(src);
+    if (!dd && !sd) {
+        for (std::size_t r = 0; r < height; ++r)
+            std::memcpy(static_cast<char*>(dst) + r * dpitch, static_cast<const char*>(src) + r * spitch, width);
+        return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    }
+    const hipMemcpyKind kind = sd ? (dd ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost) : hipMemcpyHostToDevice;
+    const hipError_t e = hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, kind, handle->stream);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+namespace {
+bool valid_communication(DdlaCommunication mode)
+{
+    return mode == DdlaCommunication::MPI || mode == DdlaCommunication::HostStagedMPI ||
+           mode == DdlaCommunication::CCL;
+}
+} // namespace
+
+ddlaStatus_t ddlaSetCommunication(const DdlaHandle_t& handle, DdlaCommunication mode)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (!valid_communication(mode)) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    handle->communication = mode;
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+DdlaCommunication ddlaGetCommunication(const DdlaHandle_t& handle)
+{
+    if (handle == nullptr) throw std::runtime_error("ddlaGetCommunication: handle is null");
+    return handle->communication;
+}
+
+bool ddlaCommunicationAvailable(const DdlaHandle_t& handle, DdlaCommunication mode)
+{
+    return handle != nullptr && valid_communication(mode);
+}
+
 ddlaStatus_t ddlaMemset(const DdlaHandle_t& handle, void* ptr, int value, std::size_t bytes)
 {
     if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;

@@ -1,0 +1,99 @@
+"""GPU tests of the runtime entries beside the drivers
+(fortran/ddla_c_api.cpp:134-163, :249-258, :304-349 of the reference):
+
+  * ddla_c_memcpy2d: rectangles between pitched buffers in every direction
+    (host <-> device, device -> device, host -> host) against numpy slicing,
+    with the bytes outside the rectangle left as they were; the empty
+    rectangle succeeds without touching null pointers; a pitch below the width,
+    a null pointer with a nonempty rectangle and a null handle are rejected
+    (include/ddla/ddla_handle_t.h:180-197);
+  * communication selection: every valid mode is accepted and read back, an
+    invalid one is status 2 and keeps the previous mode, the default is CCL,
+    the query is false for invalid modes and null handles;
+  * the single-process communicator helpers: rank 0 of a world of 1, and a
+    dimension broadcast that leaves its arguments as they are.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import libddla_amd as dd
+
+pytestmark = pytest.mark.gpu
+
+
+def device_buffer(h, nbytes):
+    p = ctypes.c_void_p()
+    dd.check_status(dd.lib.ddla_c_malloc(h.ptr, ctypes.byref(p), nbytes), "ddla_c_malloc")
+    return p
+
+
+def test_memcpy2d_every_direction(handle):
+    h = handle
+    rng = np.random.RandomState(0)
+    rows, width, spitch, dpitch = 37, 24, 40, 56  # bytes per row copied, source and destination pitches
+    src = rng.randint(0, 256, size=(rows, spitch), dtype=np.uint8)
+    # host -> device: a device buffer prefilled with a pattern
+    dev = device_buffer(h, rows * dpitch)
+    fill = np.full((rows, dpitch), 0xA5, dtype=np.uint8)
+    dd.check_status(dd.lib.ddla_c_memcpy(h.ptr, dev, fill.ctypes.data, fill.nbytes))
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, dev, dpitch, src.ctypes.data, spitch, width, rows) == 0
+    back = np.zeros((rows, dpitch), dtype=np.uint8)
+    dd.check_status(dd.lib.ddla_c_memcpy(h.ptr, back.ctypes.data, dev, back.nbytes))
+    h.synchronize()
+    want = fill.copy()
+    want[:, :width] = src[:, :width]
+    assert np.array_equal(back, want)
+    # device -> device, into a second device buffer with another pitch
+    dev2 = device_buffer(h, rows * 32)
+    zero = np.zeros((rows, 32), dtype=np.uint8)
+    dd.check_status(dd.lib.ddla_c_memcpy(h.ptr, dev2, zero.ctypes.data, zero.nbytes))
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, dev2, 32, dev, dpitch, width, rows) == 0
+    # device -> host
+    out = np.full((rows, 30), 7, dtype=np.uint8)
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, out.ctypes.data, 30, dev2, 32, width, rows) == 0
+    h.synchronize()
+    want2 = np.full((rows, 30), 7, dtype=np.uint8)
+    want2[:, :width] = src[:, :width]
+    assert np.array_equal(out, want2)
+    # host -> host finishes before returning
+    hh = np.zeros((rows, width + 8), dtype=np.uint8)
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, hh.ctypes.data, width + 8, src.ctypes.data, spitch, width, rows) == 0
+    assert np.array_equal(hh[:, :width], src[:, :width]) and not hh[:, width:].any()
+    # empty rectangles, bad arguments
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, None, 0, None, 0, 0, 5) == 0
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, None, 0, None, 0, 5, 0) == 0
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, dev, 8, src.ctypes.data, spitch, 16, 2) == 2
+    assert dd.lib.ddla_c_memcpy2d(h.ptr, None, dpitch, src.ctypes.data, spitch, width, 2) == 2
+    assert dd.lib.ddla_c_memcpy2d(None, dev, dpitch, src.ctypes.data, spitch, width, 2) == 1
+    for p in (dev, dev2):
+        dd.check_status(dd.lib.ddla_c_free(h.ptr, p))
+
+
+def test_communication_selection():
+    h = dd.Handle(dd.Backend.GPU)
+    try:
+        mode = ctypes.c_int(-1)
+        assert dd.lib.ddla_c_get_communication(h.ptr, ctypes.byref(mode)) == 0 and mode.value == 2
+        for m in (0, 1, 2):
+            assert dd.lib.ddla_c_communication_available(h.ptr, m) == 1
+            assert dd.lib.ddla_c_set_communication(h.ptr, m) == 0
+            assert dd.lib.ddla_c_get_communication(h.ptr, ctypes.byref(mode)) == 0 and mode.value == m
+        assert dd.lib.ddla_c_set_communication(h.ptr, 7) == 2
+        assert dd.lib.ddla_c_get_communication(h.ptr, ctypes.byref(mode)) == 0 and mode.value == 2
+        assert dd.lib.ddla_c_communication_available(h.ptr, 3) == 0
+        assert dd.lib.ddla_c_communication_available(None, 2) == 0
+        assert dd.lib.ddla_c_set_communication(None, 2) == 1
+        assert dd.lib.ddla_c_get_communication(None, ctypes.byref(mode)) == 1
+        assert dd.lib.ddla_c_get_communication(h.ptr, None) == 2
+    finally:
+        h.destroy()
+
+
+def test_single_process_communicator_helpers():
+    comm = dd.lib.ddla_c_default_comm()
+    assert dd.lib.ddla_c_comm_rank(comm) == 0 and dd.lib.ddla_c_comm_size(comm) == 1
+    m, n, t = ctypes.c_int(5), ctypes.c_int(6), ctypes.c_int(ord("d"))
+    assert dd.lib.ddla_c_bcast_dims(comm, ctypes.byref(m), ctypes.byref(n), ctypes.byref(t), 0) == 0
+    assert (m.value, n.value, t.value) == (5, 6, ord("d"))
