@@ -24,6 +24,7 @@
 #include <cstddef>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace ddla {
 
@@ -66,6 +67,67 @@ ddlaStatus_t ddlaMemcpy(const DdlaHandle_t& handle, void* dst, const void* src, 
 ddlaStatus_t ddlaMemset(const DdlaHandle_t& handle, void* ptr, int value, std::size_t bytes);
 ddlaStatus_t ddlaSynchronize(const DdlaHandle_t& handle);
 const char* ddlaGetErrorString(ddlaStatus_t status);
+
+// Further helpers of the reference's handle API (include/ddla/ddla_handle_t.h:
+// 146-155, 205-280, 328-334), for the GPU handle:
+//   ddlaMallocHost / ddlaFreeHost  pinned host memory (zero bytes: *ptr = null;
+//                                  freeing null succeeds)
+//   ddlaMemsetTri / ddlaMemcpyTri  one triangle of a column-major m x n matrix
+//                                  (uplo 'U': row <= column, 'L': row >= column,
+//                                  either case; strict without the diagonal);
+//                                  memset repeats the low byte of value; device
+//                                  memory takes one kernel on the handle's
+//                                  stream, host/device copies go column segment
+//                                  by segment, host-to-host copies are
+//                                  immediate.  An empty selection succeeds
+//                                  untouched; bad uplo, extents, leading
+//                                  dimensions or null pointers are
+//                                  DDLA_STATUS_INVALID_VALUE, and so is host
+//                                  memory for ddlaMemsetTri.
+//   ddlaDeviceSynchronize, events (record on / wait on / synchronize with the
+//   handle's stream), ddlaGetLastError, ddlaSetDevice, ddlaGetDeviceCount,
+//   ddlaMemGetInfo, and the rank <-> grid coordinate maps of the handle's
+//   grid order ('R' or 'C').
+using ddlaEvent_t = void*;
+ddlaStatus_t ddlaMallocHost(const DdlaHandle_t& handle, void** ptr, std::size_t bytes);
+template <typename T>
+inline ddlaStatus_t ddlaMallocHost(const DdlaHandle_t& handle, T** ptr, std::size_t bytes)
+{
+    return ddlaMallocHost(handle, reinterpret_cast<void**>(ptr), bytes);
+}
+ddlaStatus_t ddlaFreeHost(const DdlaHandle_t& handle, void* ptr);
+namespace detail {
+ddlaStatus_t memset_tri(const DdlaHandle_t& handle, char uplo, int m, int n, void* ptr, int lda, int value,
+                        std::size_t elem, bool include_diagonal);
+ddlaStatus_t memcpy_tri(const DdlaHandle_t& handle, char uplo, int m, int n, void* dst, int ldd, const void* src,
+                        int lds, std::size_t elem, bool include_diagonal);
+} // namespace detail
+template <class T>
+ddlaStatus_t ddlaMemsetTri(const DdlaHandle_t& handle, char uplo, int m, int n, T* ptr, int lda, int value = 0,
+                           bool include_diagonal = true)
+{
+    static_assert(std::is_trivially_copyable<T>::value, "triangular memory requires trivially copyable elements");
+    return detail::memset_tri(handle, uplo, m, n, ptr, lda, value, sizeof(T), include_diagonal);
+}
+template <class T>
+ddlaStatus_t ddlaMemcpyTri(const DdlaHandle_t& handle, char uplo, int m, int n, T* dst, int ldd, const T* src,
+                           int lds, bool include_diagonal = true)
+{
+    static_assert(std::is_trivially_copyable<T>::value, "triangular memory requires trivially copyable elements");
+    return detail::memcpy_tri(handle, uplo, m, n, dst, ldd, src, lds, sizeof(T), include_diagonal);
+}
+ddlaStatus_t ddlaDeviceSynchronize(const DdlaHandle_t& handle);
+ddlaStatus_t ddlaEventCreate(const DdlaHandle_t& handle, ddlaEvent_t* event);
+ddlaStatus_t ddlaEventDestroy(const DdlaHandle_t& handle, ddlaEvent_t event);
+ddlaStatus_t ddlaEventRecord(const DdlaHandle_t& handle, ddlaEvent_t event);
+ddlaStatus_t ddlaEventWait(const DdlaHandle_t& handle, ddlaEvent_t event);
+ddlaStatus_t ddlaEventSynchronize(const DdlaHandle_t& handle, ddlaEvent_t event);
+ddlaStatus_t ddlaGetLastError(const DdlaHandle_t& handle);
+ddlaStatus_t ddlaSetDevice(const DdlaHandle_t& handle);
+ddlaStatus_t ddlaGetDeviceCount(const DdlaHandle_t& handle, int* count);
+ddlaStatus_t ddlaMemGetInfo(const DdlaHandle_t& handle, std::size_t* free_bytes, std::size_t* total_bytes);
+void ddlaRankToRc(const DdlaHandle_t& handle, int rank, int& row, int& col);
+int ddlaRcToRank(const DdlaHandle_t& handle, int row, int col);
 // Replace the compute stream (the handle owns a non-null replacement; null
 // selects the default stream, which is never destroyed).
 // Rectangle copy of height rows of width bytes between pitched buffers

@@ -2,6 +2,7 @@
 // these; every launcher enqueues on the given stream and returns immediately.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <cstdint>
 
 namespace ddla {
@@ -209,6 +210,15 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
 template <class T>
 void reverse_triangle(hipStream_t s, bool out_upper, bool transpose, int n, const T* in, long long ldi, T* out,
                       long long ldo, bool zero_other);
+// One triangle of a column-major m x n matrix of elem-byte elements (upper:
+// i <= j, lower: i >= j; the diagonal only when diag): every byte set to
+// value, or copied from src (leading dimensions in elements; the regions must
+// not overlap).
+void memset_tri_bytes(hipStream_t s, bool upper, bool diag, int m, int n, void* p, long long ld, std::size_t elem,
+                      unsigned char value);
+void memcpy_tri_bytes(hipStream_t s, bool upper, bool diag, int m, int n, void* dst, long long ldd, const void* src,
+                      long long lds, std::size_t elem);
+
 // In place: exchanges the strict lower and strict upper triangles of the
 // order-n A (A(i, j) <-> A(j, i), i > j); applied twice it is the identity.
 template <class T>

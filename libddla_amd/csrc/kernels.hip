@@ -344,6 +344,50 @@ __global__ __launch_bounds__(256) void reverse_triangle_kernel(bool out_upper, b
     }
 }
 
+__device__ inline bool tri_selected(bool upper, bool diag, int i, int j)
+{
+    return upper ? (diag ? i <= j : i < j) : (diag ? i >= j : i > j);
+}
+
+// (rows / 256, min(n, 65535)) workgroups, one thread per element; columns
+// grid-strided.  Elements whose size and address allow it move as 32-bit
+// words, others byte by byte.
+__global__ __launch_bounds__(256) void memset_tri_kernel(bool upper, bool diag, int m, int n, unsigned char* p,
+                                                         long long ld_bytes, int elem, unsigned char value)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const unsigned int word = 0x01010101u * value;
+    for (int j = blockIdx.y; j < n; j += gridDim.y) {
+        if (!tri_selected(upper, diag, i, j)) continue;
+        unsigned char* e = p + (long long)j * ld_bytes + (long long)i * elem;
+        if (elem % 4 == 0 && (uintptr_t)e % 4 == 0) {
+            for (int w = 0; w < elem / 4; ++w) reinterpret_cast<unsigned int*>(e)[w] = word;
+        } else {
+            for (int b = 0; b < elem; ++b) e[b] = value;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void memcpy_tri_kernel(bool upper, bool diag, int m, int n, unsigned char* d,
+                                                         long long ldd_bytes, const unsigned char* s,
+                                                         long long lds_bytes, int elem)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    for (int j = blockIdx.y; j < n; j += gridDim.y) {
+        if (!tri_selected(upper, diag, i, j)) continue;
+        unsigned char* e = d + (long long)j * ldd_bytes + (long long)i * elem;
+        const unsigned char* f = s + (long long)j * lds_bytes + (long long)i * elem;
+        if (elem % 4 == 0 && (uintptr_t)e % 4 == 0 && (uintptr_t)f % 4 == 0) {
+            for (int w = 0; w < elem / 4; ++w)
+                reinterpret_cast<unsigned int*>(e)[w] = reinterpret_cast<const unsigned int*>(f)[w];
+        } else {
+            for (int b = 0; b < elem; ++b) e[b] = f[b];
+        }
+    }
+}
+
 // In-place exchange of the strict lower and strict upper triangles of the
 // order-n A: A(i, j) <-> A(j, i) for i > j.  The workgroup of the 32 x 32 tile
 // pair (bi, bj), bi >= bj, loads the strict-lower part of tile (bi, bj) and
@@ -1066,6 +1110,27 @@ void reverse_triangle(hipStream_t s, bool out_upper, bool transpose, int n, cons
     reverse_triangle_kernel<T><<<dim3(tiles, tiles), 256, 0, s>>>(out_upper, transpose, n, in, ldi, out, ldo,
                                                                   zero_other);
     check_launch("reverse_triangle");
+}
+
+void memset_tri_bytes(hipStream_t s, bool upper, bool diag, int m, int n, void* p, long long ld, std::size_t elem,
+                      unsigned char value)
+{
+    if (m <= 0 || n <= 0) return;
+    const dim3 grid((unsigned)((m + 255) / 256), (unsigned)std::min(n, 65535));
+    memset_tri_kernel<<<grid, 256, 0, s>>>(upper, diag, m, n, static_cast<unsigned char*>(p),
+                                          ld * (long long)elem, (int)elem, value);
+    check_launch("memset_tri");
+}
+
+void memcpy_tri_bytes(hipStream_t s, bool upper, bool diag, int m, int n, void* dst, long long ldd, const void* src,
+                      long long lds, std::size_t elem)
+{
+    if (m <= 0 || n <= 0) return;
+    const dim3 grid((unsigned)((m + 255) / 256), (unsigned)std::min(n, 65535));
+    memcpy_tri_kernel<<<grid, 256, 0, s>>>(upper, diag, m, n, static_cast<unsigned char*>(dst),
+                                          ldd * (long long)elem, static_cast<const unsigned char*>(src),
+                                          lds * (long long)elem, (int)elem);
+    check_launch("memcpy_tri");
 }
 
 template <class T>

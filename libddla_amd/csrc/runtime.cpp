@@ -3,8 +3,10 @@
 // src/ddla_handle_t.cpp:94-200 (init / set / destroy), :322-770 (memory),
 // src/ddla_desc.cpp:9-60 (check_desc), include/ddla/ddla_desc.h:102-135.
 #include "internal.h"
+#include "kernels.h"
 
 #include <cstring>
+#include <limits>
 
 namespace ddla {
 
@@ -318,6 +320,235 @@ void ddlaGetGridDims(const DdlaHandle_t& handle, int& nprows, int& npcols)
     if (handle == nullptr) throw std::runtime_error("ddlaGetGridDims: null handle");
     nprows = handle->nprows_;
     npcols = handle->npcols_;
+}
+
+ddlaStatus_t ddlaMallocHost(const DdlaHandle_t& handle, void** ptr, std::size_t bytes)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (ptr == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    *ptr = nullptr;
+    if (bytes == 0) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return ddlaStatus_t::DDLA_STATUS_ALLOC_FAILED;
+    }
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaFreeHost(const DdlaHandle_t& handle, void* ptr)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (ptr == nullptr) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    const hipError_t e = hipHostFree(ptr);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+namespace {
+// Validation shared by the triangular helpers; *empty when nothing is
+// selected (then no pointer or leading dimension is looked at).
+ddlaStatus_t check_tri(char uplo, int m, int n, std::size_t elem, bool diag, bool& upper, bool& empty)
+{
+    const char u = (uplo >= 'a' && uplo <= 'z') ? char(uplo - 'a' + 'A') : uplo;
+    if ((u != 'U' && u != 'L') || m < 0 || n < 0 || elem == 0) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    upper = u == 'U';
+    // strict upper needs a second column, strict lower a second row
+    empty = m == 0 || n == 0 || (!diag && (upper ? n < 2 : m < 2));
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+bool tri_span_ok(int m, int n, int ld, std::size_t elem)
+{
+    if (ld < std::max(1, m)) return false;
+    const std::size_t limit = std::numeric_limits<std::size_t>::max() / elem;
+    return (std::size_t)ld <= limit / (std::size_t)n;  // ld * n elements fit in size_t bytes
+}
+
+// rows [lo, hi) of column j selected by the triangle
+void tri_rows(bool upper, bool diag, int m, int j, int& lo, int& hi)
+{
+    if (upper) {
+        lo = 0;
+        hi = std::min(m, diag ? j + 1 : j);
+    } else {
+        lo = std::min(m, diag ? j : j + 1);
+        hi = m;
+    }
+}
+} // namespace
+
+namespace detail {
+
+ddlaStatus_t memset_tri(const DdlaHandle_t& handle, char uplo, int m, int n, void* ptr, int lda, int value,
+                        std::size_t elem, bool include_diagonal)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    bool upper = false, empty = false;
+    const ddlaStatus_t st = check_tri(uplo, m, n, elem, include_diagonal, upper, empty);
+    if (st != ddlaStatus_t::DDLA_STATUS_SUCCESS || empty) return st;
+    if (ptr == nullptr || !tri_span_ok(m, n, lda, elem) || !is_device_pointer(ptr))
+        return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    try {
+        kern::memset_tri_bytes(handle->stream, upper, include_diagonal, m, n, ptr, lda, elem,
+                               (unsigned char)(value & 0xff));
+    } catch (const std::exception&) {
+        return ddlaStatus_t::DDLA_STATUS_INTERNAL_ERROR;
+    }
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t memcpy_tri(const DdlaHandle_t& handle, char uplo, int m, int n, void* dst, int ldd, const void* src,
+                        int lds, std::size_t elem, bool include_diagonal)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    bool upper = false, empty = false;
+    const ddlaStatus_t st = check_tri(uplo, m, n, elem, include_diagonal, upper, empty);
+    if (st != ddlaStatus_t::DDLA_STATUS_SUCCESS || empty) return st;
+    if (dst == nullptr || src == nullptr || !tri_span_ok(m, n, ldd, elem) || !tri_span_ok(m, n, lds, elem))
+        return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    if (dst == src && ldd == lds) return ddlaStatus_t::DDLA_STATUS_SUCCESS;  // self-copy
+    const bool dd = is_device_pointer(dst), sd = is_device_pointer(src);
+    auto* d = static_cast<unsigned char*>(dst);
+    const auto* s = static_cast<const unsigned char*>(src);
+    if (dd && sd) {
+        try {
+            kern::memcpy_tri_bytes(handle->stream, upper, include_diagonal, m, n, dst, ldd, src, lds, elem);
+        } catch (const std::exception&) {
+            return ddlaStatus_t::DDLA_STATUS_INTERNAL_ERROR;
+        }
+        return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    }
+    const hipMemcpyKind kind = sd ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    for (int j = 0; j < n; ++j) {
+        int lo = 0, hi = 0;
+        tri_rows(upper, include_diagonal, m, j, lo, hi);
+        if (hi <= lo) continue;
+        unsigned char* to = d + ((std::size_t)j * ldd + lo) * elem;
+        const unsigned char* from = s + ((std::size_t)j * lds + lo) * elem;
+        const std::size_t bytes = (std::size_t)(hi - lo) * elem;
+        if (!dd && !sd) {
+            std::memcpy(to, from, bytes);
+        } else {
+            const hipError_t e = hipMemcpyAsync(to, from, bytes, kind, handle->stream);
+            if (e != hipSuccess) return internal_error(e);
+        }
+    }
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+} // namespace detail
+
+ddlaStatus_t ddlaDeviceSynchronize(const DdlaHandle_t& handle)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    hipError_t e = hipSetDevice(handle->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaEventCreate(const DdlaHandle_t& handle, ddlaEvent_t* event)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (event == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    hipEvent_t ev = nullptr;
+    const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return internal_error(e);
+    *event = ev;
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaEventDestroy(const DdlaHandle_t& handle, ddlaEvent_t event)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (event == nullptr) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    const hipError_t e = hipEventDestroy(static_cast<hipEvent_t>(event));
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaEventRecord(const DdlaHandle_t& handle, ddlaEvent_t event)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (event == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const hipError_t e = hipEventRecord(static_cast<hipEvent_t>(event), handle->stream);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaEventWait(const DdlaHandle_t& handle, ddlaEvent_t event)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (event == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const hipError_t e = hipStreamWaitEvent(handle->stream, static_cast<hipEvent_t>(event), 0);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaEventSynchronize(const DdlaHandle_t& handle, ddlaEvent_t event)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (event == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const hipError_t e = hipEventSynchronize(static_cast<hipEvent_t>(event));
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaGetLastError(const DdlaHandle_t& handle)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaSetDevice(const DdlaHandle_t& handle)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    const hipError_t e = hipSetDevice(handle->device);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaGetDeviceCount(const DdlaHandle_t& handle, int* count)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (count == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    const hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+ddlaStatus_t ddlaMemGetInfo(const DdlaHandle_t& handle, std::size_t* free_bytes, std::size_t* total_bytes)
+{
+    if (handle == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE;
+    if (free_bytes == nullptr || total_bytes == nullptr) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    hipError_t e = hipSetDevice(handle->device);
+    if (e == hipSuccess) e = hipMemGetInfo(free_bytes, total_bytes);
+    if (e != hipSuccess) return internal_error(e);
+    return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+}
+
+// src/ddla_stream.h:114-125 of the reference; -1 for a null handle or one
+// without a grid.
+void ddlaRankToRc(const DdlaHandle_t& handle, int rank, int& row, int& col)
+{
+    row = col = -1;
+    if (handle == nullptr || handle->nprows_ <= 0 || handle->npcols_ <= 0) return;
+    if (handle->major == 'R') {
+        row = rank / handle->npcols_;
+        col = rank % handle->npcols_;
+    } else {
+        row = rank % handle->nprows_;
+        col = rank / handle->nprows_;
+    }
+}
+
+int ddlaRcToRank(const DdlaHandle_t& handle, int row, int col)
+{
+    if (handle == nullptr || handle->nprows_ <= 0 || handle->npcols_ <= 0) return -1;
+    return handle->major == 'R' ? row * handle->npcols_ + col : col * handle->nprows_ + row;
 }
 
 ddlaStatus_t ddlaDescInit(int* desc, const DdlaHandle_t& handle, int m, int n, int mb, int nb, int irsrc, int icsrc,

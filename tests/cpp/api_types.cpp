@@ -5,7 +5,12 @@
 //   * ddla::vector<T> on the GPU handle (include/ddla/ddla_vector.h:27-110
 //     of the reference): growth keeps the elements, shrinking and clear()
 //     keep the allocation, moves transfer ownership, reset() releases, bad
-//     arguments throw.
+//     arguments throw;
+//   * the handle helpers of include/ddla/ddla_handle_t.h:146-334 of the
+//     reference: pinned host memory, ddlaMemsetTri / ddlaMemcpyTri in every
+//     direction (both triangles, with and without the diagonal, padding rows
+//     and the other triangle untouched, a 3-byte element type), events, device
+//     queries and the rank <-> grid maps.
 // "api_types cpu" runs the index checks only (no GPU); no argument runs both.
 // Prints one line per failure and exits non-zero if any.
 #include <ddla/ddla.h>
@@ -24,6 +29,8 @@ static int failures = 0;
             ++failures;                                                                \
         }                                                                              \
     } while (0)
+
+static void handle_helpers_on_gpu(ddla::DdlaHandle_t h);
 
 static void index_maps()
 {
@@ -77,6 +84,7 @@ static void vector_on_gpu()
         EXPECT(x.size() == 0 && x.capacity() == 1000 && x.data() == before);
         x.reset();
         EXPECT(x.size() == 0 && x.capacity() == 0 && x.data() == nullptr);
+        handle_helpers_on_gpu(h);
         bool threw = false;
         try {
             x.reserve(x.max_size() + 1);
@@ -93,6 +101,133 @@ static void vector_on_gpu()
         EXPECT(threw);
     }
     DDLA_CHECK(ddla::ddlaDestroy(h));
+}
+
+// Element (i, j) of a column-major matrix with leading dimension ld is in the
+// triangle: upper i <= j, lower i >= j, strict without the diagonal.
+static bool in_tri(bool upper, bool diag, int i, int j)
+{
+    return upper ? (diag ? i <= j : i < j) : (diag ? i >= j : i > j);
+}
+
+struct Rgb {  // 3-byte element: the byte-by-byte path of the triangular kernels
+    unsigned char r, g, b;
+};
+
+static void handle_helpers_on_gpu(ddla::DdlaHandle_t h)
+{
+    using ddla::ddlaStatus_t;
+    int count = 0;
+    DDLA_CHECK(ddla::ddlaGetDeviceCount(h, &count));
+    EXPECT(count >= 1);
+    std::size_t free_b = 0, total_b = 0;
+    DDLA_CHECK(ddla::ddlaMemGetInfo(h, &free_b, &total_b));
+    EXPECT(total_b > 0 && free_b <= total_b);
+    DDLA_CHECK(ddla::ddlaSetDevice(h));
+    DDLA_CHECK(ddla::ddlaGetLastError(h));
+    int row = 9, col = 9;
+    ddla::ddlaRankToRc(h, 0, row, col);
+    EXPECT(row == 0 && col == 0 && ddla::ddlaRcToRank(h, 0, 0) == 0);
+    ddla::ddlaRankToRc(nullptr, 0, row, col);
+    EXPECT(row == -1 && col == -1 && ddla::ddlaRcToRank(nullptr, 0, 0) == -1);
+
+    double* pinned = nullptr;
+    DDLA_CHECK(ddla::ddlaMallocHost(h, &pinned, 64 * sizeof(double)));
+    EXPECT(pinned != nullptr);
+    void* none = &count;
+    DDLA_CHECK(ddla::ddlaMallocHost(h, &none, 0));
+    EXPECT(none == nullptr);
+    ddla::ddlaEvent_t ev = nullptr;
+    DDLA_CHECK(ddla::ddlaEventCreate(h, &ev));
+    auto sync = [&] {
+        DDLA_CHECK(ddla::ddlaEventRecord(h, ev));
+        DDLA_CHECK(ddla::ddlaEventWait(h, ev));
+        DDLA_CHECK(ddla::ddlaEventSynchronize(h, ev));
+    };
+
+    // 7 x 5 doubles with leading dimension 9 (two padding rows per column)
+    const int m = 7, n = 5, ld = 9;
+    const std::size_t bytes = (std::size_t)ld * n * sizeof(double);
+    std::vector<double> host(ld * n), back(ld * n), sentinel(ld * n, -1.0);
+    for (int i = 0; i < ld * n; ++i) host[i] = 100.0 + i;
+    ddla::vector<double> dev(h, ld * n), dev2(h, ld * n);
+    for (int strict = 0; strict < 2; ++strict)
+        for (char uplo : {'U', 'l'}) {
+            const bool upper = uplo == 'U', diag = strict == 0;
+            // memset: zero the triangle of a copy of host
+            DDLA_CHECK(ddla::ddlaMemcpy(h, dev.data(), host.data(), bytes));
+            DDLA_CHECK(ddla::ddlaMemsetTri(h, uplo, m, n, dev.data(), ld, 0, diag));
+            DDLA_CHECK(ddla::ddlaMemcpy(h, back.data(), dev.data(), bytes));
+            sync();
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < ld; ++i)
+                    EXPECT(back[i + j * ld] == ((i < m && in_tri(upper, diag, i, j)) ? 0.0 : host[i + j * ld]));
+            // host -> device, device -> device, device -> host, host -> host copies of the triangle
+            DDLA_CHECK(ddla::ddlaMemcpy(h, dev2.data(), sentinel.data(), bytes));
+            DDLA_CHECK(ddla::ddlaMemcpyTri(h, uplo, m, n, dev2.data(), ld, host.data(), ld, diag));
+            DDLA_CHECK(ddla::ddlaMemcpy(h, dev.data(), sentinel.data(), bytes));
+            DDLA_CHECK(ddla::ddlaMemcpyTri(h, uplo, m, n, dev.data(), ld, dev2.data(), ld, diag));
+            std::vector<double> out(ld * n, -2.0), hh(ld * n, -3.0);
+            DDLA_CHECK(ddla::ddlaMemcpyTri(h, uplo, m, n, out.data(), ld, dev.data(), ld, diag));
+            sync();
+            DDLA_CHECK(ddla::ddlaMemcpyTri(h, uplo, m, n, hh.data(), ld, out.data(), ld, diag));
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < ld; ++i) {
+                    const bool sel = i < m && in_tri(upper, diag, i, j);
+                    EXPECT(out[i + j * ld] == (sel ? host[i + j * ld] : -2.0));
+                    EXPECT(hh[i + j * ld] == (sel ? host[i + j * ld] : -3.0));
+                }
+        }
+    // memset repeats the low byte: value 0x101 on floats writes 0x01010101
+    {
+        const int fm = 33, fn = 4;
+        std::vector<float> f(fm * fn, 2.5f), fb(fm * fn);
+        ddla::vector<float> fd(h, fm * fn);
+        DDLA_CHECK(ddla::ddlaMemcpy(h, fd.data(), f.data(), f.size() * sizeof(float)));
+        DDLA_CHECK(ddla::ddlaMemsetTri(h, 'L', fm, fn, fd.data(), fm, 0x101));
+        DDLA_CHECK(ddla::ddlaMemcpy(h, fb.data(), fd.data(), f.size() * sizeof(float)));
+        sync();
+        unsigned int pattern = 0x01010101u;
+        float want;
+        std::memcpy(&want, &pattern, sizeof(want));
+        for (int j = 0; j < fn; ++j)
+            for (int i = 0; i < fm; ++i)
+                EXPECT(std::memcmp(&fb[i + j * fm], i >= j ? &want : &f[i + j * fm], sizeof(float)) == 0);
+    }
+    // 3-byte elements (byte-by-byte kernel path), device to device
+    {
+        const int rm = 6, rn = 6, rl = 6;
+        std::vector<Rgb> src(rl * rn), dst(rl * rn, Rgb{1, 2, 3}), got(rl * rn);
+        for (int i = 0; i < rl * rn; ++i) src[i] = Rgb{(unsigned char)i, (unsigned char)(i + 1), (unsigned char)(i + 2)};
+        ddla::vector<Rgb> rs(h, rl * rn), rd(h, rl * rn);
+        DDLA_CHECK(ddla::ddlaMemcpy(h, rs.data(), src.data(), src.size() * sizeof(Rgb)));
+        DDLA_CHECK(ddla::ddlaMemcpy(h, rd.data(), dst.data(), dst.size() * sizeof(Rgb)));
+        DDLA_CHECK(ddla::ddlaMemcpyTri(h, 'U', rm, rn, rd.data(), rl, rs.data(), rl, false));
+        DDLA_CHECK(ddla::ddlaMemcpy(h, got.data(), rd.data(), got.size() * sizeof(Rgb)));
+        sync();
+        for (int j = 0; j < rn; ++j)
+            for (int i = 0; i < rm; ++i) {
+                const Rgb& e = got[i + j * rl];
+                const Rgb& w = i < j ? src[i + j * rl] : dst[i + j * rl];
+                EXPECT(e.r == w.r && e.g == w.g && e.b == w.b);
+            }
+    }
+    // argument rules
+    EXPECT(ddla::ddlaMemsetTri(h, 'X', m, n, dev.data(), ld) == ddlaStatus_t::DDLA_STATUS_INVALID_VALUE);
+    EXPECT(ddla::ddlaMemsetTri(h, 'U', m, n, dev.data(), m - 1) == ddlaStatus_t::DDLA_STATUS_INVALID_VALUE);
+    EXPECT(ddla::ddlaMemsetTri(h, 'U', -1, n, dev.data(), ld) == ddlaStatus_t::DDLA_STATUS_INVALID_VALUE);
+    EXPECT(ddla::ddlaMemsetTri(h, 'U', m, n, host.data(), ld) == ddlaStatus_t::DDLA_STATUS_INVALID_VALUE);
+    EXPECT(ddla::ddlaMemsetTri(h, 'U', 0, n, static_cast<double*>(nullptr), 0) == ddlaStatus_t::DDLA_STATUS_SUCCESS);
+    EXPECT(ddla::ddlaMemsetTri(h, 'U', m, 1, static_cast<double*>(nullptr), 0, 0, false) ==
+           ddlaStatus_t::DDLA_STATUS_SUCCESS);
+    EXPECT(ddla::ddlaMemcpyTri(h, 'L', m, n, dev.data(), ld, static_cast<const double*>(nullptr), ld) ==
+           ddlaStatus_t::DDLA_STATUS_INVALID_VALUE);
+    EXPECT(ddla::ddlaMemcpyTri(nullptr, 'L', m, n, dev.data(), ld, dev2.data(), ld) ==
+           ddlaStatus_t::DDLA_STATUS_INVALID_HANDLE);
+    DDLA_CHECK(ddla::ddlaFreeHost(h, pinned));
+    DDLA_CHECK(ddla::ddlaFreeHost(h, nullptr));
+    DDLA_CHECK(ddla::ddlaEventDestroy(h, ev));
+    DDLA_CHECK(ddla::ddlaDeviceSynchronize(h));
 }
 
 int main(int argc, char** argv)

@@ -5,7 +5,13 @@ tests/cpp/api_types.cpp (built next to the library by the csrc Makefile):
     reference), for block sizes 1-64, 1-5 processes, every source process;
   * ddla::vector<T> (include/ddla/ddla_vector.h:27-110 of the reference) on
     the GPU handle: growth keeps the values, shrinking / clear() keep the
-    allocation, moves transfer it, reset() releases it, bad arguments throw.
+    allocation, moves transfer it, reset() releases it, bad arguments throw;
+  * the handle helpers (include/ddla/ddla_handle_t.h:146-334 of the
+    reference): pinned host memory, ddlaMemsetTri / ddlaMemcpyTri in every
+    direction against element-wise expectations (both triangles, with and
+    without the diagonal, padding and the other triangle untouched, a 3-byte
+    element), events, device queries, the rank <-> grid maps and the argument
+    rules.
 """
 import os
 import subprocess
@@ -28,6 +34,6 @@ def test_block_cyclic_index_maps():
 
 
 @pytest.mark.gpu
-def test_vector_on_gpu_handle():
-    print("case ddla::vector on the GPU handle", flush=True)
+def test_vector_and_handle_helpers_on_gpu():
+    print("case ddla::vector and the handle helpers on the GPU handle", flush=True)
     run_driver()
