@@ -207,6 +207,48 @@ ddlaStatus_t ddlaDescInit(int* desc, const DdlaHandle_t& handle, int m, int n, i
 // Validation of a descriptor against the handle's grid (src/ddla_desc.cpp:9-60).
 void check_desc(const int* desc, const DdlaHandle_t& handle);
 
+// Descriptor-level index maps on the handle's grid (the reference's
+// include/ddla/ddla_desc.h:143-191): local row / column of a global one, or -1
+// when this process does not own it (or it lies beyond M / N); global row /
+// column of a local one.  A null handle gives -1.
+namespace desc_detail {
+inline bool grid(const DdlaHandle_t& h, int& nprows, int& npcols, int& myprow, int& mypcol)
+{
+    if (h == nullptr) return false;
+    ddlaGetGridDims(h, nprows, npcols);
+    ddlaGetGridCoords(h, myprow, mypcol);
+    return nprows > 0 && npcols > 0;
+}
+} // namespace desc_detail
+inline int indx_g2l_r(const int* desc, const DdlaHandle_t& handle, int gindx)
+{
+    int pr = 0, pc = 0, r = -1, c = -1;
+    if (!desc_detail::grid(handle, pr, pc, r, c) || gindx >= desc[DDLA_M_] ||
+        r != indxg2p(gindx, desc[DDLA_MB_], desc[DDLA_RSRC_], pr))
+        return -1;
+    return indxg2l(gindx, desc[DDLA_MB_], pr);
+}
+inline int indx_g2l_c(const int* desc, const DdlaHandle_t& handle, int gindx)
+{
+    int pr = 0, pc = 0, r = -1, c = -1;
+    if (!desc_detail::grid(handle, pr, pc, r, c) || gindx >= desc[DDLA_N_] ||
+        c != indxg2p(gindx, desc[DDLA_NB_], desc[DDLA_CSRC_], pc))
+        return -1;
+    return indxg2l(gindx, desc[DDLA_NB_], pc);
+}
+inline int indx_l2g_r(const int* desc, const DdlaHandle_t& handle, int lindx)
+{
+    int pr = 0, pc = 0, r = -1, c = -1;
+    if (!desc_detail::grid(handle, pr, pc, r, c)) return -1;
+    return indxl2g(lindx, desc[DDLA_MB_], r, desc[DDLA_RSRC_], pr);
+}
+inline int indx_l2g_c(const int* desc, const DdlaHandle_t& handle, int lindx)
+{
+    int pr = 0, pc = 0, r = -1, c = -1;
+    if (!desc_detail::grid(handle, pr, pc, r, c)) return -1;
+    return indxl2g(lindx, desc[DDLA_NB_], c, desc[DDLA_CSRC_], pc);
+}
+
 template <class T> struct real_type { using type = T; };
 template <class R> struct real_type<std::complex<R>> { using type = R; };
 

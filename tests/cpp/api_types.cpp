@@ -10,7 +10,8 @@
 //     reference: pinned host memory, ddlaMemsetTri / ddlaMemcpyTri in every
 //     direction (both triangles, with and without the diagonal, padding rows
 //     and the other triangle untouched, a 3-byte element type), events, device
-//     queries and the rank <-> grid maps.
+//     queries, the rank <-> grid maps and the descriptor-level index maps
+//     (include/ddla/ddla_desc.h:143-191).
 // "api_types cpu" runs the index checks only (no GPU); no argument runs both.
 // Prints one line per failure and exits non-zero if any.
 #include <ddla/ddla.h>
@@ -130,6 +131,13 @@ static void handle_helpers_on_gpu(ddla::DdlaHandle_t h)
     EXPECT(row == 0 && col == 0 && ddla::ddlaRcToRank(h, 0, 0) == 0);
     ddla::ddlaRankToRc(nullptr, 0, row, col);
     EXPECT(row == -1 && col == -1 && ddla::ddlaRcToRank(nullptr, 0, 0) == -1);
+    // descriptor-level maps at a 1 x 1 grid: identities inside M x N, -1 beyond
+    int desc[ddla::DDLA_DLEN_];
+    DDLA_CHECK(ddla::ddlaDescInit(desc, h, 100, 70, 16, 16, 0, 0));
+    for (int g = 0; g < 100; ++g) EXPECT(ddla::indx_g2l_r(desc, h, g) == g && ddla::indx_l2g_r(desc, h, g) == g);
+    for (int g = 0; g < 70; ++g) EXPECT(ddla::indx_g2l_c(desc, h, g) == g && ddla::indx_l2g_c(desc, h, g) == g);
+    EXPECT(ddla::indx_g2l_r(desc, h, 100) == -1 && ddla::indx_g2l_c(desc, h, 70) == -1);
+    EXPECT(ddla::indx_g2l_r(desc, nullptr, 0) == -1 && ddla::indx_l2g_c(desc, nullptr, 0) == -1);
 
     double* pinned = nullptr;
     DDLA_CHECK(ddla::ddlaMallocHost(h, &pinned, 64 * sizeof(double)));
