@@ -2,7 +2,7 @@
 (libddla_amd.autograd).  References, all FP64 on the CPU:
   * torch's own autograd of torch.linalg.cholesky / cholesky_solve / logdet
     (the same symmetric-input convention), relative error <= 1e-10;
-  * the Gaussian-process likelihood (config 5: n = 4096 points uniform in
+  * the Gaussian-process likelihood (n = 4096 points uniform in
     [0, 1]^3, seed 4, RBF l = 0.2, s = 1, jitter 1e-2): dLoss/dK against the
     closed form (K^-1 - alpha alpha^T) / 2 and dLoss/dl against
     sum((K^-1 - alpha alpha^T) / 2 * dK/dl) (Rasmussen & Williams 2006,
@@ -109,7 +109,7 @@ GP_LOSSES = {"fused": ag.gp_negative_log_likelihood, "composed": ag.gp_negative_
 
 
 @pytest.mark.parametrize("which", sorted(GP_LOSSES))
-def test_gp_likelihood_gradient_config5(handle, which):
+def test_gp_likelihood_gradient_n4096(handle, which):
     """Both the fused Function (closed-form backward through potri) and the
     composed Cholesky / solve / logdet Functions."""
     print(f"case gp likelihood {which}", flush=True)

@@ -185,7 +185,7 @@ def test_batched_invalid_arguments(handle):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_config4_full_batch(handle, dtype):
+def test_full_batch_4096_members(handle, dtype):
     """4096 members, orders uniform in [32, 512] (seed 3), A_i = G G^T / n + 0.5 I,
     one right-hand side each: every member succeeds; a sample of 64 members has
     backward error ||b - A x||_inf / (||A||_inf ||x||_inf) <= 64 n eps and a

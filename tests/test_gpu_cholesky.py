@@ -120,8 +120,8 @@ def test_factor_matches_oracle_seeded(handle, n, uplo):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
 
 
-def test_pposv_config1(handle):
-    """Config 1: n=2048, nrhs=16, A = G G^T / n + I (seed 0), B seed-0 normals
+def test_pposv_n2048_nrhs16(handle):
+    """n=2048, nrhs=16, A = G G^T / n + I (seed 0), B seed-0 normals
     (stream 1); backward error ||AX-B|| / (||A|| ||X|| n eps) and the oracle's
     solution."""
     n, nrhs = 2048, 16
@@ -340,7 +340,7 @@ def test_cholesky_logdet_matches_slogdet(handle):
 
 @pytest.mark.parametrize("n", [8192])
 def test_full_size_pposv_properties(handle, n):
-    """Config 2 at full size: backward error, and sampled entries of L L^T
+    """FP64 pposv at n = 8192, nrhs = 64: backward error, and sampled entries of L L^T
     against A (a size-independent reconstruction check)."""
     print(f"case full-size pposv n={n}", flush=True)
     nrhs = 64
