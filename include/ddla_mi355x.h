@@ -64,6 +64,13 @@ int ddla_c_comm_size(int f_comm);                                   /* :319-326 
 void ddla_c_finalize(void);                                         /* :328-337 */
 int ddla_c_bcast_dims(int f_comm, int* m, int* n, int* type_char, int root); /* :339-349 */
 int ddla_c_synchronize(void* handle);                               /* :267-271 */
+/* Spin limit of the bounded waits inside the persistent kernels (panel
+ * factorization, triangular-solve sweeps); no counterpart in the reference
+ * (its vendor routines have no such waits).  Process-wide: returns the previous
+ * limit; spins <= 0 restores the default (2^23 polls).  A wait that exceeds it
+ * makes the call fail with status 6 ("... wait limit ..."); a diagnostic for
+ * hung-looking runs and for tests of that error path. */
+int ddla_c_set_wait_limit(int spins);
 /* ddlaSetStream / ddlaGetStream (include/ddla/ddla_handle_t.h:300, :337);
  * no C entry in the reference. */
 int ddla_c_set_stream(void* handle, void* stream);

@@ -188,6 +188,8 @@ int ddla_c_communication_available(void* handle, int communication)
     return ddla::ddlaCommunicationAvailable(H(handle), static_cast<ddla::DdlaCommunication>(communication)) ? 1 : 0;
 }
 
+int ddla_c_set_wait_limit(int spins) { return ddla::kern::set_wait_limit(spins); }
+
 // One process: the communicator helpers describe a world of one rank.
 int ddla_c_default_comm() { return 0; }
 int ddla_c_comm_rank(int) { return 0; }

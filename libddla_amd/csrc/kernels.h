@@ -8,6 +8,15 @@
 namespace ddla {
 namespace kern {
 
+// Spin limit of every bounded wait in the persistent kernels (panel
+// factorization, triangular-solve sweeps): a wait that polls more often sets
+// the launch's error word and gives up, and the host reports the error.
+// Process-wide; kDefaultWaitLimit unless changed (tests force the error path
+// with a tiny limit).
+constexpr int kDefaultWaitLimit = 1 << 23;
+int wait_limit();
+int set_wait_limit(int spins);  // returns the previous limit; spins <= 0 restores the default
+
 // Triangle mask applied by the GEMM epilogue, in the coordinates of the
 // matrix C that is written: element (i, j) of C is written only if
 //   kTriNone, or kTriLower and i + diag >= j, or kTriUpper and i + diag <= j.

@@ -13,6 +13,7 @@
 #include "batched_tile.h"
 #include "split_tile.h"
 
+#include <atomic>
 #include <cfloat>
 #include <cstdint>
 #include <climits>
@@ -520,6 +521,17 @@ void gemm_vbatched(hipStream_t s, char ta, char tb, const int* m, const int* n, 
     launch_vbatched_cfg<T, 64, 64, 16>(s, !is_n(ta), !is_n(tb), v, batch, max_m, max_n);
 }
 
+namespace {
+std::atomic<int> g_wait_limit{kDefaultWaitLimit};
+} // namespace
+
+int wait_limit() { return g_wait_limit.load(std::memory_order_relaxed); }
+
+int set_wait_limit(int spins)
+{
+    return g_wait_limit.exchange(spins > 0 ? spins : kDefaultWaitLimit, std::memory_order_relaxed);
+}
+
 int split_rows(int m) { return (m + split_detail::kTile - 1) / split_detail::kTile * split_detail::kTile + split_detail::kTile; }
 
 size_t split_buffer_bytes(int n, int products)
@@ -634,7 +646,7 @@ void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, 
     if (kb > kMaxCols * kB) throw std::invalid_argument("panel_factor: panel wider than 768 columns");
     if (k % kB != 0) throw std::invalid_argument("panel_factor: panel must start on a 64-column boundary");
     PanelArgs<T> a{A, lda, upper ? 1 : 0, n, k, kb, info, static_cast<T*>(work),
-                   panel_sync<T>(work) + (size_t)(k / kB) * kSyncInts, panel_error_word<T>(n, work), 1 << 23};
+                   panel_sync<T>(work) + (size_t)(k / kB) * kSyncInts, panel_error_word<T>(n, work), wait_limit()};
     const int rows = (n - k + kB - 1) / kB;
     panel_kernel<T><<<rows, 256, 0, s>>>(a);
     check_launch("panel_factor");
@@ -805,7 +817,7 @@ void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
     using namespace solve_detail;
     const int nblk = (n + kSB - 1) / kSB, slabs = (nrhs + kSB - 1) / kSB;
     if (slabs > 65535) throw std::invalid_argument("potrs: too many right-hand sides for one launch");
-    SolveArgs<T> a{n, nrhs, nblk, L, ldl, upper ? 1 : 0, dinv, B, ldb, static_cast<int*>(work), error, 1 << 23};
+    SolveArgs<T> a{n, nrhs, nblk, L, ldl, upper ? 1 : 0, dinv, B, ldb, static_cast<int*>(work), error, wait_limit()};
     const dim3 grid((unsigned)nblk, (unsigned)slabs), block(kThreads);
     zero_ints_kernel<<<1, 256, 0, s>>>(a.sync, 2 * slabs);
     check_launch("potrs");
@@ -825,7 +837,7 @@ void trsm_sweep(hipStream_t s, bool upper, bool backward, int n, int nrhs, const
     using namespace solve_detail;
     const int nblk = (n + kSB - 1) / kSB, slabs = (nrhs + kSB - 1) / kSB;
     if (slabs > 65535) throw std::invalid_argument("trsm_sweep: too many right-hand sides for one launch");
-    SolveArgs<T> a{n, nrhs, nblk, L, ldl, upper ? 1 : 0, dinv, B, ldb, static_cast<int*>(work), error, 1 << 23};
+    SolveArgs<T> a{n, nrhs, nblk, L, ldl, upper ? 1 : 0, dinv, B, ldb, static_cast<int*>(work), error, wait_limit()};
     const dim3 grid((unsigned)nblk, (unsigned)slabs), block(kThreads);
     zero_ints_kernel<<<1, 256, 0, s>>>(a.sync, 2 * slabs);
     check_launch("trsm_sweep");

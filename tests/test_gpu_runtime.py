@@ -11,7 +11,10 @@
     invalid one is status 2 and keeps the previous mode, the default is CCL,
     the query is false for invalid modes and null handles;
   * the single-process communicator helpers: rank 0 of a world of 1, and a
-    dimension broadcast that leaves its arguments as they are.
+    dimension broadcast that leaves its arguments as they are;
+  * the wait-limit error path of the persistent kernels, forced from the host
+    with a spin limit of one poll (no kernel faults or hangs: every wait is
+    bounded and every workgroup exits).
 """
 import ctypes
 
@@ -97,3 +100,41 @@ def test_single_process_communicator_helpers():
     m, n, t = ctypes.c_int(5), ctypes.c_int(6), ctypes.c_int(ord("d"))
     assert dd.lib.ddla_c_bcast_dims(comm, ctypes.byref(m), ctypes.byref(n), ctypes.byref(t), 0) == 0
     assert (m.value, n.value, t.value) == (5, 6, ord("d"))
+
+
+def test_wait_limit_error_is_reported_and_handle_recovers(handle):
+    """The bounded waits of the persistent kernels, forced from the host: with
+    a spin limit of 1 poll the solve sweeps of ppotrs (order 4096, 64 blocks)
+    and the panel factorization of ppotrf give up on a predecessor that is
+    still computing, every workgroup exits, and the call fails with the wait
+    error.  With the default restored the same handle solves correctly."""
+    import pytest as _pytest
+
+    from oracle import oracle as orc
+
+    n, nrhs = 4096, 8
+    a = orc.spd_matrix(n, seed=9, delta=0.5)
+    b = orc.normal_matrix(9, 1, n, nrhs)
+    A = dd.DistributedMatrix.from_global_matrix(handle, a, mb=64, nb=64)
+    _, info = dd.ppotrf(A, "L")
+    assert info == 0
+    B = dd.DistributedMatrix.from_global_matrix(handle, b, mb=64, nb=64)
+    previous = dd.lib.ddla_c_set_wait_limit(1)
+    try:
+        assert previous == 1 << 23
+        print("case forced wait limit: ppotrs", flush=True)
+        with _pytest.raises(RuntimeError, match="wait limit"):
+            dd.ppotrs(A, B, "L")
+        print("case forced wait limit: ppotrf", flush=True)
+        A2 = dd.DistributedMatrix.from_global_matrix(handle, a, mb=64, nb=64)
+        with _pytest.raises(RuntimeError, match="wait exceeded its limit"):
+            dd.ppotrf(A2, "L")
+    finally:
+        assert dd.lib.ddla_c_set_wait_limit(0) == 1
+    assert dd.lib.ddla_c_set_wait_limit(0) == 1 << 23
+    A3 = dd.DistributedMatrix.from_global_matrix(handle, a, mb=64, nb=64)
+    B3 = dd.DistributedMatrix.from_global_matrix(handle, b, mb=64, nb=64)
+    X, info = dd.pposv(A3, B3, "L")
+    assert info == 0
+    x = X.to_global_matrix()
+    assert orc.backward_error(a, x, b) < 1e-14
