@@ -124,13 +124,13 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
                    const SplitPlan* split = nullptr)
 {
     if (upper) {
-        // In the upper-storage view every trailing update reads its operands
-        // along k (transposed products), which the matrix-core kernels run at
-        // about half the rate of the lower view.  So the two strict triangles
-        // are exchanged in place, the lower one (now A's upper triangle
-        // transposed) is factored, and the exchange is undone: the factor
-        // lands in the upper triangle and the caller's strict lower triangle
-        // is restored bit for bit (two HBM passes over the matrix).
+        // The factorization's kernels are tuned for the lower view; run in
+        // the upper view (transposed operand access throughout) the same
+        // factorization took about twice as long.  So the two strict
+        // triangles are exchanged in place, the lower one (now A's upper
+        // triangle transposed) is factored, and the exchange is undone: the
+        // factor lands in the upper triangle and the caller's strict lower
+        // triangle is restored bit for bit (two HBM passes over the matrix).
         const double bytes = 2.0 * n * (n - 1) * sizeof(T);
         {
             LaunchTimer timer(h, kClassOther, 0, bytes);
