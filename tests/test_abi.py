@@ -93,3 +93,21 @@ def test_numloc_matches_block_cyclic_count():
                     for iproc in range(nprocs):
                         assert lib.ddla_c_numloc(n, nb, iproc, src, nprocs) == numroc_bruteforce(
                             n, nb, iproc, src, nprocs)
+
+
+def test_wait_limit_setter_returns_previous_and_restores_default():
+    """Process-wide spin limit of the persistent kernels' waits: no GPU call."""
+    default = 1 << 23
+    assert lib.ddla_c_set_wait_limit(5) == default
+    assert lib.ddla_c_set_wait_limit(7) == 5
+    assert lib.ddla_c_set_wait_limit(0) == 7  # <= 0 restores the default
+    assert lib.ddla_c_set_wait_limit(-3) == default
+    assert lib.ddla_c_set_wait_limit(0) == default
+
+
+def test_single_process_communicator_entries():
+    """The communicator helpers of a one-process build need no GPU."""
+    comm = lib.ddla_c_default_comm()
+    assert lib.ddla_c_comm_rank(comm) == 0 and lib.ddla_c_comm_size(comm) == 1
+    assert lib.ddla_c_communication_available(None, 2) == 0
+    assert lib.ddla_c_set_communication(None, 2) == 1
