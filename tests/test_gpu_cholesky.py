@@ -359,3 +359,65 @@ def test_full_size_pposv_properties(handle, n):
     j = rng.randint(0, n, 3000)
     rec = np.einsum("ij,ij->i", l[i, :], l[j, :])
     assert np.abs(rec - a[i, j]).max() < 1e-13 * n
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n", [1, 130, 1100])
+def test_local_potrf_device_info(handle, n, uplo, dtype):
+    """The local potrf (include/ddla/ddla.h:2160 of the reference; C entry
+    ddla_{s,d}_potrf): asynchronous on the handle's stream with a device info
+    word.  The factor against the oracle (1e-12 / 1e-4 relative to max |L|),
+    the NaN-poisoned other triangle and the padding rows (lda = n + 3) left bit
+    for bit, info 0; a planted negative pivot reports its 1-based row."""
+    import torch
+
+    print(f"case local potrf n={n} uplo={uplo} {np.dtype(dtype).name}", flush=True)
+    lda = n + 3
+    a = orc.spd_matrix(n, seed=n + 11, delta=0.2)
+    keep = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    buf = np.full((lda, n), -5.0)
+    buf[:n, :n] = np.where(keep, a, np.nan)
+    buf = buf.astype(dtype)
+    t = torch.tensor(np.ascontiguousarray(buf.T), device="cuda")
+    info = torch.full((1,), -7, dtype=torch.int32, device="cuda")
+    fn = dd.lib.ddla_d_potrf if dtype == np.float64 else dd.lib.ddla_s_potrf
+    torch.cuda.synchronize()
+    assert fn(handle.ptr, uplo.encode(), n, ctypes.c_void_p(t.data_ptr()), lda, ctypes.c_void_p(info.data_ptr())) == 0
+    handle.synchronize()
+    assert int(info.item()) == 0
+    out = t.cpu().numpy().T
+    want, winfo = orc.potrf(uplo, a.astype(dtype))
+    assert winfo == 0
+    err = np.abs(out[:n, :n][keep] - want[keep]).max() / np.abs(want[keep]).max()
+    assert err < (1e-12 if dtype == np.float64 else 1e-4), err
+    assert np.isnan(out[:n, :n][~keep]).all()
+    assert (out[n:, :] == buf[n:, :]).all()
+    if n > 1:
+        bad = np.where(keep, a, 0.0).astype(dtype)
+        r = n // 2
+        bad[r, r] = -1.0
+        t2 = torch.tensor(np.ascontiguousarray(np.pad(bad, ((0, 3), (0, 0))).T), device="cuda")
+        torch.cuda.synchronize()
+        assert fn(handle.ptr, uplo.encode(), n, ctypes.c_void_p(t2.data_ptr()), lda,
+                  ctypes.c_void_p(info.data_ptr())) == 0
+        handle.synchronize()
+        _, oinfo = orc.potrf(uplo, bad)
+        assert int(info.item()) == oinfo and 1 <= oinfo <= r + 1
+
+
+def test_local_potrf_argument_rules(handle):
+    """n = 0 sets info to 0 without touching A; lda < n, a null info word and
+    a bad uplo are errors (status 6)."""
+    import torch
+
+    info = torch.full((1,), -7, dtype=torch.int32, device="cuda")
+    t = torch.eye(4, dtype=torch.float64, device="cuda")
+    pi, pt = ctypes.c_void_p(info.data_ptr()), ctypes.c_void_p(t.data_ptr())
+    torch.cuda.synchronize()
+    assert dd.lib.ddla_d_potrf(handle.ptr, b"L", 0, None, 1, pi) == 0
+    handle.synchronize()
+    assert int(info.item()) == 0
+    assert dd.lib.ddla_d_potrf(handle.ptr, b"L", 4, pt, 3, pi) == 6
+    assert dd.lib.ddla_d_potrf(handle.ptr, b"L", 4, pt, 4, None) == 6
+    assert dd.lib.ddla_d_potrf(handle.ptr, b"X", 4, pt, 4, pi) == 6
