@@ -14,7 +14,10 @@
     dimension broadcast that leaves its arguments as they are;
   * the wait-limit error path of the persistent kernels, forced from the host
     with a spin limit of one poll (no kernel faults or hangs: every wait is
-    bounded and every workgroup exits).
+    bounded and every workgroup exits);
+  * the three-phase ppotrs (workspace query, run on caller scratch) against
+    the driver, bit for bit; the stream hand-over of ddla_c_set_stream; the
+    device normal generator against the oracle's convention.
 """
 import ctypes
 
@@ -138,3 +141,85 @@ def test_wait_limit_error_is_reported_and_handle_recovers(handle):
     assert info == 0
     x = X.to_global_matrix()
     assert orc.backward_error(a, x, b) < 1e-14
+
+
+def test_ppotrs_three_phase_matches_driver(handle):
+    """ppotrs_workspace_size + ppotrs_run on caller scratch give the driver's
+    result bit for bit; a workspace below the query is rejected."""
+    import torch
+
+    from oracle import oracle as orc
+
+    n, nrhs = 700, 9
+    a = orc.spd_matrix(n, seed=12, delta=0.3)
+    b = orc.normal_matrix(12, 1, n, nrhs)
+    A = dd.DistributedMatrix.from_global_matrix(handle, a, mb=64, nb=64)
+    _, info = dd.ppotrf(A, "L")
+    assert info == 0
+    B1 = dd.DistributedMatrix.from_global_matrix(handle, b, mb=64, nb=64)
+    B2 = dd.DistributedMatrix.from_global_matrix(handle, b, mb=64, nb=64)
+    dd.ppotrs(A, B1, "L")
+    bb, hb = ctypes.c_longlong(), ctypes.c_longlong()
+    assert dd.lib.ddla_d_ppotrs_workspace_size(handle.ptr, b"L", n, nrhs, A.desc_ptr, B2.desc_ptr, ctypes.byref(bb),
+                                               ctypes.byref(hb)) == 0
+    assert bb.value > 0 and hb.value >= 0
+    work = torch.empty(bb.value, dtype=torch.uint8, device="cuda")
+    host = (ctypes.c_char * max(1, hb.value))()
+    torch.cuda.synchronize()
+    st = dd.lib.ddla_d_ppotrs_run(handle.ptr, b"L", b"L", b"N", n, nrhs, ctypes.c_void_p(A.data_ptr), A.desc_ptr,
+                                  ctypes.c_void_p(B2.data_ptr), B2.desc_ptr, ctypes.c_void_p(work.data_ptr()),
+                                  bb.value, ctypes.cast(host, ctypes.c_void_p), hb.value)
+    assert st == 0
+    handle.synchronize()
+    assert B2.to_global_matrix().tobytes() == B1.to_global_matrix().tobytes()
+    st = dd.lib.ddla_d_ppotrs_run(handle.ptr, b"L", b"L", b"N", n, nrhs, ctypes.c_void_p(A.data_ptr), A.desc_ptr,
+                                  ctypes.c_void_p(B2.data_ptr), B2.desc_ptr, ctypes.c_void_p(work.data_ptr()),
+                                  bb.value - 256, ctypes.cast(host, ctypes.c_void_p), hb.value)
+    assert st == 6
+
+
+def test_set_stream_takes_ownership_and_null_restores_default():
+    """ddla_c_set_stream (include/ddla/ddla_handle_t.h:286-300 of the reference):
+    the handle takes over a stream created by the caller, runs on it, and
+    destroys it when replaced by the null (default) stream."""
+    from oracle import oracle as orc
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    h = dd.Handle(dd.Backend.GPU)
+    try:
+        h.set_stream(stream.value)
+        assert h.stream == stream.value
+        n, nrhs = 300, 4
+        a = orc.spd_matrix(n, seed=13, delta=0.3)
+        b = orc.normal_matrix(13, 1, n, nrhs)
+        A = dd.DistributedMatrix.from_global_matrix(h, a, mb=64, nb=64)
+        B = dd.DistributedMatrix.from_global_matrix(h, b, mb=64, nb=64)
+        _, info = dd.pposv(A, B, "L")
+        assert info == 0
+        assert orc.backward_error(a, B.to_global_matrix(), b) < 1e-14
+        A.free()
+        B.free()
+        h.set_stream(0)  # the handle destroys the stream it owned
+        assert h.stream == 0
+    finally:
+        h.destroy()
+
+
+def test_device_normals_match_the_oracle(handle):
+    """ddla_c_fill_normal (the benchmark's generator) against the oracle's
+    restatement of the same Philox4x64-10 + Box-Muller convention, element by
+    element (the device and host libm differ by at most a few ulps)."""
+    import torch
+
+    from oracle import oracle as orc
+
+    for seed, stream, first, count in ((1, 0, 0, 1001), (2, 1, 12345, 777), (7, 9, (1 << 33) + 3, 64)):
+        out = torch.empty(count, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        assert dd.lib.ddla_c_fill_normal(handle.ptr, seed, stream, first, count, ctypes.c_void_p(out.data_ptr())) == 0
+        handle.synchronize()
+        want = orc.fill_normal(seed, stream, first, count)
+        got = out.cpu().numpy()
+        assert np.abs(got - want).max() <= 1e-14 * max(1.0, np.abs(want).max())
