@@ -223,3 +223,34 @@ def test_device_normals_match_the_oracle(handle):
         want = orc.fill_normal(seed, stream, first, count)
         got = out.cpu().numpy()
         assert np.abs(got - want).max() <= 1e-14 * max(1.0, np.abs(want).max())
+
+
+def test_launch_profiler_records_the_schedule(handle):
+    """The per-launch event profiler behind the benchmark's roofline figures:
+    one FP64 pposv at n = 2048 (four 512-column panels, uplo L) records the
+    launches the schedule makes, per class: 4 panels, 1 trailing update on the
+    handle's stream (columns beyond panel p + 2 exist only for p = 0), 5
+    look-ahead updates, 1 solve (both sweeps) and 1 diagonal-block inversion;
+    every duration is positive, the trailing update's flops are its
+    m (m + 1) kb with m = 512, kb = 512."""
+    from oracle import oracle as orc
+
+    n, nrhs = 2048, 4
+    a = orc.spd_matrix(n, seed=14, delta=0.3)
+    b = orc.normal_matrix(14, 1, n, nrhs)
+    A = dd.DistributedMatrix.from_global_matrix(handle, a, mb=512, nb=512)
+    B = dd.DistributedMatrix.from_global_matrix(handle, b, mb=512, nb=512)
+    handle.profile(True)
+    try:
+        _, info = dd.pposv(A, B, "L")
+        prof = handle.profile_read()
+    finally:
+        handle.profile(False)
+    assert info == 0
+    counts = {k: v[0] for k, v in prof.items()}
+    assert counts["panel"] == 4 and counts["trailing_update"] == 1 and counts["lookahead_update"] == 5
+    assert counts["solve"] == 1 and counts["trtri"] == 1
+    for k, (cnt, ms, fl, _) in prof.items():
+        if cnt:
+            assert ms > 0, k
+    assert prof["trailing_update"][2] == 512 * 513 * 512
