@@ -46,6 +46,7 @@ from .ops import (
     pposv_run,
     pposv_workspace_size,
     ppotrs,
+    ppotrs_run,
     ppotrs_workspace_size,
     ptrtrs,
     trsm,
@@ -53,7 +54,7 @@ from .ops import (
 
 __all__ = [
     "LIBRARY_PATH", "lib", "check_status", "Backend", "Handle", "DistributedMatrix", "descinit",
-    "ppotrf", "ppotrs", "pposv", "ptrtrs", "trsm", "potrf_bottom_right", "ppotrf_bottom_right", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size",
+    "ppotrf", "ppotrs", "pposv", "ptrtrs", "trsm", "potrf_bottom_right", "ppotrf_bottom_right", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size", "ppotrs_run",
     "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
     "FACTOR_FP32", "FACTOR_BF16X3", "FACTOR_BF16X6",
     "diag_logdet", "pdiag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",

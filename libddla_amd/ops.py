@@ -118,6 +118,16 @@ def pposv_workspace_size(A, B, uplo: str = "L", side: str = "L"):
     return bb.value, hb.value
 
 
+def ppotrs_run(A, B, uplo, backend_work: int, backend_bytes: int, side: str = "L", trans: str = "N"):
+    """The solve on caller scratch (from ppotrs_workspace_size); asynchronous on the handle's stream."""
+    nrhs = B.n if char(side) == b"L" else B.m
+    st = _fn(A, "ppotrs_run")(A.handle.ptr, char(side), char(uplo), char(trans), A.n, nrhs,
+                              ctypes.c_void_p(A.data_ptr), A.desc_ptr, ctypes.c_void_p(B.data_ptr), B.desc_ptr,
+                              ctypes.c_void_p(backend_work), backend_bytes, ctypes.c_void_p(0), 0)
+    check_status(st, f"ddla_{A.type_char}_ppotrs_run")
+    return B
+
+
 def pposv_run(A, B, uplo, backend_work: int, backend_bytes: int, side: str = "L", trans: str = "N"):
     info = ctypes.c_int(0)
     nrhs = B.n if char(side) == b"L" else B.m

@@ -172,6 +172,10 @@ def test_ppotrs_three_phase_matches_driver(handle):
     assert st == 0
     handle.synchronize()
     assert B2.to_global_matrix().tobytes() == B1.to_global_matrix().tobytes()
+    B3 = dd.DistributedMatrix.from_global_matrix(handle, b, mb=64, nb=64)  # the Python wrapper of the same run
+    dd.ppotrs_run(A, B3, "L", work.data_ptr(), bb.value)
+    handle.synchronize()
+    assert B3.to_global_matrix().tobytes() == B1.to_global_matrix().tobytes()
     st = dd.lib.ddla_d_ppotrs_run(handle.ptr, b"L", b"L", b"N", n, nrhs, ctypes.c_void_p(A.data_ptr), A.desc_ptr,
                                   ctypes.c_void_p(B2.data_ptr), B2.desc_ptr, ctypes.c_void_p(work.data_ptr()),
                                   bb.value - 256, ctypes.cast(host, ctypes.c_void_p), hb.value)
