@@ -13,9 +13,12 @@
 //   potrf_bottom_right(p)   include/ddla/ddla.h:1090, 1115
 //   diag_logdet             include/ddla/ddla.h:2901
 //   check / size / run      src/Cholesky/cholesky_workspace.h:133-249
-// Differences: there is no MPI (ddlaSet takes the grid shape only and
-// accepts 1 x 1), and only the GPU backend exists (ddlaInit with the CPU
-// backend throws, as the reference does for a backend that is not built).
+// Differences: the library itself needs no MPI.  ddlaSet takes the grid
+// shape and accepts 1 x 1; a caller that compiles with MPI (DDLA_USE_MPI
+// defined, or <mpi.h> on the include path) also gets the reference's
+// communicator overloads, inline here, which accept a one-rank communicator.
+// Only the GPU backend exists (ddlaInit with the CPU backend throws, as the
+// reference does for a backend that is not built).
 #ifndef DDLA_MI355X_DDLA_H
 #define DDLA_MI355X_DDLA_H
 
@@ -25,6 +28,15 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+
+#if !defined(DDLA_USE_MPI) && !defined(DDLA_NO_MPI) && defined(__has_include)
+#if __has_include(<mpi.h>)
+#define DDLA_USE_MPI 1
+#endif
+#endif
+#if defined(DDLA_USE_MPI)
+#include <mpi.h>
+#endif
 
 namespace ddla {
 
@@ -50,7 +62,27 @@ using DdlaHandle_t = DdlaStream*;
 // compute stream on the current device); nprows * npcols must be 1 (values
 // <= 0 select the default 1 x 1 grid).
 ddlaStatus_t ddlaInit(DdlaHandle_t& handle, DdlaBackend requested_backend);
+#if defined(DDLA_USE_MPI)
+ddlaStatus_t ddlaSet(DdlaHandle_t handle, int nprows, int npcols, char major = 'R');
+// The reference's signatures (include/ddla/ddla_handle_t.h:58-64).  The caller
+// owns MPI initialization; the communicator must hold exactly one rank (this
+// build runs one process on one GPU), else std::invalid_argument, before any
+// device work.
+inline ddlaStatus_t ddlaSet(DdlaHandle_t handle, const MPI_Comm& comm, int nprows, int npcols, char major = 'R')
+{
+    int size = 0;
+    if (MPI_Comm_size(comm, &size) != MPI_SUCCESS) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
+    if (size != 1)
+        throw std::invalid_argument("ddlaSet: this build runs one process on one GPU; the communicator must hold one rank");
+    return ddlaSet(handle, nprows, npcols, major);
+}
+inline ddlaStatus_t ddlaSet(DdlaHandle_t handle, const MPI_Comm& comm = MPI_COMM_WORLD, char major = 'R')
+{
+    return ddlaSet(handle, comm, 1, 1, major);
+}
+#else
 ddlaStatus_t ddlaSet(DdlaHandle_t handle, int nprows = 1, int npcols = 1, char major = 'R');
+#endif
 ddlaStatus_t ddlaDestroy(DdlaHandle_t& handle);
 bool ddlaBackendAvailable(DdlaBackend backend);
 DdlaBackend ddlaGetBackend(const DdlaHandle_t& handle);
@@ -281,7 +313,9 @@ template <typename T>
 void trsm(const DdlaHandle_t& handle, char side, char uplo, char trans, char diag, int m, int n, const T& alpha,
           T* A, int lda, T* B, int ldb);
 
-// Local factorization with a device-resident info word; asynchronous.
+// Local factorization with a device-resident info word; asynchronous.  *info
+// receives 0, the 1-based index of the first failed pivot, or -1000 when a
+// wait inside the factorization exceeded its spin limit (factor not valid).
 template <typename T>
 void potrf(const DdlaHandle_t& handle, char uplo, int n, T* A, int lda, int* info);
 

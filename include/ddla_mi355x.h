@@ -71,6 +71,15 @@ int ddla_c_synchronize(void* handle);                               /* :267-271 
  * makes the call fail with status 6 ("... wait limit ..."); a diagnostic for
  * hung-looking runs and for tests of that error path. */
 int ddla_c_set_wait_limit(int spins);
+/* Panel width of the blocked factorization (the depth of its trailing
+ * updates); no counterpart in the reference (its vendor potrf picks its own
+ * blocking).  Process-wide: cols <= 0 restores the built-in rule (768 columns
+ * for FP64 from order 16384, else 512); other values are rounded down to a
+ * multiple of 64 within [64, 768] (the bf16 split-product factorization caps
+ * it at 512).  Returns the previous override (0 = none).  The factor does not
+ * depend on it beyond rounding; it exists so that tests can run the wide
+ * panels of large orders on small matrices. */
+int ddla_c_set_panel_width(int cols);
 /* ddlaSetStream / ddlaGetStream (include/ddla/ddla_handle_t.h:300, :337);
  * no C entry in the reference. */
 int ddla_c_set_stream(void* handle, void* stream);
@@ -108,8 +117,9 @@ int ddla_d_trsm(void* handle, char side, char uplo, char trans, char diag, int m
  * cholesky_impl::{ppotrf,ppotrs,pposv}_{workspace_size,run} of
  * src/Cholesky/cholesky_workspace.h:133-249.  The size query allocates
  * nothing; a run validates nothing beyond the workspace size and allocates
- * nothing (ppotrf/pposv run synchronize to read info; ppotrs run is
- * asynchronous). -------------------------------------------------------- */
+ * nothing.  Every run returns synchronized: ppotrf/pposv run read info, and
+ * all three read the device error word of the persistent kernels' bounded
+ * waits and fail with status 6 ("... wait limit ...") if it is set. ------ */
 int ddla_s_ppotrf_workspace_size(void* handle, char uplo, int n, const int* descA, long long* backend_bytes,
                                  long long* host_bytes);
 int ddla_d_ppotrf_workspace_size(void* handle, char uplo, int n, const int* descA, long long* backend_bytes,
@@ -157,7 +167,12 @@ int ddla_d_pposv_mixed_run(void* handle, char uplo, int n, int nrhs, void* A, co
                            const int* descB, int factor_precision, double tol, int max_iter, int* iter, int* info,
                            void* backend_work, long long backend_bytes);
 
-/* ---- local potrf, device info (C++ ddla::potrf, include/ddla/ddla.h:2160) */
+/* ---- local potrf, device info (C++ ddla::potrf, include/ddla/ddla.h:2160).
+ * Asynchronous on the handle's stream; *d_info (device) receives 0, the
+ * 1-based LAPACK index of the first failed pivot, or
+ * DDLA_POTRF_INFO_WAIT_LIMIT when a wait inside the factorization exceeded
+ * the spin limit (ddla_c_set_wait_limit; the factor is then not valid). --- */
+#define DDLA_POTRF_INFO_WAIT_LIMIT (-1000)
 int ddla_s_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
 int ddla_d_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
 

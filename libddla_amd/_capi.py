@@ -97,6 +97,7 @@ _SIGNATURES = {
     "ddla_c_memset": (_i, [_vp, _vp, _i, _ll]),
     "ddla_c_memcpy2d": (_i, [_vp, _vp, _ll, _vp, _ll, _ll, _ll]),
     "ddla_c_set_wait_limit": (_i, [_i]),
+    "ddla_c_set_panel_width": (_i, [_i]),
     "ddla_c_set_communication": (_i, [_vp, _i]),
     "ddla_c_get_communication": (_i, [_vp, ctypes.POINTER(_i)]),
     "ddla_c_communication_available": (_i, [_vp, _i]),

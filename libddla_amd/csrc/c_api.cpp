@@ -189,6 +189,7 @@ int ddla_c_communication_available(void* handle, int communication)
 }
 
 int ddla_c_set_wait_limit(int spins) { return ddla::kern::set_wait_limit(spins); }
+int ddla_c_set_panel_width(int cols) { return ddla::set_panel_width(cols); }
 
 // One process: the communicator helpers describe a world of one rank.
 int ddla_c_default_comm() { return 0; }

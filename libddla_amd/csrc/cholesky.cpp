@@ -15,6 +15,7 @@
 #include "internal.h"
 #include "kernels.h"
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <limits>
@@ -38,7 +39,16 @@ constexpr int kIB = 64;  // diagonal block of potf2 and of the solves
 // n = 32768 ran 209.3 / 205.0 / 206.4 ms with 512 / 768 / 1024, while at
 // n = 4096 (panel-bound) 512 is best.  The split-product path keeps 512 (its
 // panel buffers hold 512 columns).
-template <class T> int outer_block(int n) { return std::is_same<T, double>::value && n >= 16384 ? 768 : 512; }
+// A process-wide override (set_panel_width; 0 = the rule above) lets tests
+// run the 768-column panels at small orders.
+std::atomic<int> g_panel_width{0};
+template <class T>
+int outer_block(int n, bool split)
+{
+    const int w = g_panel_width.load(std::memory_order_relaxed);
+    if (w > 0) return split ? std::min(w, 512) : w;
+    return std::is_same<T, double>::value && n >= 16384 ? 768 : 512;
+}
 
 // The blocked factorization always runs on the lower triangle (uplo 'U' is
 // brought there by potrf_blocked): element (i, j), i >= j, of L is
@@ -142,7 +152,7 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
         return;
     }
     const LView<T> v{A, lda};
-    const int NB = outer_block<T>(n);
+    const int NB = outer_block<T>(n, split != nullptr);
     const int np = (n + NB - 1) / NB;
     hipStream_t main = h->stream, side = h->side, look = h->look;
     auto col = [&](int q) { return std::min(n, q * NB); };  // first column of panel q (n past the end)
@@ -214,6 +224,12 @@ void bad_workspace(const char* who)
 char upper_char(char c) { return (c >= 'a' && c <= 'z') ? char(c - 'a' + 'A') : c; }
 
 } // namespace
+
+int set_panel_width(int cols)
+{
+    const int w = cols <= 0 ? 0 : std::min(768, std::max(64, cols / 64 * 64));
+    return g_panel_width.exchange(w, std::memory_order_relaxed);
+}
 
 namespace cholesky_impl {
 

@@ -116,4 +116,11 @@ inline T* take(char*& cursor, std::size_t count)
 // Launch helpers shared by the Cholesky schedules.
 void fill_int(hipStream_t s, int* p, int value);
 
+// Panel width (the trailing update's depth) of the blocked factorization,
+// process-wide: cols <= 0 restores the built-in rule (768 columns for FP64
+// from order 16384, else 512); other values are rounded down to a multiple of
+// 64 within [64, 768] (the split-product path caps it at 512).  Returns the
+// previous override (0 = none).
+int set_panel_width(int cols);
+
 } // namespace ddla

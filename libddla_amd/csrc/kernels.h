@@ -14,6 +14,9 @@ namespace kern {
 // Process-wide; kDefaultWaitLimit unless changed (tests force the error path
 // with a tiny limit).
 constexpr int kDefaultWaitLimit = 1 << 23;
+// Device info of a factorization abandoned at the spin limit (the factor is
+// then not valid); negative, so it is never a pivot index.
+constexpr int kInfoWaitLimit = -1000;
 int wait_limit();
 int set_wait_limit(int spins);  // returns the previous limit; spins <= 0 restores the default
 
@@ -55,8 +58,9 @@ void vbatched_validate(hipStream_t s, int na, int nb, const int* m, const int* n
 // A[i + j*lda], or A[j + i*lda] when upper.  panel_factor factors columns
 // [k, k + kb) (kb <= 768, every earlier trailing update applied): the
 // diagonal blocks, their inverses and the rows below.  On a non-positive or
-// NaN pivot in global column p (0-based) it sets *info = p + 1; it does
-// nothing if *info != 0 on entry.  work holds panel_work_bytes<T>(n) bytes,
+// NaN pivot in global column p (0-based) it sets *info = p + 1, and
+// kInfoWaitLimit if one of its waits gave up (unless *info was set first); it
+// does nothing if *info != 0 on entry.  work holds panel_work_bytes<T>(n) bytes,
 // prepared once per factorization by panel_reset (one fill kernel); *error
 // (the last int of work, see panel_error_word) is set if an internal wait
 // ever exceeded its limit.

@@ -50,7 +50,8 @@ struct PanelArgs {
     long long lda;
     int upper;
     int n, k, kb;
-    int* info;       // device info: the panel is skipped if nonzero on entry; receives k + 64 t + j (1-based)
+    int* info;       // device info: the panel is skipped if nonzero on entry; receives k + 64 t + j (1-based),
+                     // or kInfoWaitLimit when a wait gave up
     T* inv;          // kMaxCols inverses D_j, 64 x 64 column-major each
     int* sync;       // this panel's kSyncInts counters, zero on entry
     int* error;      // set to 1 when a wait exceeds spin_limit
@@ -209,6 +210,10 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> a)
                 if (++spins > a.spin_limit) {
                     st = -1;
                     atomicExch(a.error, 1);
+                    // the factor is not valid: the device info says so too
+                    // (unless a failed pivot was recorded first), and every
+                    // later panel and update of this factorization is skipped
+                    atomicCAS(a.info, 0, kInfoWaitLimit);
                     atomicExch(failed, 1);
                     break;
                 }

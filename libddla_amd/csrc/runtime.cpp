@@ -292,6 +292,10 @@ ddlaStatus_t ddlaSetStream(const DdlaHandle_t& handle, void* stream)
     if (!handle->initialized) return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
     hipStream_t next = static_cast<hipStream_t>(stream);
     if (next == handle->stream) return ddlaStatus_t::DDLA_STATUS_SUCCESS;
+    // The look-ahead streams are the handle's own: taking one as the compute
+    // stream would destroy it twice and serialize the look-ahead.
+    if (next != nullptr && (next == handle->side || next == handle->look))
+        return ddlaStatus_t::DDLA_STATUS_INVALID_VALUE;
     DDLA_HIP(hipStreamSynchronize(handle->stream));
     if (handle->owns_stream && handle->stream != nullptr) DDLA_HIP(hipStreamDestroy(handle->stream));
     handle->stream = next;

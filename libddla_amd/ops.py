@@ -119,7 +119,8 @@ def pposv_workspace_size(A, B, uplo: str = "L", side: str = "L"):
 
 
 def ppotrs_run(A, B, uplo, backend_work: int, backend_bytes: int, side: str = "L", trans: str = "N"):
-    """The solve on caller scratch (from ppotrs_workspace_size); asynchronous on the handle's stream."""
+    """The solve on caller scratch (from ppotrs_workspace_size).  Returns synchronized; raises
+    DdlaExceptionError if a triangular-solve sweep exceeded its wait limit."""
     nrhs = B.n if char(side) == b"L" else B.m
     st = _fn(A, "ppotrs_run")(A.handle.ptr, char(side), char(uplo), char(trans), A.n, nrhs,
                               ctypes.c_void_p(A.data_ptr), A.desc_ptr, ctypes.c_void_p(B.data_ptr), B.desc_ptr,

@@ -17,6 +17,8 @@
 #include <ddla/ddla.h>
 #include <ddla/ddla_vector.h>
 
+#include "internal.h"  // white-box: the handle's own look-ahead streams
+
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -236,6 +238,13 @@ static void handle_helpers_on_gpu(ddla::DdlaHandle_t h)
     DDLA_CHECK(ddla::ddlaFreeHost(h, nullptr));
     DDLA_CHECK(ddla::ddlaEventDestroy(h, ev));
     DDLA_CHECK(ddla::ddlaDeviceSynchronize(h));
+    // ddlaSetStream refuses the handle's private look-ahead streams (the
+    // reference returns INVALID_VALUE for a queue the handle owns) and keeps
+    // its compute stream.
+    void* before = ddla::ddlaGetStream(h);
+    EXPECT(ddla::ddlaSetStream(h, static_cast<void*>(h->side)) == ddlaStatus_t::DDLA_STATUS_INVALID_VALUE);
+    EXPECT(ddla::ddlaSetStream(h, static_cast<void*>(h->look)) == ddlaStatus_t::DDLA_STATUS_INVALID_VALUE);
+    EXPECT(ddla::ddlaGetStream(h) == before);
 }
 
 int main(int argc, char** argv)

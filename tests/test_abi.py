@@ -105,6 +105,17 @@ def test_wait_limit_setter_returns_previous_and_restores_default():
     assert lib.ddla_c_set_wait_limit(0) == default
 
 
+def test_panel_width_setter_rounds_clamps_and_restores():
+    """Process-wide panel width override of the blocked factorization: no GPU
+    call.  Returns the previous override (0 = the built-in rule)."""
+    assert lib.ddla_c_set_panel_width(768) == 0
+    assert lib.ddla_c_set_panel_width(700) == 768  # rounded down to 640
+    assert lib.ddla_c_set_panel_width(5000) == 640  # clamped to 768
+    assert lib.ddla_c_set_panel_width(10) == 768  # at least 64
+    assert lib.ddla_c_set_panel_width(0) == 64
+    assert lib.ddla_c_set_panel_width(-1) == 0
+
+
 def test_single_process_communicator_entries():
     """The communicator helpers of a one-process build need no GPU."""
     comm = lib.ddla_c_default_comm()

@@ -37,3 +37,27 @@ def test_block_cyclic_index_maps():
 def test_vector_and_handle_helpers_on_gpu():
     print("case ddla::vector and the handle helpers on the GPU handle", flush=True)
     run_driver()
+
+
+MPI_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "api_mpi")
+
+
+def run_mpi_driver(*args):
+    assert os.path.exists(MPI_DRIVER), f"{MPI_DRIVER} is not built (make -C libddla_amd/csrc)"
+    out = subprocess.run([MPI_DRIVER, *args], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "0 failure(s)" in out.stdout
+
+
+def test_communicator_overloads_of_ddlaset():
+    """ddlaSet(handle, const MPI_Comm&, ...) (include/ddla/ddla_handle_t.h:58-64
+    of the reference) compiled as an MPI caller compiles: more than one rank is
+    refused before any device work, one rank forwards to the 1 x 1 grid, and
+    integer grid shapes still pick the grid-shape overload."""
+    run_mpi_driver()
+
+
+@pytest.mark.gpu
+def test_communicator_overloads_set_up_gpu_handles():
+    print("case ddlaSet through a one-rank communicator on the GPU handle", flush=True)
+    run_mpi_driver("gpu")

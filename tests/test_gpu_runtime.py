@@ -112,6 +112,7 @@ def test_wait_limit_error_is_reported_and_handle_recovers(handle):
     still computing, every workgroup exits, and the call fails with the wait
     error.  With the default restored the same handle solves correctly."""
     import pytest as _pytest
+    import torch
 
     from oracle import oracle as orc
 
@@ -132,6 +133,16 @@ def test_wait_limit_error_is_reported_and_handle_recovers(handle):
         A2 = dd.DistributedMatrix.from_global_matrix(handle, a, mb=64, nb=64)
         with _pytest.raises(RuntimeError, match="wait exceeded its limit"):
             dd.ppotrf(A2, "L")
+        # the local potrf is asynchronous: the abandoned factorization shows
+        # in its device info word (DDLA_POTRF_INFO_WAIT_LIMIT)
+        print("case forced wait limit: local potrf", flush=True)
+        t = torch.tensor(np.ascontiguousarray(a.T), device="cuda")
+        d_info = torch.full((1,), -7, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        assert dd.lib.ddla_d_potrf(handle.ptr, b"L", n, ctypes.c_void_p(t.data_ptr()), n,
+                                   ctypes.c_void_p(d_info.data_ptr())) == 0
+        handle.synchronize()
+        assert int(d_info.item()) == -1000
     finally:
         assert dd.lib.ddla_c_set_wait_limit(0) == 1
     assert dd.lib.ddla_c_set_wait_limit(0) == 1 << 23
