@@ -149,6 +149,19 @@ int ddla_d_pposv_run(void* handle, char side, char uplo, char trans, int n, int 
                      const int* descA, void* B, int ib, int jb, const int* descB, int* info, void* backend_work,
                      long long backend_bytes, void* host_work, long long host_bytes);
 
+/* ---- FP64 pposv that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot call hipMalloc (code under stream capture, tight
+ * serving loops, harnesses that forbid allocation).  Same arguments, checks,
+ * info and results as ddla_d_pposv, but the scratch is a static 64 MiB device
+ * array of the library (one per GPU, held until the process ends) instead of
+ * the handle's scratch.  A call that needs more -- the pposv workspace size:
+ * about 512 bytes per row of A, plus 8 n nrhs bytes for side 'R' -- fails
+ * with status 6 and a message naming both sizes.  Every call on one GPU
+ * shares the array: calls that overlap on one GPU (two handles, two threads)
+ * race.  Returns synchronized. */
+int ddla_d_pposv_noalloc(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                         const int* descA, void* B, int ib, int jb, const int* descB, int* info);
+
 /* ---- mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
  * semantics): FP64 A X = B, side 'L', factored in FP32 on the matrix cores
  * (factor_precision 0: FP32 trailing updates; 1: bf16x3 and 2: bf16x6 split

@@ -112,6 +112,7 @@ _SIGNATURES = {
     "ddla_d_spd_generate": (_i, [_vp, _i, ctypes.c_ulonglong, ctypes.c_double, _vp, _i, _vp]),
     "ddla_d_pposv_mixed": (_i, [_vp, _c, _i, _i, _vp, _i, _i, _ip, _vp, _i, _i, _ip, _i, ctypes.c_double, _i, _ip,
                                 _ip]),
+    "ddla_d_pposv_noalloc": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _i, _i, _ip, _vp, _i, _i, _ip, _ip]),
     "ddla_d_pposv_mixed_workspace_size": (_i, [_vp, _c, _i, _i, _ip, _ip, _llp, _llp]),
     "ddla_d_pposv_mixed_run": (_i, [_vp, _c, _i, _i, _vp, _ip, _vp, _ip, _i, ctypes.c_double, _i, _ip, _ip, _vp,
                                     _ll]),

@@ -587,6 +587,18 @@ int ddla_c_graph_destroy(void* graph)
     });
 }
 
+// ---- FP64 pposv on the static scratch (allocates nothing) ---------------------
+int ddla_d_pposv_noalloc(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                         const int* descA, void* B, int ib, int jb, const int* descB, int* info)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if (info == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+        ddla::pposv_noalloc(h, side, uplo, trans, n, nrhs, static_cast<double*>(A), ia, ja, descA,
+                            static_cast<double*>(B), ib, jb, descB, *info);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
 // ---- mixed-precision pposv -----------------------------------------------------
 int ddla_d_pposv_mixed(void* handle, char uplo, int n, int nrhs, void* A, int ia, int ja, const int* descA, void* B,
                        int ib, int jb, const int* descB, int factor_precision, double tol, int max_iter, int* iter,

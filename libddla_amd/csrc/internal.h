@@ -123,4 +123,10 @@ void fill_int(hipStream_t s, int* p, int value);
 // previous override (0 = none).
 int set_panel_width(int cols);
 
+// FP64 pposv on the library's static device scratch (noalloc.cpp): no
+// hipMalloc / hipFree; throws std::invalid_argument when the call needs more
+// scratch than the static array holds.  Returns synchronized.
+void pposv_noalloc(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, double* A, int ia, int ja,
+                   const int* descA, double* B, int ib, int jb, const int* descB, int& info);
+
 } // namespace ddla
