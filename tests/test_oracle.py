@@ -213,6 +213,29 @@ def test_logdet_cases(d):
         assert phase_close(got.imag, want.imag, 3e-12)
 
 
+@pytest.mark.parametrize("d", [[2, 3, 5], [-2, 3, 5], [0, -3, 5], [np.finfo(np.float32).max, np.finfo(np.float32).tiny],
+                               [1.1 + i % 3 for i in range(11)], [-0.7] * 1025, []])
+def test_logdet_cases_fp32(d):
+    """oracle_sdiag_logdet (the float instantiation): the same rules on float32
+    entries, against the long-double sum of the float32 values."""
+    n = len(d)
+    a = np.full((n + 3, max(n, 1)), np.nan, dtype=np.float32)
+    for i, x in enumerate(d):
+        a[i, i] = x
+    mag, flags = orc.diag_logdet(a, n)
+    got = orc.finish_logdet(mag, flags)
+    want = logdet_reference([float(np.float32(x)) for x in d])
+    if math.isinf(want.real):
+        assert math.isinf(got.real) and got.real < 0 and got.imag == 0
+    else:
+        assert abs(got.real - want.real) < 3e-12 * (1 + abs(want.real))
+        assert phase_close(got.imag, want.imag, 3e-12)
+    bad = a.copy()
+    if n:
+        bad[0, 0] = np.inf
+        assert orc.diag_logdet(bad, n)[1] & 1
+
+
 @pytest.mark.parametrize("bad", [np.inf, np.nan, -np.inf])
 def test_logdet_nonfinite_flag(bad):
     a = np.diag([1.0, bad])
@@ -239,6 +262,26 @@ def test_gemm_against_numpy(ta, tb):
     opb = b if tb == "N" else b.T
     got = orc.gemm(ta, tb, m, n, k, 1.5, a, b, -0.5, c)
     assert np.abs(got - (1.5 * opa @ opb - 0.5 * c)).max() < 1e-13
+
+
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_sgemm_against_numpy(ta, tb):
+    """oracle_sgemm (the float instantiation) on the same closed-form inputs,
+    against numpy's product of the float32 operands in float64."""
+    m, n, k = 7, 5, 9
+    a = closed_form(m * k, (m, k) if ta == "N" else (k, m)).astype(np.float32)
+    b = closed_form(k * n, (k, n) if tb == "N" else (n, k)).astype(np.float32)
+    c = (closed_form(m * n, (m, n)) + 0.25).astype(np.float32)
+    opa = (a if ta == "N" else a.T).astype(np.float64)
+    opb = (b if tb == "N" else b.T).astype(np.float64)
+    got = orc.gemm(ta, tb, m, n, k, 1.5, a, b, -0.5, c)
+    assert got.dtype == np.float32
+    assert np.abs(got - (1.5 * opa @ opb - 0.5 * c)).max() < 1e-5
+    # alpha 0 and k 0 scale C only; beta 0 never reads C
+    nan = np.full((m, n), np.nan, dtype=np.float32)
+    assert (orc.gemm(ta, tb, m, n, k, 0.0, np.full_like(a, np.nan), b, 2.0, c) == 2.0 * c).all()
+    assert np.abs(orc.gemm(ta, tb, m, n, k, 1.0, a, b, 0.0, nan) - opa @ opb).max() < 1e-5
 
 
 def test_gemm_special_cases():
