@@ -50,7 +50,12 @@ dLoss/dK and dLoss/dl are compared with the FP64 CPU closed form.
 cpu_baseline: the reference's CPU path at a 1 x 1 grid is one LAPACK dpotrf_
 and two BLAS dtrsm_ calls (src/Cholesky/potrf.cpp:46-49,
 src/BLAS/trsm.cpp:135-141); it is timed here as OpenBLAS dpotrf + dpotrs
-(scipy's bundled OpenBLAS) on a bounded sample of the same workload.
+(scipy's bundled OpenBLAS) on the host cores: once on the headline
+workload (n = 32768, nrhs = 64, about 10 s), and on the reference's own
+CPU-handle case (n = 2048, nrhs = 16, A = G G^T / n + I, seed 0) with the
+backward error of its solution and of this library's GPU pposv on the same
+inputs.  The host's CPU model, logical CPU count and OpenBLAS thread count
+are recorded.
 """
 import argparse
 import json
@@ -70,49 +75,125 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(h, dd, n_cpu, nrhs, reps):
-    """OpenBLAS dpotrf + dpotrs on an n_cpu sample generated by the same convention."""
-    import ctypes
+def host_cpu():
+    """CPU model and logical CPU count of this host (/proc/cpuinfo, os.cpu_count())."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
 
-    import numpy as np
-    from scipy.linalg import lapack
 
-    threads = None
+def openblas_threads():
+    """(threads, version) of the OpenBLAS that scipy's LAPACK wrappers call."""
     try:
         from threadpoolctl import threadpool_info
 
         for info in threadpool_info():
             if info.get("internal_api") == "openblas":
-                threads = info.get("num_threads")
-                version = info.get("version")
+                return info.get("num_threads"), info.get("version")
     except Exception:
-        version = None
-    a_dev = dd.DistributedMatrix(h, n_cpu, n_cpu, 512, 512, np.float64)
-    g_dev = dd.DistributedMatrix(h, n_cpu, n_cpu, 512, 512, np.float64)
-    dd.check_status(dd.lib.ddla_d_spd_generate(h.ptr, n_cpu, 1, 0.1, ctypes.c_void_p(a_dev.data_ptr), n_cpu,
+        pass
+    return None, None
+
+
+def host_spd(h, dd, n, seed, delta):
+    """A = G G^T / n + delta I of the project's seed convention, formed on the GPU, as a host Fortran array."""
+    import ctypes
+
+    import numpy as np
+
+    a_dev = dd.DistributedMatrix(h, n, n, 512, 512, np.float64)
+    g_dev = dd.DistributedMatrix(h, n, n, 512, 512, np.float64)
+    dd.check_status(dd.lib.ddla_d_spd_generate(h.ptr, n, seed, delta, ctypes.c_void_p(a_dev.data_ptr), n,
                                                ctypes.c_void_p(g_dev.data_ptr)))
     g_dev.free()
     a = a_dev.to_global_matrix()
     a_dev.free()
-    b = np.asfortranarray(np.random.RandomState(1).standard_normal((n_cpu, nrhs)))
-    times = []
-    for _ in range(reps):
-        work = np.array(a, order="F", copy=True)
-        t0 = time.perf_counter()
-        c, info = lapack.dpotrf(work, lower=1, clean=0, overwrite_a=1)
-        x, info2 = lapack.dpotrs(c, b, lower=1)
-        times.append(time.perf_counter() - t0)
-        assert info == 0 and info2 == 0
-    t = sorted(times)[len(times) // 2]
+    return a
+
+
+def host_normals(h, dd, seed, stream, rows, cols):
+    import ctypes
+
+    import numpy as np
+
+    b_dev = dd.DistributedMatrix(h, rows, cols, 512, 512, np.float64)
+    dd.check_status(dd.lib.ddla_c_fill_normal(h.ptr, seed, stream, 0, rows * cols, ctypes.c_void_p(b_dev.data_ptr)))
+    b = b_dev.to_global_matrix()
+    b_dev.free()
+    return b
+
+
+def cpu_baseline(h, dd, n_cpu, nrhs, reps):
+    """The reference's CPU path (LAPACK dpotrf + dpotrs in OpenBLAS) on the host cores: the headline
+    workload (FP64 pposv, n_cpu = 32768 by default, nrhs 64, A = G G^T / n + 0.1 I, seed 1), and the
+    reference's own CPU-handle case (n = 2048, nrhs 16, A = G G^T / n + I, seed 0) with its backward
+    error ||A X - B|| / (||A|| ||X|| n eps) beside that of this library's GPU pposv on the same inputs."""
+    import numpy as np
+    from scipy.linalg import lapack
+
+    threads, version = openblas_threads()
+
+    def timed(a, b, reps):
+        times, x = [], None
+        for _ in range(reps):
+            work = np.array(a, order="F", copy=True)
+            t0 = time.perf_counter()
+            c, info = lapack.dpotrf(work, lower=1, clean=0, overwrite_a=1)
+            x, info2 = lapack.dpotrs(c, b, lower=1)
+            times.append(time.perf_counter() - t0)
+            assert info == 0 and info2 == 0
+        return sorted(times)[len(times) // 2], x
+
+    a = host_spd(h, dd, n_cpu, 1, 0.1)
+    b = np.asfortranarray(host_normals(h, dd, 1, 1, n_cpu, nrhs))
+    t, _ = timed(a, b, reps)
+    del a
     flops = n_cpu ** 3 / 3 + 2.0 * n_cpu ** 2 * nrhs
+
+    # the reference's CPU-handle case: GPU pposv and the host LAPACK solve on the same inputs
+    n1, r1 = 2048, 16
+    a1 = host_spd(h, dd, n1, 0, 1.0)
+    b1 = np.asfortranarray(host_normals(h, dd, 0, 1, n1, r1))
+    t1, x_cpu = timed(a1, b1, 5)
+    A1 = dd.DistributedMatrix.from_global_matrix(h, a1, mb=128, nb=128)
+    B1 = dd.DistributedMatrix.from_global_matrix(h, b1, mb=128, nb=128)
+    X1, info = dd.pposv(A1, B1, "L")
+    assert info == 0
+    x_gpu = X1.to_global_matrix()
+    eps = float(np.finfo(np.float64).eps)
+
+    def berr(x):
+        r = a1 @ x - b1
+        return float(np.linalg.norm(r, np.inf) / (np.linalg.norm(a1, np.inf) * np.linalg.norm(x, np.inf) * n1 * eps))
+
+    flops1 = n1 ** 3 / 3 + 2.0 * n1 ** 2 * r1
+    cpu = host_cpu()
     return {
         "value": round(flops / t / 1e12, 4),
         "unit": "TFLOP/s",
-        "cores": threads or os.cpu_count(),
+        "cores": threads or cpu["nproc"],
         "kind": "reference",
         "impl": f"OpenBLAS {version or '?'} dpotrf+dpotrs (scipy build), the LAPACK calls of the reference's "
                 "1x1 CPU handle",
-        "sample": f"FP64 pposv n={n_cpu} nrhs={nrhs}, same A convention; median of {reps}, {t:.2f} s each",
+        "sample": f"FP64 pposv n={n_cpu} nrhs={nrhs} (the headline workload: A = G G^T/n + 0.1 I, seed 1); "
+                  f"median of {reps}, {t:.2f} s each",
+        "ms": round(t * 1e3, 1),
+        "host": {"cpu_model": cpu["model"], "nproc": cpu["nproc"], "openblas_threads": threads},
+        "cpu_handle_case": {
+            "workload": f"pposv n={n1} nrhs={r1}, A = G G^T/n + I, seed 0 (the reference's CPU-handle case)",
+            "cpu_ms": round(t1 * 1e3, 3),
+            "cpu_gflops": round(flops1 / t1 / 1e9, 1),
+            "cpu_backward_error_n_eps": berr(x_cpu),
+            "gpu_backward_error_n_eps": berr(x_gpu),
+            "max_rel_diff_gpu_vs_cpu": float(np.abs(x_gpu - x_cpu).max() / np.abs(x_cpu).max()),
+        },
     }
 
 
@@ -394,8 +475,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nrhs", type=int, default=64)
-    ap.add_argument("--cpu-n", type=int, default=16384)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-n", type=int, default=32768)
+    ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--deltas", type=float, nargs="+", default=[4e-2, 4e-4, 4e-6])
@@ -504,7 +585,10 @@ def main():
         "backward_error": bwd,
         "roofline": {"bound": "mfma", "kernel": "trailing_update (FP64 MFMA syrk)", "achieved": round(achieved, 3),
                      "peak": FP64_PEAK, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK, 4),
-                     "traffic": traffic, "launches_per_step": cnt // max(1, args.steps)},
+                     "traffic": traffic,
+                     "traffic_source": "profiles/trailing_update_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                       "passes over this workload, not measured in this run)",
+                     "launches_per_step": cnt // max(1, args.steps)},
         "breakdown": breakdown,
         "solve_phase": solve_phase,
     }

@@ -205,11 +205,22 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
 }
 
 // Left solve A X = B with A = L L^T given by its L view; B is n x nrhs: the
-// forward and the backward sweep, one launch each.
+// forward and the backward sweep, one launch each.  stage_work (optional):
+// the staged solve's work (kern::stage_work_bytes), used when
+// kern::stage_solve(n, nrhs).
 template <class T>
 void solve_left(DdlaStream* h, bool upper, int n, int nrhs, const T* F, long long ldf, T* B, long long ldb,
-                const T* dinv, void* sync, int* error)
+                const T* dinv, void* sync, int* error, void* stage_work)
 {
+    if (stage_work != nullptr && kern::stage_solve(n, nrhs)) {
+        {
+            LaunchTimer timer(h, kClassTrtri, 2.0 * n * 512.0 * 512.0 + (double)n * 512.0 * 512.0 / 3.0);
+            kern::stage_prepare<T>(h->stream, upper, n, F, ldf, dinv, stage_work);
+        }
+        LaunchTimer timer(h, kClassSolve, 2.0 * n * n * nrhs, (double)n * n * sizeof(T));
+        kern::stage_sweeps<T>(h->stream, upper, n, nrhs, F, ldf, B, ldb, stage_work, error);
+        return;
+    }
     LaunchTimer timer(h, kClassSolve, 2.0 * n * n * nrhs, (double)n * n * sizeof(T));
     kern::potrs_sweeps<T>(h->stream, upper, n, nrhs, F, ldf, dinv, B, ldb, sync, error);
 }
@@ -295,26 +306,35 @@ void ppotrs_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n
     if (rows && cols && !B) throw std::invalid_argument("ppotrs: null local right-hand side");
 }
 
+// Backend bytes of a ppotrs run; staged: with the staged solve's work when
+// the order and right-hand sides take that route.
+template <class T>
+std::size_t ppotrs_bytes(char side, int n, int nrhs, bool staged)
+{
+    if (n == 0 || nrhs == 0) return 0;
+    // [ inverses of the 64 x 64 diagonal blocks | sweep counters | error word | staged-solve work |
+    //   transposed B (side 'R') ]
+    std::size_t b = detail::padded_bytes<T>(diag_inverse_elems(n)) + kern::solve_work_bytes(nrhs) +
+                    detail::padded_bytes<int>(1);
+    if (staged) b += detail::padded_bytes<char>(kern::stage_work_bytes(n, nrhs, sizeof(T)));
+    if (side == 'R') b += detail::padded_bytes<T>((std::size_t)n * nrhs);
+    return b;
+}
+
 template <class T>
 void ppotrs_workspace_size(const DdlaHandle_t&, char side, int n, int nrhs, const int*, const int*,
                            std::size_t* backend_bytes, std::size_t* host_bytes)
 {
-    *backend_bytes = *host_bytes = 0;
-    if (n == 0 || nrhs == 0) return;
-    // [ inverses of the 64 x 64 diagonal blocks | sweep counters | error word | transposed B (side 'R') ]
-    *backend_bytes = detail::padded_bytes<T>(diag_inverse_elems(n)) + kern::solve_work_bytes(nrhs) +
-                     detail::padded_bytes<int>(1);
-    if (side == 'R') *backend_bytes += detail::padded_bytes<T>((std::size_t)n * nrhs);
+    *host_bytes = 0;
+    *backend_bytes = ppotrs_bytes<T>(side, n, nrhs, true);
 }
 
 template <class T>
-void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char, int n, int nrhs, const T* A, const int* da, T* B,
-                const int* db, void* backend_work, std::size_t backend_bytes, void*, std::size_t)
+void ppotrs_exec(const DdlaHandle_t& h, char side, char uplo, int n, int nrhs, const T* A, const int* da, T* B,
+                 const int* db, void* backend_work, std::size_t backend_bytes, bool staged)
 {
     if (n == 0 || nrhs == 0) return;
-    std::size_t need = 0, host = 0;
-    ppotrs_workspace_size<T>(h, side, n, nrhs, da, db, &need, &host);
-    if (backend_bytes < need || backend_work == nullptr) bad_workspace("ppotrs");
+    if (backend_bytes < ppotrs_bytes<T>(side, n, nrhs, staged) || backend_work == nullptr) bad_workspace("ppotrs");
     const bool upper = uplo == 'U';
     const long long lda = da[DDLA_LLD_], ldb = db[DDLA_LLD_];
     char* w = static_cast<char*>(backend_work);
@@ -322,6 +342,9 @@ void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char, int n, int nr
     void* sync = w;
     w += kern::solve_work_bytes(nrhs);
     int* error = detail::take<int>(w, 1);
+    void* stage_work = nullptr;
+    if (staged && kern::stage_solve(n, nrhs))
+        stage_work = detail::take<char>(w, kern::stage_work_bytes(n, nrhs, sizeof(T)));
     fill_int(h->stream, error, 0);
     {
         LaunchTimer timer(h, kClassTrtri, (double)n * kIB * kIB / 3.0);
@@ -330,17 +353,24 @@ void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char, int n, int nr
     // A = L L^T is symmetric: trans 'N' and 'C' solve the same system.
     // side 'R' (X A = B, B nrhs x n) is A X^T = B^T.
     if (side == 'L') {
-        solve_left<T>(h, upper, n, nrhs, A, lda, B, ldb, dinv, sync, error);
+        solve_left<T>(h, upper, n, nrhs, A, lda, B, ldb, dinv, sync, error, stage_work);
     } else {
         T* bt = detail::take<T>(w, (std::size_t)n * nrhs);
         kern::transpose<T>(h->stream, nrhs, n, B, ldb, bt, n);
-        solve_left<T>(h, upper, n, nrhs, A, lda, bt, n, dinv, sync, error);
+        solve_left<T>(h, upper, n, nrhs, A, lda, bt, n, dinv, sync, error, stage_work);
         kern::transpose<T>(h->stream, n, nrhs, bt, n, B, ldb);
     }
     int host_error = 0;
     DDLA_HIP(hipMemcpyAsync(&host_error, error, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     DDLA_HIP(hipStreamSynchronize(h->stream));
     if (host_error != 0) throw std::runtime_error("ppotrs: a triangular-solve sweep exceeded its wait limit");
+}
+
+template <class T>
+void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char, int n, int nrhs, const T* A, const int* da, T* B,
+                const int* db, void* backend_work, std::size_t backend_bytes, void*, std::size_t)
+{
+    ppotrs_exec<T>(h, side, uplo, n, nrhs, A, da, B, db, backend_work, backend_bytes, true);
 }
 
 void pposv_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A, int ia,
@@ -378,6 +408,32 @@ void pposv_run(const DdlaHandle_t& h, char side, char uplo, char trans, int n, i
 }
 
 } // namespace cholesky_impl
+
+template <class T>
+std::size_t pposv_unstaged_bytes(char side, int n, int nrhs)
+{
+    std::size_t fb = 0, fh = 0;
+    cholesky_impl::ppotrf_workspace_size<T>(nullptr, 'L', n, static_cast<T*>(nullptr), nullptr, &fb, &fh);
+    return std::max(fb, cholesky_impl::ppotrs_bytes<T>(side, n, nrhs, false));
+}
+
+template <class T>
+void pposv_run_unstaged(const DdlaHandle_t& h, char side, char uplo, int n, int nrhs, T* A, const int* da, T* B,
+                        const int* db, int& info, void* backend_work, std::size_t backend_bytes)
+{
+    if (n == 0) {
+        info = 0;
+        return;
+    }
+    cholesky_impl::ppotrf_run<T>(h, uplo, n, A, 1, 1, da, info, backend_work, backend_bytes, nullptr, 0);
+    if (info == 0)
+        cholesky_impl::ppotrs_exec<T>(h, side, uplo, n, nrhs, A, da, B, db, backend_work, backend_bytes, false);
+}
+
+template std::size_t pposv_unstaged_bytes<double>(char, int, int);
+template void pposv_run_unstaged<double>(const DdlaHandle_t&, char, char, int, int, double*, const int*, double*,
+                                         const int*, int&, void*, std::size_t);
+
 
 // ---------------------------------------------------------------------------
 // Public drivers: check, size, scratch, run, synchronize.

@@ -123,6 +123,15 @@ void fill_int(hipStream_t s, int* p, int value);
 // previous override (0 = none).
 int set_panel_width(int cols);
 
+// pposv with the 64-row sweeps whatever the order (no staged-solve work, so
+// the workspace stays at about 512 bytes per row): its backend bytes and its
+// run (ia = ja = ib = jb = 1, arguments already checked; cholesky.cpp).
+template <class T>
+std::size_t pposv_unstaged_bytes(char side, int n, int nrhs);
+template <class T>
+void pposv_run_unstaged(const DdlaHandle_t& h, char side, char uplo, int n, int nrhs, T* A, const int* descA, T* B,
+                        const int* descB, int& info, void* backend_work, std::size_t backend_bytes);
+
 // FP64 pposv on the library's static device scratch (noalloc.cpp): no
 // hipMalloc / hipFree; throws std::invalid_argument when the call needs more
 // scratch than the static array holds.  Returns synchronized.

@@ -111,6 +111,23 @@ template <class T>
 void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long long ldl, const T* dinv, T* B,
                   long long ldb, void* work, int* error);
 
+// Staged solve (stage_tile.h): the same forward and backward sweeps over
+// 512-row stages, for orders from kStageMinOrder up with fewer than
+// kTrsmMinColumns right-hand sides (stage_solve).  stage_prepare forms, once
+// per factor, the inverse of every stage's diagonal block and the two sweeps'
+// coupling blocks from the factor and dinv; stage_sweeps then runs both
+// sweeps in place on B, as often as needed.  work holds
+// stage_work_bytes(n, nrhs, sizeof(T)) bytes (prepared data, per-slab
+// accumulators and counters); *error as for potrs_sweeps.
+constexpr int kStageMinOrder = 8192;
+bool stage_solve(int n, int nrhs);
+size_t stage_work_bytes(int n, int nrhs, size_t elem);
+template <class T>
+void stage_prepare(hipStream_t s, bool upper, int n, const T* L, long long ldl, const T* dinv, void* work);
+template <class T>
+void stage_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long long ldl, T* B, long long ldb,
+                  void* work, int* error);
+
 // Batched fused potrf + potrs + logdet of independent matrices (order n[i] <=
 // 64 * max_blocks <= 1024, stored triangle uplo, leading dimension lda[i]; B[i]
 // n[i] x nrhs with ldb[i]) in one persistent launch of `grid` workgroups, after

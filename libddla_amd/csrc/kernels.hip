@@ -8,6 +8,7 @@
 #include "kernels.h"
 #include "gemm_tile.h"
 #include "solve_tile.h"
+#include "stage_tile.h"
 #include "potf2_tile.h"
 #include "panel_tile.h"
 #include "batched_tile.h"
@@ -829,6 +830,105 @@ void potrs_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
     check_launch("potrs");
 }
 
+bool stage_solve(int n, int nrhs) { return n >= kStageMinOrder && nrhs > 0 && !trsm_splits(n, nrhs); }
+
+namespace {
+// Layout of the staged solve's work: [E | M forward | M backward | accumulators | counters].
+struct StageLayout {
+    size_t e, mf, mb, acc, sync, total;
+};
+StageLayout stage_layout(int n, int nrhs, size_t elem)
+{
+    using namespace stage_detail;
+    const size_t nst = (size_t)(n + kS - 1) / kS, slabs = (size_t)(nrhs + kSB - 1) / kSB;
+    const size_t nblk = (size_t)(n + kSB - 1) / kSB;
+    const size_t block = nst * kS * kS * elem;
+    StageLayout l{};
+    size_t off = 0;
+    auto put = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) / 256 * 256;
+        return at;
+    };
+    l.e = put(block);
+    l.mf = put(block);
+    l.mb = put(block);
+    l.acc = put(slabs * (size_t)n * kSB * elem);
+    l.sync = put(slabs * (size_t)sync_ints((int)nst, (int)nblk) * sizeof(int));
+    l.total = off;
+    return l;
+}
+
+template <class T>
+stage_detail::StageArgs<T> stage_args(bool upper, int n, int nrhs, const T* L, long long ldl, const T* dinv, T* B,
+                                      long long ldb, void* work, int* error)
+{
+    using namespace stage_detail;
+    const StageLayout l = stage_layout(n, nrhs, sizeof(T));
+    char* w = static_cast<char*>(work);
+    StageArgs<T> a{};
+    a.n = n;
+    a.nrhs = nrhs;
+    a.nst = (n + kS - 1) / kS;
+    a.nblk = (n + kSB - 1) / kSB;
+    a.L = L;
+    a.ldl = ldl;
+    a.upper = upper ? 1 : 0;
+    a.dinv = dinv;
+    a.E = reinterpret_cast<T*>(w + l.e);
+    a.M = reinterpret_cast<T*>(w + l.mf);
+    a.B = B;
+    a.ldb = ldb;
+    a.acc = reinterpret_cast<T*>(w + l.acc);
+    a.sync = reinterpret_cast<int*>(w + l.sync);
+    a.error = error;
+    a.spin_limit = wait_limit();
+    return a;
+}
+} // namespace
+
+size_t stage_work_bytes(int n, int nrhs, size_t elem) { return stage_solve(n, nrhs) ? stage_layout(n, nrhs, elem).total : 0; }
+
+template <class T>
+void stage_prepare(hipStream_t s, bool upper, int n, const T* L, long long ldl, const T* dinv, void* work)
+{
+    using namespace stage_detail;
+    StageArgs<T> a = stage_args<T>(upper, n, 1, L, ldl, dinv, nullptr, 0, work, nullptr);
+    const StageLayout l = stage_layout(n, 1, sizeof(T));
+    stage_inverse_kernel<T><<<dim3((unsigned)a.nst, kBlocks), kThreads, 0, s>>>(a);
+    check_launch("stage_prepare");
+    if (a.nst < 2) return;
+    const dim3 grid((unsigned)(a.nst - 1), kBlocks, kBlocks);
+    stage_product_kernel<T, false><<<grid, kThreads, 0, s>>>(a);
+    check_launch("stage_prepare");
+    a.M = reinterpret_cast<T*>(static_cast<char*>(work) + l.mb);
+    stage_product_kernel<T, true><<<grid, kThreads, 0, s>>>(a);
+    check_launch("stage_prepare");
+}
+
+template <class T>
+void stage_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long long ldl, T* B, long long ldb,
+                  void* work, int* error)
+{
+    using namespace stage_detail;
+    if (n <= 0 || nrhs <= 0) return;
+    if (!stage_solve(n, nrhs)) throw std::logic_error("stage_sweeps: order or right-hand sides out of range");
+    StageArgs<T> a = stage_args<T>(upper, n, nrhs, L, ldl, nullptr, B, ldb, work, error);
+    const StageLayout l = stage_layout(n, nrhs, sizeof(T));
+    const int slabs = (nrhs + kSB - 1) / kSB;
+    const int counters = slabs * sync_ints(a.nst, a.nblk);
+    const dim3 grid((unsigned)a.nblk, (unsigned)slabs);
+    solve_detail::zero_ints_kernel<<<(counters + 255) / 256, 256, 0, s>>>(a.sync, counters);
+    check_launch("stage_sweeps");
+    stage_sweep_kernel<T, false><<<grid, kThreads, 0, s>>>(a);
+    check_launch("stage_sweeps");
+    a.M = reinterpret_cast<T*>(static_cast<char*>(work) + l.mb);
+    solve_detail::zero_ints_kernel<<<(counters + 255) / 256, 256, 0, s>>>(a.sync, counters);
+    check_launch("stage_sweeps");
+    stage_sweep_kernel<T, true><<<grid, kThreads, 0, s>>>(a);
+    check_launch("stage_sweeps");
+}
+
 template <class T>
 void trsm_sweep(hipStream_t s, bool upper, bool backward, int n, int nrhs, const T* L, long long ldl, const T* dinv,
                 T* B, long long ldb, void* work, int* error)
@@ -1194,6 +1294,8 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template void potrs_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, const T*, T*, long long,   \
                                   void*, int*);                                                               \
     template void add_diagonal<T>(hipStream_t, int, T*, long long, T);                                        \
+    template void stage_prepare<T>(hipStream_t, bool, int, const T*, long long, const T*, void*);                \
+    template void stage_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, T*, long long, void*, int*);  \
     template void trsm_sweep<T>(hipStream_t, bool, bool, int, int, const T*, long long, const T*, T*, long long, \
                                 void*, int*);                                                                 \
     template void trsm<T>(hipStream_t, bool, bool, int, int, const T*, long long, const T*, T*, long long,       \

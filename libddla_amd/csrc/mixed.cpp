@@ -31,7 +31,7 @@ constexpr int kIB = 64;
 std::size_t diag_inverse_elems(int n) { return (std::size_t)((n + kIB - 1) / kIB) * kIB * kIB; }
 
 struct MixedLayout {
-    std::size_t a32, norm, scalars, x, r, w, info, panel, dinv, sync, error, split[3], total;
+    std::size_t a32, norm, scalars, x, r, w, info, panel, dinv, sync, error, stage, split[3], total;
 };
 
 MixedLayout mixed_layout(int n, int nrhs)
@@ -55,6 +55,7 @@ MixedLayout mixed_layout(int n, int nrhs)
     l.dinv = put(diag_inverse_elems(n) * sizeof(float));
     l.sync = put(kern::solve_work_bytes(nrhs));
     l.error = put(sizeof(int));
+    l.stage = put(kern::stage_work_bytes(n, nrhs, sizeof(float)));
     for (std::size_t& b : l.split) b = put(kern::split_buffer_bytes(n, 6));
     l.total = off;
     return l;
@@ -143,9 +144,17 @@ void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, co
             LaunchTimer timer(h, kClassTrtri, (double)n * kIB * kIB / 3.0);
             kern::trtri_diag_blocks<float>(s, upper, n, a32, n, dinv);
         }
+        // every correction solves with the same factor: the staged solve's
+        // blocks are prepared once
+        const bool staged = kern::stage_solve(n, nrhs);
+        if (staged) {
+            LaunchTimer timer(h, kClassTrtri, 2.0 * n * 512.0 * 512.0 + (double)n * 512.0 * 512.0 / 3.0);
+            kern::stage_prepare<float>(s, upper, n, a32, n, dinv, base + l.stage);
+        }
         auto solve32 = [&]() {
             LaunchTimer timer(h, kClassSolve, 2.0 * n * n * nrhs, (double)n * n * sizeof(float));
-            kern::potrs_sweeps<float>(s, upper, n, nrhs, a32, n, dinv, w, n, base + l.sync, error);
+            if (staged) kern::stage_sweeps<float>(s, upper, n, nrhs, a32, n, w, n, base + l.stage, error);
+            else kern::potrs_sweeps<float>(s, upper, n, nrhs, a32, n, dinv, w, n, base + l.sync, error);
         };
         kern::convert<float, double>(s, n, nrhs, B, ldb, w, n);
         solve32();

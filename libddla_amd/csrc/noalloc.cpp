@@ -61,16 +61,16 @@ void pposv_noalloc(const DdlaHandle_t& h, char side, char uplo, char trans, int 
         info = 0;
         return;
     }
-    std::size_t bytes = 0, host = 0;
-    cholesky_impl::pposv_workspace_size<double>(h, side, uplo, n, nrhs, A, descA, descB, &bytes, &host);
+    // The 64-row sweeps keep the workspace at about 512 bytes per row; the
+    // staged solve's prepared blocks (12 KB per row in FP64) would not fit.
+    const std::size_t bytes = pposv_unstaged_bytes<double>(side, n, nrhs);
     if (bytes > kNoAllocScratchBytes)
         throw std::invalid_argument("pposv_noalloc: this order and right-hand-side count need " +
                                     std::to_string(bytes) + " bytes of scratch, more than the " +
                                     std::to_string(kNoAllocScratchBytes) +
                                     " bytes of the static scratch (use pposv, which allocates)");
     void* work = scratch_address(h->device);
-    cholesky_impl::pposv_run<double>(h, side, uplo, trans, n, nrhs, A, ia, ja, descA, B, ib, jb, descB, info, work,
-                                     bytes, nullptr, 0);
+    pposv_run_unstaged<double>(h, side, uplo, n, nrhs, A, descA, B, descB, info, work, bytes);
     DDLA_HIP(hipStreamSynchronize(h->stream));
 }
 

@@ -115,12 +115,25 @@ __device__ inline void block_mma(typename Mfma<T>::acc_t (&acc)[2], const T (*As
     using Op = Mfma<T>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = (wave & 3) * 16 + (lane & 15), rt0 = (wave >> 2) * 2;
-#pragma unroll 4
-    for (int ks = 0; ks < kSB; ks += 4) {
-        const int kk = ks + (lane >> 4);
-        const T b = BT ? Bs[col][kk] : Bs[kk][col];
+    // The operands of KC k-steps (24 registers) are read from LDS before
+    // their products, so that the reads overlap one another instead of each
+    // pair of products waiting for its own.
+    constexpr int KC = sizeof(T) == 4 ? 8 : 4;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) acc[i] = Op::mma(sign * As[kk][(rt0 + i) * 16 + (lane & 15)], b, acc[i]);
+    for (int k0 = 0; k0 < kSB / 4; k0 += KC) {
+        T av[2][KC], bv[KC];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+            const int kk = 4 * (k0 + j) + (lane >> 4);
+            bv[j] = sign * (BT ? Bs[col][kk] : Bs[kk][col]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i][j] = As[kk][(rt0 + i) * 16 + (lane & 15)];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) acc[i] = Op::mma(av[i][j], bv[j], acc[i]);
     }
 }
 
@@ -134,10 +147,17 @@ __device__ inline T factor_element(const SolveArgs<T>& a, long long i0, long lon
     const int idx = threadIdx.x + kThreads * e;
     r = rows_contiguous ? (idx & (kSB - 1)) : (idx >> 6);
     k = rows_contiguous ? (idx >> 6) : (idx & (kSB - 1));
-    if (r >= rows || k >= cols) return T(0);
-    const long long i = transposed ? j0 + k : i0 + r;
-    const long long j = transposed ? i0 + r : j0 + k;
-    return a.upper ? a.L[j + i * a.ldl] : a.L[i + j * a.ldl];
+    if (rows <= 0 || cols <= 0) return T(0);
+    // Branch-free: outside the block the load is clamped to its last row /
+    // column and the value dropped, so every lane issues the same loads and
+    // the compiler can wait for them by count instead of draining all
+    // outstanding loads (which would also wait for the prefetched block).
+    const bool in = r < rows && k < cols;
+    const int rc = min(r, rows - 1), kc = min(k, cols - 1);
+    const long long i = transposed ? j0 + kc : i0 + rc;
+    const long long j = transposed ? i0 + rc : j0 + kc;
+    const T v = a.upper ? a.L[j + i * a.ldl] : a.L[i + j * a.ldl];
+    return in ? v : T(0);
 }
 
 // Ds[k][row] := D(row, k) with D = Dinv of diagonal block bt (forward) or
