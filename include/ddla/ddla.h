@@ -365,6 +365,49 @@ void gemmVbatched2s(const DdlaHandle_t& handle, char transA_0, char transB_0, in
                     T** d_C_array_1, int* d_ldc_1, bool C0_left, int batch_count, const int* segment_sizes,
                     int segment_count);
 
+// Mixed real/complex overloads (the reference's include/ddla/ddla.h:1878-1960):
+// one operand real, the output and scalars complex; 'C' on a real operand is
+// 'T'.  Two-stage forms: stage 0 with the typed operands, the complex
+// intermediate C0, and a real or complex external stage-1 operand.
+template <typename R>
+void gemmVbatched(const DdlaHandle_t& handle, char transA, char transB, int* m, int* n, int* k, std::complex<R> alpha,
+                  const R* const* A, int* lda, const std::complex<R>* const* B, int* ldb, std::complex<R> beta,
+                  std::complex<R>** C, int* ldc, int batch_count);
+template <typename R>
+void gemmVbatched(const DdlaHandle_t& handle, char transA, char transB, int* m, int* n, int* k, std::complex<R> alpha,
+                  const std::complex<R>* const* A, int* lda, const R* const* B, int* ldb, std::complex<R> beta,
+                  std::complex<R>** C, int* ldc, int batch_count);
+template <typename R>
+void gemmVbatched2s(const DdlaHandle_t& handle, char transA_0, char transB_0, int* m_0, int* n_0, int* k_0,
+                    std::complex<R> alpha_0, const R* const* A_0, int* lda_0, const std::complex<R>* const* B_0,
+                    int* ldb_0, std::complex<R> beta_0, std::complex<R>** C_0, int* ldc_0, char transA_1,
+                    char transB_1, int* m_1, int* n_1, int* k_1, std::complex<R> alpha_1, const R* const* AB_1,
+                    int* lda_1, int* ldb_1, std::complex<R> beta_1, std::complex<R>** C_1, int* ldc_1, bool C0_left,
+                    int batch_count, const int* segment_sizes, int segment_count);
+template <typename R>
+void gemmVbatched2s(const DdlaHandle_t& handle, char transA_0, char transB_0, int* m_0, int* n_0, int* k_0,
+                    std::complex<R> alpha_0, const std::complex<R>* const* A_0, int* lda_0, const R* const* B_0,
+                    int* ldb_0, std::complex<R> beta_0, std::complex<R>** C_0, int* ldc_0, char transA_1,
+                    char transB_1, int* m_1, int* n_1, int* k_1, std::complex<R> alpha_1, const R* const* AB_1,
+                    int* lda_1, int* ldb_1, std::complex<R> beta_1, std::complex<R>** C_1, int* ldc_1, bool C0_left,
+                    int batch_count, const int* segment_sizes, int segment_count);
+template <typename R>
+void gemmVbatched2s(const DdlaHandle_t& handle, char transA_0, char transB_0, int* m_0, int* n_0, int* k_0,
+                    std::complex<R> alpha_0, const R* const* A_0, int* lda_0, const std::complex<R>* const* B_0,
+                    int* ldb_0, std::complex<R> beta_0, std::complex<R>** C_0, int* ldc_0, char transA_1,
+                    char transB_1, int* m_1, int* n_1, int* k_1, std::complex<R> alpha_1,
+                    const std::complex<R>* const* AB_1, int* lda_1, int* ldb_1, std::complex<R> beta_1,
+                    std::complex<R>** C_1, int* ldc_1, bool C0_left, int batch_count, const int* segment_sizes,
+                    int segment_count);
+template <typename R>
+void gemmVbatched2s(const DdlaHandle_t& handle, char transA_0, char transB_0, int* m_0, int* n_0, int* k_0,
+                    std::complex<R> alpha_0, const std::complex<R>* const* A_0, int* lda_0, const R* const* B_0,
+                    int* ldb_0, std::complex<R> beta_0, std::complex<R>** C_0, int* ldc_0, char transA_1,
+                    char transB_1, int* m_1, int* n_1, int* k_1, std::complex<R> alpha_1,
+                    const std::complex<R>* const* AB_1, int* lda_1, int* ldb_1, std::complex<R> beta_1,
+                    std::complex<R>** C_1, int* ldc_1, bool C0_left, int batch_count, const int* segment_sizes,
+                    int segment_count);
+
 // Mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
 // semantics): factor in FP32 on the matrix cores -- trailing updates in FP32
 // (factor = 0) or as split bfloat16 products accumulated in FP32, bf16x3

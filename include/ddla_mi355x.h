@@ -97,6 +97,22 @@ int ddla_s_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs
 int ddla_d_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
                  const int* descA, void* B, int ib, int jb, const int* descB, int* info);            /* :1308 */
 
+/* ---- complex Hermitian drivers: replace fortran/ddla_c_api.cpp:1173-1190
+ * (ppotrf), :1281-1299 (ppotrs), :1317-1335 (pposv).  A = L L^H (uplo 'L')
+ * or U^H U ('U'); matrices are std::complex<float> ('c') / <double> ('z')
+ * arrays, real part first.  side 'R' solves X A = B; trans 'N' and 'C' solve
+ * the same system.  Same statuses, info and synchronization as the real ones. */
+int ddla_c_ppotrf(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info); /* :1173 */
+int ddla_z_ppotrf(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info); /* :1182 */
+int ddla_c_ppotrs(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                  const int* descA, void* B, const int* descB);                                      /* :1281 */
+int ddla_z_ppotrs(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                  const int* descA, void* B, const int* descB);                                      /* :1290 */
+int ddla_c_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                 const int* descA, void* B, int ib, int jb, const int* descB, int* info);            /* :1317 */
+int ddla_z_pposv(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                 const int* descA, void* B, int ib, int jb, const int* descB, int* info);            /* :1326 */
+
 /* ---- triangular solves: replace fortran/ddla_c_api.cpp:351-367 (ptrtrs)
  * and :1451-1468 (trsm).  B := op(A)^-1 B (side 'L', A m x m) or B op(A)^-1
  * (side 'R', A n x n); uplo 'L' / 'U', trans 'N' / 'T' / 'C', diag 'N' /
@@ -162,6 +178,36 @@ int ddla_d_pposv_run(void* handle, char side, char uplo, char trans, int n, int 
 int ddla_d_pposv_noalloc(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
                          const int* descA, void* B, int ib, int jb, const int* descB, int* info);
 
+/* Complex three-phase entries (same contracts as the real ones above). */
+int ddla_c_ppotrf_workspace_size(void* handle, char uplo, int n, const int* descA, long long* backend_bytes,
+                                 long long* host_bytes);
+int ddla_z_ppotrf_workspace_size(void* handle, char uplo, int n, const int* descA, long long* backend_bytes,
+                                 long long* host_bytes);
+int ddla_c_ppotrf_run(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info,
+                      void* backend_work, long long backend_bytes, void* host_work, long long host_bytes);
+int ddla_z_ppotrf_run(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info,
+                      void* backend_work, long long backend_bytes, void* host_work, long long host_bytes);
+int ddla_c_ppotrs_workspace_size(void* handle, char side, int n, int nrhs, const int* descA, const int* descB,
+                                 long long* backend_bytes, long long* host_bytes);
+int ddla_z_ppotrs_workspace_size(void* handle, char side, int n, int nrhs, const int* descA, const int* descB,
+                                 long long* backend_bytes, long long* host_bytes);
+int ddla_c_ppotrs_run(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                      const int* descA, void* B, const int* descB, void* backend_work, long long backend_bytes,
+                      void* host_work, long long host_bytes);
+int ddla_z_ppotrs_run(void* handle, char side, char uplo, char trans, int n, int nrhs, const void* A,
+                      const int* descA, void* B, const int* descB, void* backend_work, long long backend_bytes,
+                      void* host_work, long long host_bytes);
+int ddla_c_pposv_workspace_size(void* handle, char side, char uplo, int n, int nrhs, const int* descA,
+                                const int* descB, long long* backend_bytes, long long* host_bytes);
+int ddla_z_pposv_workspace_size(void* handle, char side, char uplo, int n, int nrhs, const int* descA,
+                                const int* descB, long long* backend_bytes, long long* host_bytes);
+int ddla_c_pposv_run(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                     const int* descA, void* B, int ib, int jb, const int* descB, int* info, void* backend_work,
+                     long long backend_bytes, void* host_work, long long host_bytes);
+int ddla_z_pposv_run(void* handle, char side, char uplo, char trans, int n, int nrhs, void* A, int ia, int ja,
+                     const int* descA, void* B, int ib, int jb, const int* descB, int* info, void* backend_work,
+                     long long backend_bytes, void* host_work, long long host_bytes);
+
 /* ---- mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
  * semantics): FP64 A X = B, side 'L', factored in FP32 on the matrix cores
  * (factor_precision 0: FP32 trailing updates; 1: bf16x3 and 2: bf16x6 split
@@ -189,6 +235,9 @@ int ddla_d_pposv_mixed_run(void* handle, char uplo, int n, int nrhs, void* A, co
 int ddla_s_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
 int ddla_d_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
 
+int ddla_c_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
+int ddla_z_potrf(void* handle, char uplo, int n, void* A, int lda, int* d_info);
+
 /* ---- Cholesky from the bottom-right corner, host info (C++
  * ddla::potrf_bottom_right / ppotrf_bottom_right, include/ddla/ddla.h:1090,
  * 1115; C entries fortran/ddla_c_api.cpp:1191-1258): uplo 'U' A = U U^T,
@@ -202,10 +251,15 @@ int ddla_d_ppotrf_bottom_right(void* handle, char uplo, int n, void* A, const in
  * entry in the reference): result[0] = real part, result[1] = phase. ----- */
 int ddla_s_diag_logdet(void* handle, int n, const void* A, int lda, float* result);
 int ddla_d_diag_logdet(void* handle, int n, const void* A, int lda, double* result);
+/* complex diagonals: result[0] = sum log |d_i|, result[1] = principal phase in (-pi, pi]. */
+int ddla_c_diag_logdet(void* handle, int n, const void* A, int lda, float* result);
+int ddla_z_diag_logdet(void* handle, int n, const void* A, int lda, double* result);
 /* pdiag_logdet (C++ ddla::pdiag_logdet, include/ddla/ddla.h:2914; no C entry
  * in the reference): the same on the leading n x n block of a descriptor. */
 int ddla_s_pdiag_logdet(void* handle, int n, const void* A, const int* descA, float* result);
 int ddla_d_pdiag_logdet(void* handle, int n, const void* A, const int* descA, double* result);
+int ddla_c_pdiag_logdet(void* handle, int n, const void* A, const int* descA, float* result);
+int ddla_z_pdiag_logdet(void* handle, int n, const void* A, const int* descA, double* result);
 
 /* ---- gemmVbatched (C++ ddla::gemmVbatched, include/ddla/ddla.h:1836; the
  * reference binds no C entry, README.md:967-970).  All arrays are device
@@ -216,6 +270,25 @@ int ddla_s_gemm_vbatched(void* handle, char transA, char transB, int* d_m, int* 
 int ddla_d_gemm_vbatched(void* handle, char transA, char transB, int* d_m, int* d_n, int* d_k, const double* alpha,
                          const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb, const double* beta,
                          void** d_C, int* d_ldc, int batch_count);
+
+/* complex types: alpha / beta point to (real, imaginary) pairs */
+int ddla_c_gemm_vbatched(void* handle, char transA, char transB, int* d_m, int* d_n, int* d_k, const void* alpha,
+                         const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb, const void* beta,
+                         void** d_C, int* d_ldc, int batch_count);
+int ddla_z_gemm_vbatched(void* handle, char transA, char transB, int* d_m, int* d_n, int* d_k, const void* alpha,
+                         const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb, const void* beta,
+                         void** d_C, int* d_ldc, int batch_count);
+/* Mixed real/complex (C++ overloads of include/ddla/ddla.h:1878-1960): complex
+ * output and scalars, real_operand 0 makes A the real operand, 1 makes B real;
+ * 'C' on the real operand is 'T'. */
+int ddla_c_gemm_vbatched_mixed(void* handle, int real_operand, char transA, char transB, int* d_m, int* d_n,
+                               int* d_k, const void* alpha, const void* const* d_A, int* d_lda,
+                               const void* const* d_B, int* d_ldb, const void* beta, void** d_C, int* d_ldc,
+                               int batch_count);
+int ddla_z_gemm_vbatched_mixed(void* handle, int real_operand, char transA, char transB, int* d_m, int* d_n,
+                               int* d_k, const void* alpha, const void* const* d_A, int* d_lda,
+                               const void* const* d_B, int* d_ldb, const void* beta, void** d_C, int* d_ldc,
+                               int batch_count);
 
 /* ---- gemmVbatched2s (C++ ddla::gemmVbatched2s, include/ddla/ddla.h:1857-1877;
  * no C entry in the reference): per host segment (segment_sizes is host
@@ -236,6 +309,35 @@ int ddla_d_gemm_vbatched2s(void* handle, char transA_0, char transB_0, int* d_m_
                            const void* const* d_AB_1, int* d_lda_1, int* d_ldb_1, const double* beta_1,
                            void** d_C_1, int* d_ldc_1, int C0_left, int batch_count, const int* segment_sizes,
                            int segment_count);
+
+int ddla_c_gemm_vbatched2s(void* handle, char transA_0, char transB_0, int* d_m_0, int* d_n_0, int* d_k_0,
+                           const void* alpha_0, const void* const* d_A_0, int* d_lda_0, const void* const* d_B_0,
+                           int* d_ldb_0, const void* beta_0, void** d_C_0, int* d_ldc_0, char transA_1,
+                           char transB_1, int* d_m_1, int* d_n_1, int* d_k_1, const void* alpha_1,
+                           const void* const* d_AB_1, int* d_lda_1, int* d_ldb_1, const void* beta_1, void** d_C_1,
+                           int* d_ldc_1, int C0_left, int batch_count, const int* segment_sizes, int segment_count);
+int ddla_z_gemm_vbatched2s(void* handle, char transA_0, char transB_0, int* d_m_0, int* d_n_0, int* d_k_0,
+                           const void* alpha_0, const void* const* d_A_0, int* d_lda_0, const void* const* d_B_0,
+                           int* d_ldb_0, const void* beta_0, void** d_C_0, int* d_ldc_0, char transA_1,
+                           char transB_1, int* d_m_1, int* d_n_1, int* d_k_1, const void* alpha_1,
+                           const void* const* d_AB_1, int* d_lda_1, int* d_ldb_1, const void* beta_1, void** d_C_1,
+                           int* d_ldc_1, int C0_left, int batch_count, const int* segment_sizes, int segment_count);
+/* Mixed two-stage forms: real_operand selects the real stage-0 operand (0: A_0,
+ * 1: B_0); ab1_real != 0 makes the external stage-1 operand real. */
+int ddla_c_gemm_vbatched2s_mixed(void* handle, int real_operand, int ab1_real, char transA_0, char transB_0,
+                                 int* d_m_0, int* d_n_0, int* d_k_0, const void* alpha_0, const void* const* d_A_0,
+                                 int* d_lda_0, const void* const* d_B_0, int* d_ldb_0, const void* beta_0,
+                                 void** d_C_0, int* d_ldc_0, char transA_1, char transB_1, int* d_m_1, int* d_n_1,
+                                 int* d_k_1, const void* alpha_1, const void* const* d_AB_1, int* d_lda_1,
+                                 int* d_ldb_1, const void* beta_1, void** d_C_1, int* d_ldc_1, int C0_left,
+                                 int batch_count, const int* segment_sizes, int segment_count);
+int ddla_z_gemm_vbatched2s_mixed(void* handle, int real_operand, int ab1_real, char transA_0, char transB_0,
+                                 int* d_m_0, int* d_n_0, int* d_k_0, const void* alpha_0, const void* const* d_A_0,
+                                 int* d_lda_0, const void* const* d_B_0, int* d_ldb_0, const void* beta_0,
+                                 void** d_C_0, int* d_ldc_0, char transA_1, char transB_1, int* d_m_1, int* d_n_1,
+                                 int* d_k_1, const void* alpha_1, const void* const* d_AB_1, int* d_lda_1,
+                                 int* d_ldb_1, const void* beta_1, void** d_C_1, int* d_ldc_1, int C0_left,
+                                 int batch_count, const int* segment_sizes, int segment_count);
 
 /* ---- batched SPD solves of uneven order (no counterpart in the reference;
  * per member the semantics of pposv and 2 * Re diag_logdet of the factor):

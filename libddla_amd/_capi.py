@@ -161,6 +161,32 @@ for _p in ("s", "d"):
                                             _i]),
     })
 
+for _p in ("c", "z"):
+    _SIGNATURES.update({
+        f"ddla_{_p}_ppotrf": (_i, [_vp, _c, _i, _vp, _i, _i, _ip, _ip]),
+        f"ddla_{_p}_ppotrs": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _ip, _vp, _ip]),
+        f"ddla_{_p}_pposv": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _i, _i, _ip, _vp, _i, _i, _ip, _ip]),
+        f"ddla_{_p}_ppotrf_workspace_size": (_i, [_vp, _c, _i, _ip, _llp, _llp]),
+        f"ddla_{_p}_ppotrf_run": (_i, [_vp, _c, _i, _vp, _i, _i, _ip, _ip, _vp, _ll, _vp, _ll]),
+        f"ddla_{_p}_ppotrs_workspace_size": (_i, [_vp, _c, _i, _i, _ip, _ip, _llp, _llp]),
+        f"ddla_{_p}_ppotrs_run": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _ip, _vp, _ip, _vp, _ll, _vp, _ll]),
+        f"ddla_{_p}_pposv_workspace_size": (_i, [_vp, _c, _c, _i, _i, _ip, _ip, _llp, _llp]),
+        f"ddla_{_p}_pposv_run": (_i, [_vp, _c, _c, _c, _i, _i, _vp, _i, _i, _ip, _vp, _i, _i, _ip, _ip, _vp, _ll,
+                                      _vp, _ll]),
+        f"ddla_{_p}_potrf": (_i, [_vp, _c, _i, _vp, _i, _vp]),
+        f"ddla_{_p}_diag_logdet": (_i, [_vp, _i, _vp, _i, _vp]),
+        f"ddla_{_p}_pdiag_logdet": (_i, [_vp, _i, _vp, _ip, _vp]),
+        f"ddla_{_p}_gemm_vbatched": (_i, [_vp, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+        f"ddla_{_p}_gemm_vbatched2s": (_i, [_vp, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _ip,
+                                            _i]),
+        f"ddla_{_p}_gemm_vbatched_mixed": (_i, [_vp, _i, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                _vp, _vp, _i]),
+        f"ddla_{_p}_gemm_vbatched2s_mixed": (_i, [_vp, _i, _i, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                  _vp, _vp, _vp, _c, _c, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                  _vp, _vp, _i, _i, _ip, _i]),
+    })
+
 for _name, (_res, _args) in _SIGNATURES.items():
     _fn = getattr(lib, _name)
     _fn.restype = _res

@@ -7,6 +7,7 @@
 
 #include <ddla_mi355x.h>
 
+#include <complex>
 #include <exception>
 #include <string>
 
@@ -50,6 +51,14 @@ int run(void* handle, F&& body)
 }
 
 inline ddla::DdlaHandle_t H(void* p) { return static_cast<ddla::DdlaHandle_t>(p); }
+
+// A scalar passed by address: T itself for the real types, a (real,
+// imaginary) pair behind a void pointer for the complex ones.
+template <class T, class S>
+T scalar(const S* p)
+{
+    return *static_cast<const T*>(static_cast<const void*>(p));
+}
 inline int S(ddla::ddlaStatus_t s) { return static_cast<int>(s); }
 
 } // namespace
@@ -224,7 +233,7 @@ void* ddla_c_get_stream(void* handle)
 }
 
 // ---- drivers ---------------------------------------------------------------
-#define DDLA_C_DRIVERS(P, T)                                                                                     \
+#define DDLA_C_DRIVERS(P, T, R, S)                                                                                  \
     int ddla_##P##_ppotrf(void* handle, char uplo, int n, void* A, int ia, int ja, const int* descA, int* info) \
     {                                                                                                          \
         return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
@@ -331,6 +340,67 @@ void* ddla_c_get_stream(void* handle)
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
     }                                                                                                          \
+    int ddla_##P##_diag_logdet(void* handle, int n, const void* A, int lda, R* result)                         \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (result == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                         \
+            std::complex<R> z;                                                                                 \
+            ddla::diag_logdet<T>(h, n, static_cast<const T*>(A), lda, z);                                      \
+            result[0] = z.real();                                                                              \
+            result[1] = z.imag();                                                                              \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_pdiag_logdet(void* handle, int n, const void* A, const int* descA, R* result)               \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (result == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                         \
+            std::complex<R> z;                                                                                 \
+            ddla::pdiag_logdet<T>(h, n, static_cast<const T*>(A), descA, z);                                   \
+            result[0] = z.real();                                                                              \
+            result[1] = z.imag();                                                                              \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_gemm_vbatched(void* handle, char ta, char tb, int* d_m, int* d_n, int* d_k, const S* alpha, \
+                                 const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb,       \
+                                 const S* beta, void** d_C, int* d_ldc, int batch_count)                       \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (alpha == nullptr || beta == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                       \
+            ddla::gemmVbatched<T>(h, ta, tb, d_m, d_n, d_k, scalar<T>(alpha), reinterpret_cast<const T* const*>(d_A),    \
+                                  d_lda, reinterpret_cast<const T* const*>(d_B), d_ldb, scalar<T>(beta),                 \
+                                  reinterpret_cast<T**>(d_C), d_ldc, batch_count);                             \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }                                                                                                          \
+    int ddla_##P##_gemm_vbatched2s(                                                                            \
+        void* handle, char ta0, char tb0, int* d_m0, int* d_n0, int* d_k0, const S* alpha0,                    \
+        const void* const* d_A0, int* d_lda0, const void* const* d_B0, int* d_ldb0, const S* beta0,            \
+        void** d_C0, int* d_ldc0, char ta1, char tb1, int* d_m1, int* d_n1, int* d_k1, const S* alpha1,        \
+        const void* const* d_AB1, int* d_lda1, int* d_ldb1, const S* beta1, void** d_C1, int* d_ldc1,          \
+        int C0_left, int batch_count, const int* segment_sizes, int segment_count)                             \
+    {                                                                                                          \
+        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
+            if (!alpha0 || !beta0 || !alpha1 || !beta1) return DDLA_C_STATUS_INVALID_VALUE;                    \
+            ddla::gemmVbatched2s<T>(h, ta0, tb0, d_m0, d_n0, d_k0, scalar<T>(alpha0),                                    \
+                                    reinterpret_cast<const T* const*>(d_A0), d_lda0,                           \
+                                    reinterpret_cast<const T* const*>(d_B0), d_ldb0, scalar<T>(beta0),                   \
+                                    reinterpret_cast<T**>(d_C0), d_ldc0, ta1, tb1, d_m1, d_n1, d_k1, scalar<T>(alpha1),  \
+                                    reinterpret_cast<const T* const*>(d_AB1), d_lda1, d_ldb1, scalar<T>(beta1),          \
+                                    reinterpret_cast<T**>(d_C1), d_ldc1, C0_left != 0, batch_count,            \
+                                    segment_sizes, segment_count);                                             \
+            return DDLA_C_STATUS_SUCCESS;                                                                      \
+        });                                                                                                    \
+    }
+
+DDLA_C_DRIVERS(s, float, float, float)
+DDLA_C_DRIVERS(d, double, double, double)
+DDLA_C_DRIVERS(c, std::complex<float>, float, void)
+DDLA_C_DRIVERS(z, std::complex<double>, double, void)
+#undef DDLA_C_DRIVERS
+
+#define DDLA_C_BOTTOM_RIGHT(P, T)                                                                                \
     int ddla_##P##_potrf_bottom_right(void* handle, char uplo, int n, void* A, int lda, int* info)             \
     {                                                                                                          \
         return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
@@ -346,64 +416,80 @@ void* ddla_c_get_stream(void* handle)
             ddla::ppotrf_bottom_right<T>(h, uplo, n, static_cast<T*>(A), descA, *info);                        \
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
-    }                                                                                                          \
-    int ddla_##P##_diag_logdet(void* handle, int n, const void* A, int lda, T* result)                         \
+    }
+
+DDLA_C_BOTTOM_RIGHT(s, float)
+DDLA_C_BOTTOM_RIGHT(d, double)
+#undef DDLA_C_BOTTOM_RIGHT
+
+// Mixed real/complex gemmVbatched(2s) (the C++ overloads of
+// include/ddla/ddla.h:1878-1960; no C entry in the reference): real_operand 0
+// makes A (stage 0: A_0) the real operand, 1 makes B (B_0) real; ab1_real
+// makes the stage-1 external operand real.
+#define DDLA_C_MIXED(P, R)                                                                                     \
+    int ddla_##P##_gemm_vbatched_mixed(void* handle, int real_operand, char ta, char tb, int* d_m, int* d_n,   \
+                                       int* d_k, const void* alpha, const void* const* d_A,          \
+                                       int* d_lda, const void* const* d_B, int* d_ldb,                          \
+                                       const void* beta, void** d_C, int* d_ldc, int batch_count)    \
     {                                                                                                          \
         return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
-            if (result == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                         \
-            std::complex<T> z;                                                                                 \
-            ddla::diag_logdet<T>(h, n, static_cast<const T*>(A), lda, z);                                      \
-            result[0] = z.real();                                                                              \
-            result[1] = z.imag();                                                                              \
+            using C = std::complex<R>;                                                                         \
+            if (alpha == nullptr || beta == nullptr || (real_operand != 0 && real_operand != 1))               \
+                return DDLA_C_STATUS_INVALID_VALUE;                                                            \
+            if (real_operand == 0)                                                                             \
+                ddla::gemmVbatched<R>(h, ta, tb, d_m, d_n, d_k, scalar<C>(alpha), reinterpret_cast<const R* const*>(d_A),\
+                                      d_lda, reinterpret_cast<const C* const*>(d_B), d_ldb, scalar<C>(beta),             \
+                                      reinterpret_cast<C**>(d_C), d_ldc, batch_count);                         \
+            else                                                                                               \
+                ddla::gemmVbatched<R>(h, ta, tb, d_m, d_n, d_k, scalar<C>(alpha), reinterpret_cast<const C* const*>(d_A),\
+                                      d_lda, reinterpret_cast<const R* const*>(d_B), d_ldb, scalar<C>(beta),             \
+                                      reinterpret_cast<C**>(d_C), d_ldc, batch_count);                         \
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
     }                                                                                                          \
-    int ddla_##P##_pdiag_logdet(void* handle, int n, const void* A, const int* descA, T* result)               \
+    int ddla_##P##_gemm_vbatched2s_mixed(                                                                      \
+        void* handle, int real_operand, int ab1_real, char ta0, char tb0, int* d_m0, int* d_n0, int* d_k0,     \
+        const void* alpha0, const void* const* d_A0, int* d_lda0, const void* const* d_B0,          \
+        int* d_ldb0, const void* beta0, void** d_C0, int* d_ldc0, char ta1, char tb1, int* d_m1,    \
+        int* d_n1, int* d_k1, const void* alpha1, const void* const* d_AB1, int* d_lda1,            \
+        int* d_ldb1, const void* beta1, void** d_C1, int* d_ldc1, int C0_left, int batch_count,     \
+        const int* segment_sizes, int segment_count)                                                           \
     {                                                                                                          \
         return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
-            if (result == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                                         \
-            std::complex<T> z;                                                                                 \
-            ddla::pdiag_logdet<T>(h, n, static_cast<const T*>(A), descA, z);                                   \
-            result[0] = z.real();                                                                              \
-            result[1] = z.imag();                                                                              \
-            return DDLA_C_STATUS_SUCCESS;                                                                      \
-        });                                                                                                    \
-    }                                                                                                          \
-    int ddla_##P##_gemm_vbatched(void* handle, char ta, char tb, int* d_m, int* d_n, int* d_k, const T* alpha, \
-                                 const void* const* d_A, int* d_lda, const void* const* d_B, int* d_ldb,       \
-                                 const T* beta, void** d_C, int* d_ldc, int batch_count)                       \
-    {                                                                                                          \
-        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
-            if (alpha == nullptr || beta == nullptr) return DDLA_C_STATUS_INVALID_VALUE;                       \
-            ddla::gemmVbatched<T>(h, ta, tb, d_m, d_n, d_k, *alpha, reinterpret_cast<const T* const*>(d_A),    \
-                                  d_lda, reinterpret_cast<const T* const*>(d_B), d_ldb, *beta,                 \
-                                  reinterpret_cast<T**>(d_C), d_ldc, batch_count);                             \
-            return DDLA_C_STATUS_SUCCESS;                                                                      \
-        });                                                                                                    \
-    }                                                                                                          \
-    int ddla_##P##_gemm_vbatched2s(                                                                            \
-        void* handle, char ta0, char tb0, int* d_m0, int* d_n0, int* d_k0, const T* alpha0,                    \
-        const void* const* d_A0, int* d_lda0, const void* const* d_B0, int* d_ldb0, const T* beta0,            \
-        void** d_C0, int* d_ldc0, char ta1, char tb1, int* d_m1, int* d_n1, int* d_k1, const T* alpha1,        \
-        const void* const* d_AB1, int* d_lda1, int* d_ldb1, const T* beta1, void** d_C1, int* d_ldc1,          \
-        int C0_left, int batch_count, const int* segment_sizes, int segment_count)                             \
-    {                                                                                                          \
-        return run(handle, [&](ddla::DdlaHandle_t h) -> int {                                                  \
-            if (!alpha0 || !beta0 || !alpha1 || !beta1) return DDLA_C_STATUS_INVALID_VALUE;                    \
-            ddla::gemmVbatched2s<T>(h, ta0, tb0, d_m0, d_n0, d_k0, *alpha0,                                    \
-                                    reinterpret_cast<const T* const*>(d_A0), d_lda0,                           \
-                                    reinterpret_cast<const T* const*>(d_B0), d_ldb0, *beta0,                   \
-                                    reinterpret_cast<T**>(d_C0), d_ldc0, ta1, tb1, d_m1, d_n1, d_k1, *alpha1,  \
-                                    reinterpret_cast<const T* const*>(d_AB1), d_lda1, d_ldb1, *beta1,          \
-                                    reinterpret_cast<T**>(d_C1), d_ldc1, C0_left != 0, batch_count,            \
-                                    segment_sizes, segment_count);                                             \
+            using C = std::complex<R>;                                                                         \
+            if (!alpha0 || !beta0 || !alpha1 || !beta1 || (real_operand != 0 && real_operand != 1))            \
+                return DDLA_C_STATUS_INVALID_VALUE;                                                            \
+            auto cc = [](const void* const* p) { return reinterpret_cast<const C* const*>(p); };               \
+            auto rr = [](const void* const* p) { return reinterpret_cast<const R* const*>(p); };               \
+            C** c0 = reinterpret_cast<C**>(d_C0);                                                              \
+            C** c1 = reinterpret_cast<C**>(d_C1);                                                              \
+            if (real_operand == 0 && ab1_real)                                                                 \
+                ddla::gemmVbatched2s<R>(h, ta0, tb0, d_m0, d_n0, d_k0, scalar<C>(alpha0), rr(d_A0), d_lda0, cc(d_B0),     \
+                                        d_ldb0, scalar<C>(beta0), c0, d_ldc0, ta1, tb1, d_m1, d_n1, d_k1, scalar<C>(alpha1),        \
+                                        rr(d_AB1), d_lda1, d_ldb1, scalar<C>(beta1), c1, d_ldc1, C0_left != 0,            \
+                                        batch_count, segment_sizes, segment_count);                            \
+            else if (real_operand == 1 && ab1_real)                                                            \
+                ddla::gemmVbatched2s<R>(h, ta0, tb0, d_m0, d_n0, d_k0, scalar<C>(alpha0), cc(d_A0), d_lda0, rr(d_B0),     \
+                                        d_ldb0, scalar<C>(beta0), c0, d_ldc0, ta1, tb1, d_m1, d_n1, d_k1, scalar<C>(alpha1),        \
+                                        rr(d_AB1), d_lda1, d_ldb1, scalar<C>(beta1), c1, d_ldc1, C0_left != 0,            \
+                                        batch_count, segment_sizes, segment_count);                            \
+            else if (real_operand == 0)                                                                        \
+                ddla::gemmVbatched2s<R>(h, ta0, tb0, d_m0, d_n0, d_k0, scalar<C>(alpha0), rr(d_A0), d_lda0, cc(d_B0),     \
+                                        d_ldb0, scalar<C>(beta0), c0, d_ldc0, ta1, tb1, d_m1, d_n1, d_k1, scalar<C>(alpha1),        \
+                                        cc(d_AB1), d_lda1, d_ldb1, scalar<C>(beta1), c1, d_ldc1, C0_left != 0,            \
+                                        batch_count, segment_sizes, segment_count);                            \
+            else                                                                                               \
+                ddla::gemmVbatched2s<R>(h, ta0, tb0, d_m0, d_n0, d_k0, scalar<C>(alpha0), cc(d_A0), d_lda0, rr(d_B0),     \
+                                        d_ldb0, scalar<C>(beta0), c0, d_ldc0, ta1, tb1, d_m1, d_n1, d_k1, scalar<C>(alpha1),        \
+                                        cc(d_AB1), d_lda1, d_ldb1, scalar<C>(beta1), c1, d_ldc1, C0_left != 0,            \
+                                        batch_count, segment_sizes, segment_count);                            \
             return DDLA_C_STATUS_SUCCESS;                                                                      \
         });                                                                                                    \
     }
 
-DDLA_C_DRIVERS(s, float)
-DDLA_C_DRIVERS(d, double)
-#undef DDLA_C_DRIVERS
+DDLA_C_MIXED(c, float)
+DDLA_C_MIXED(z, double)
+#undef DDLA_C_MIXED
 
 // ---- triangular solves -----------------------------------------------------------
 #define DDLA_C_TRIANGULAR(P, T)                                                                                \

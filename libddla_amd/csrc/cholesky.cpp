@@ -14,6 +14,7 @@
 // the blocked right-looking order of src/Cholesky/ppotrf.cpp:108-322.
 #include "internal.h"
 #include "kernels.h"
+#include "zkernels.h"
 
 #include <atomic>
 #include <cmath>
@@ -260,8 +261,12 @@ void ppotrf_workspace_size(const DdlaHandle_t&, char, int n, T*, const int*, std
 {
     *backend_bytes = *host_bytes = 0;
     if (n <= 0) return;
-    // [ info word | panel work: diagonal-block inverses, panel counters, error word ]
-    *backend_bytes = detail::padded_bytes<int>(1) + detail::padded_bytes<char>(kern::panel_work_bytes<T>(n));
+    if constexpr (is_complex<T>::value) {
+        *backend_bytes = complex_impl::ppotrf_bytes<typename real_type<T>::type>(n);
+    } else {
+        // [ info word | panel work: diagonal-block inverses, panel counters, error word ]
+        *backend_bytes = detail::padded_bytes<int>(1) + detail::padded_bytes<char>(kern::panel_work_bytes<T>(n));
+    }
 }
 
 template <class T>
@@ -272,20 +277,25 @@ void ppotrf_run(const DdlaHandle_t& h, char uplo, int n, T* A, int, int, const i
         info = 0;
         return;
     }
-    std::size_t need = 0, host = 0;
-    ppotrf_workspace_size<T>(h, uplo, n, A, d, &need, &host);
-    if (backend_bytes < need || backend_work == nullptr) bad_workspace("ppotrf");
-    char* w = static_cast<char*>(backend_work);
-    int* d_info = detail::take<int>(w, 1);
-    char* panel = detail::take<char>(w, kern::panel_work_bytes<T>(n));
-    fill_int(h->stream, d_info, 0);
-    potrf_blocked<T>(h, uplo == 'U', n, A, d[DDLA_LLD_], d_info, panel);
-    int error = 0;
-    DDLA_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    DDLA_HIP(hipMemcpyAsync(&error, kern::panel_error_word<T>(n, panel), sizeof(int), hipMemcpyDeviceToHost,
-                            h->stream));
-    DDLA_HIP(hipStreamSynchronize(h->stream));
-    if (error != 0) throw std::runtime_error("ppotrf: a panel factorization wait exceeded its limit");
+    if constexpr (is_complex<T>::value) {
+        complex_impl::ppotrf_run<typename real_type<T>::type>(h, uplo, n, A, d, info, backend_work, backend_bytes);
+        return;
+    } else {
+        std::size_t need = 0, host = 0;
+        ppotrf_workspace_size<T>(h, uplo, n, A, d, &need, &host);
+        if (backend_bytes < need || backend_work == nullptr) bad_workspace("ppotrf");
+        char* w = static_cast<char*>(backend_work);
+        int* d_info = detail::take<int>(w, 1);
+        char* panel = detail::take<char>(w, kern::panel_work_bytes<T>(n));
+        fill_int(h->stream, d_info, 0);
+        potrf_blocked<T>(h, uplo == 'U', n, A, d[DDLA_LLD_], d_info, panel);
+        int error = 0;
+        DDLA_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        DDLA_HIP(hipMemcpyAsync(&error, kern::panel_error_word<T>(n, panel), sizeof(int), hipMemcpyDeviceToHost,
+                                h->stream));
+        DDLA_HIP(hipStreamSynchronize(h->stream));
+        if (error != 0) throw std::runtime_error("ppotrf: a panel factorization wait exceeded its limit");
+    }
 }
 
 void ppotrs_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A,
@@ -326,7 +336,10 @@ void ppotrs_workspace_size(const DdlaHandle_t&, char side, int n, int nrhs, cons
                            std::size_t* backend_bytes, std::size_t* host_bytes)
 {
     *host_bytes = 0;
-    *backend_bytes = ppotrs_bytes<T>(side, n, nrhs, true);
+    if constexpr (is_complex<T>::value)
+        *backend_bytes = complex_impl::ppotrs_bytes<typename real_type<T>::type>(side, n, nrhs);
+    else
+        *backend_bytes = ppotrs_bytes<T>(side, n, nrhs, true);
 }
 
 template <class T>
@@ -370,7 +383,11 @@ template <class T>
 void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, char, int n, int nrhs, const T* A, const int* da, T* B,
                 const int* db, void* backend_work, std::size_t backend_bytes, void*, std::size_t)
 {
-    ppotrs_exec<T>(h, side, uplo, n, nrhs, A, da, B, db, backend_work, backend_bytes, true);
+    if constexpr (is_complex<T>::value)
+        complex_impl::ppotrs_run<typename real_type<T>::type>(h, side, uplo, n, nrhs, A, da, B, db, backend_work,
+                                                              backend_bytes);
+    else
+        ppotrs_exec<T>(h, side, uplo, n, nrhs, A, da, B, db, backend_work, backend_bytes, true);
 }
 
 void pposv_check(const DdlaHandle_t& h, char side, char uplo, char trans, int n, int nrhs, const void* A, int ia,
@@ -496,9 +513,13 @@ void potrf(const DdlaHandle_t& h, char uplo, int n, T* A, int lda, int* info)
     }
     if (A == nullptr) throw std::runtime_error("potrf: null A");
     if (lda < std::max(1, n)) throw std::runtime_error("potrf: lda too small");
-    void* work = h->ensure_scratch(kern::panel_work_bytes<T>(n));
-    fill_int(h->stream, info, 0);
-    potrf_blocked<T>(h, u == 'U', n, A, lda, info, work);
+    if constexpr (is_complex<T>::value) {
+        complex_impl::potrf<typename real_type<T>::type>(h, u == 'U', n, A, lda, info);
+    } else {
+        void* work = h->ensure_scratch(kern::panel_work_bytes<T>(n));
+        fill_int(h->stream, info, 0);
+        potrf_blocked<T>(h, u == 'U', n, A, lda, info, work);
+    }
 }
 
 template <class T>
@@ -517,22 +538,27 @@ void diag_logdet(const DdlaHandle_t& h, int n, const T* a, int lda, std::complex
         result = {R(0), R(0)};
         return;
     }
-    const std::size_t part = detail::padded_bytes<char>(kern::logdet_work_bytes());
-    char* w = static_cast<char*>(h->ensure_scratch(part + detail::padded_bytes<kern::LogdetResult>(1)));
-    auto* d_out = reinterpret_cast<kern::LogdetResult*>(w + part);
-    kern::diag_logdet<T>(h->stream, n, a, lda, w, d_out);
-    kern::LogdetResult out{};
-    DDLA_HIP(hipMemcpyAsync(&out, d_out, sizeof(out), hipMemcpyDeviceToHost, h->stream));
-    DDLA_HIP(hipStreamSynchronize(h->stream));
-    if (out.flags & 1) throw std::domain_error("diag_logdet: nonfinite diagonal");
-    if (out.flags & 2) {
-        result = {-std::numeric_limits<R>::infinity(), R(0)};
+    if constexpr (is_complex<T>::value) {
+        complex_impl::diag_logdet<R>(h, n, a, lda, result);
         return;
+    } else {
+        const std::size_t part = detail::padded_bytes<char>(kern::logdet_work_bytes());
+        char* w = static_cast<char*>(h->ensure_scratch(part + detail::padded_bytes<kern::LogdetResult>(1)));
+        auto* d_out = reinterpret_cast<kern::LogdetResult*>(w + part);
+        kern::diag_logdet<T>(h->stream, n, a, lda, w, d_out);
+        kern::LogdetResult out{};
+        DDLA_HIP(hipMemcpyAsync(&out, d_out, sizeof(out), hipMemcpyDeviceToHost, h->stream));
+        DDLA_HIP(hipStreamSynchronize(h->stream));
+        if (out.flags & 1) throw std::domain_error("diag_logdet: nonfinite diagonal");
+        if (out.flags & 2) {
+            result = {-std::numeric_limits<R>::infinity(), R(0)};
+            return;
+        }
+        const double pi = std::acos(-1.0);
+        double phase = std::remainder(((out.flags >> 2) & 1) * pi, 2 * pi);
+        if (phase <= -pi) phase = pi;
+        result = {static_cast<R>(out.magnitude), static_cast<R>(phase)};
     }
-    const double pi = std::acos(-1.0);
-    double phase = std::remainder(((out.flags >> 2) & 1) * pi, 2 * pi);
-    if (phase <= -pi) phase = pi;
-    result = {static_cast<R>(out.magnitude), static_cast<R>(phase)};
 }
 
 namespace {
@@ -594,8 +620,13 @@ void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* d_m, int
     int* summary = static_cast<int*>(h->ensure_scratch(detail::padded_bytes<int>(4)));
     int mx[3];
     vbatched_maxima(h, "gemmVbatched", transA, transB, d_m, d_n, d_k, d_lda, d_ldb, d_ldc, batch, summary, mx);
-    kern::gemm_vbatched<T>(h->stream, transA, transB, d_m, d_n, d_k, alpha, d_A, d_lda, d_B, d_ldb, beta, d_C,
-                           d_ldc, batch, mx[0], mx[1]);
+    if constexpr (is_complex<T>::value)
+        kern::zgemm_vbatched<typename real_type<T>::type>(
+            h->stream, true, true, transA, transB, d_m, d_n, d_k, alpha, reinterpret_cast<const void* const*>(d_A),
+            d_lda, reinterpret_cast<const void* const*>(d_B), d_ldb, beta, d_C, d_ldc, batch, mx[0], mx[1]);
+    else
+        kern::gemm_vbatched<T>(h->stream, transA, transB, d_m, d_n, d_k, alpha, d_A, d_lda, d_B, d_ldb, beta, d_C,
+                               d_ldc, batch, mx[0], mx[1]);
 }
 
 // src/BLAS/gemm_vbatched.cpp:420-523 and
@@ -637,15 +668,149 @@ void gemmVbatched2s(const DdlaHandle_t& h, char transA_0, char transB_0, int* d_
     const T* const* C0 = const_cast<const T* const*>(d_C_0);
     for (int s = 0, c = 0; s < segment_count; c += segment_sizes[s], ++s) {
         const int nb = segment_sizes[s];
-        kern::gemm_vbatched<T>(h->stream, transA_0, transB_0, d_m_0 + c, d_n_0 + c, d_k_0 + c, alpha_0, d_A_0 + c,
-                               d_lda_0 + c, d_B_0 + c, d_ldb_0 + c, beta_0, d_C_0 + c, d_ldc_0 + c, nb, mx0[0],
-                               mx0[1]);
         const T* const* left = C0_left ? C0 + c : d_AB_1 + c;
         const T* const* right = C0_left ? d_AB_1 + c : C0 + c;
-        kern::gemm_vbatched<T>(h->stream, transA_1, transB_1, d_m_1 + c, d_n_1 + c, d_k_1 + c, alpha_1, left,
-                               d_lda_1 + c, right, d_ldb_1 + c, beta_1, d_C_1 + c, d_ldc_1 + c, nb, mx1[0], mx1[1]);
+        if constexpr (is_complex<T>::value) {
+            using R = typename real_type<T>::type;
+            auto vp = [](const T* const* p) { return reinterpret_cast<const void* const*>(p); };
+            kern::zgemm_vbatched<R>(h->stream, true, true, transA_0, transB_0, d_m_0 + c, d_n_0 + c, d_k_0 + c,
+                                    alpha_0, vp(d_A_0 + c), d_lda_0 + c, vp(d_B_0 + c), d_ldb_0 + c, beta_0,
+                                    d_C_0 + c, d_ldc_0 + c, nb, mx0[0], mx0[1]);
+            kern::zgemm_vbatched<R>(h->stream, true, true, transA_1, transB_1, d_m_1 + c, d_n_1 + c, d_k_1 + c,
+                                    alpha_1, vp(left), d_lda_1 + c, vp(right), d_ldb_1 + c, beta_1, d_C_1 + c,
+                                    d_ldc_1 + c, nb, mx1[0], mx1[1]);
+        } else {
+            kern::gemm_vbatched<T>(h->stream, transA_0, transB_0, d_m_0 + c, d_n_0 + c, d_k_0 + c, alpha_0,
+                                   d_A_0 + c, d_lda_0 + c, d_B_0 + c, d_ldb_0 + c, beta_0, d_C_0 + c, d_ldc_0 + c,
+                                   nb, mx0[0], mx0[1]);
+            kern::gemm_vbatched<T>(h->stream, transA_1, transB_1, d_m_1 + c, d_n_1 + c, d_k_1 + c, alpha_1, left,
+                                   d_lda_1 + c, right, d_ldb_1 + c, beta_1, d_C_1 + c, d_ldc_1 + c, nb, mx1[0],
+                                   mx1[1]);
+        }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Mixed real/complex gemmVbatched and gemmVbatched2s (the reference's
+// include/ddla/ddla.h:1878-1960): one operand real, output and scalars
+// complex; 'C' on a real operand is 'T'.  Same validation and zero-product
+// rules as the one-type overloads.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class R>
+void mixed_vbatched(const DdlaHandle_t& h, const char* who, bool a_complex, bool b_complex, char transA, char transB,
+                    int* d_m, int* d_n, int* d_k, std::complex<R> alpha, const void* const* d_A, int* d_lda,
+                    const void* const* d_B, int* d_ldb, std::complex<R> beta, std::complex<R>** d_C, int* d_ldc,
+                    int batch)
+{
+    check_vbatched_handle(h, who);
+    if (batch < 0) throw std::invalid_argument(std::string(who) + ": negative batch count");
+    if (batch == 0) return;
+    if (d_A == nullptr || d_B == nullptr || d_C == nullptr)
+        throw std::invalid_argument(std::string(who) + ": null matrix pointer array");
+    if (!valid_op(transA) || !valid_op(transB))
+        throw std::invalid_argument(std::string(who) + ": invalid transpose operation");
+    int* summary = static_cast<int*>(h->ensure_scratch(detail::padded_bytes<int>(4)));
+    int mx[3];
+    vbatched_maxima(h, who, transA, transB, d_m, d_n, d_k, d_lda, d_ldb, d_ldc, batch, summary, mx);
+    kern::zgemm_vbatched<R>(h->stream, a_complex, b_complex, transA, transB, d_m, d_n, d_k, alpha, d_A, d_lda, d_B,
+                            d_ldb, beta, d_C, d_ldc, batch, mx[0], mx[1]);
+}
+
+// Stage 0 with operand kinds (a0, b0), stage 1 with the complex intermediate
+// C0 and an external operand AB1 of kind ab1 (complex or real).
+template <class R>
+void mixed_vbatched2s(const DdlaHandle_t& h, bool a0_complex, bool b0_complex, bool ab1_complex, char transA_0,
+                      char transB_0, int* d_m_0, int* d_n_0, int* d_k_0, std::complex<R> alpha_0,
+                      const void* const* d_A_0, int* d_lda_0, const void* const* d_B_0, int* d_ldb_0,
+                      std::complex<R> beta_0, std::complex<R>** d_C_0, int* d_ldc_0, char transA_1, char transB_1,
+                      int* d_m_1, int* d_n_1, int* d_k_1, std::complex<R> alpha_1, const void* const* d_AB_1,
+                      int* d_lda_1, int* d_ldb_1, std::complex<R> beta_1, std::complex<R>** d_C_1, int* d_ldc_1,
+                      bool C0_left, int batch, const int* segment_sizes, int segment_count)
+{
+    const char* who = "gemmVbatched2s";
+    check_vbatched_handle(h, who);
+    if (batch < 0) throw std::invalid_argument("gemmVbatched2s: negative batch count");
+    if (batch == 0) return;
+    if (beta_0 != std::complex<R>(0)) throw std::invalid_argument("gemmVbatched2s: stage-zero beta must be zero");
+    if (segment_sizes == nullptr || segment_count <= 0)
+        throw std::invalid_argument("gemmVbatched2s: missing segment sizes");
+    long long total = 0;
+    for (int s = 0; s < segment_count; ++s) {
+        if (segment_sizes[s] <= 0) throw std::invalid_argument("gemmVbatched2s: segment sizes must be positive");
+        total += segment_sizes[s];
+    }
+    if (total != batch) throw std::invalid_argument("gemmVbatched2s: segment total does not match batch count");
+    if (!d_A_0 || !d_B_0 || !d_C_0 || !d_AB_1 || !d_C_1)
+        throw std::invalid_argument("gemmVbatched2s: null matrix pointer array");
+    if (!valid_op(transA_0) || !valid_op(transB_0) || !valid_op(transA_1) || !valid_op(transB_1))
+        throw std::invalid_argument("gemmVbatched2s: invalid transpose operation");
+    int* summary = static_cast<int*>(h->ensure_scratch(detail::padded_bytes<int>(4)));
+    int mx0[3], mx1[3];
+    vbatched_maxima(h, who, transA_0, transB_0, d_m_0, d_n_0, d_k_0, d_lda_0, d_ldb_0, d_ldc_0, batch, summary, mx0);
+    vbatched_maxima(h, who, transA_1, transB_1, d_m_1, d_n_1, d_k_1, d_lda_1, d_ldb_1, d_ldc_1, batch, summary, mx1);
+    const void* const* C0 = reinterpret_cast<const void* const*>(d_C_0);
+    for (int s = 0, c = 0; s < segment_count; c += segment_sizes[s], ++s) {
+        const int nb = segment_sizes[s];
+        kern::zgemm_vbatched<R>(h->stream, a0_complex, b0_complex, transA_0, transB_0, d_m_0 + c, d_n_0 + c,
+                                d_k_0 + c, alpha_0, d_A_0 + c, d_lda_0 + c, d_B_0 + c, d_ldb_0 + c, beta_0, d_C_0 + c,
+                                d_ldc_0 + c, nb, mx0[0], mx0[1]);
+        const void* const* left = C0_left ? C0 + c : d_AB_1 + c;
+        const void* const* right = C0_left ? d_AB_1 + c : C0 + c;
+        const bool left_complex = C0_left ? true : ab1_complex, right_complex = C0_left ? ab1_complex : true;
+        kern::zgemm_vbatched<R>(h->stream, left_complex, right_complex, transA_1, transB_1, d_m_1 + c, d_n_1 + c,
+                                d_k_1 + c, alpha_1, left, d_lda_1 + c, right, d_ldb_1 + c, beta_1, d_C_1 + c,
+                                d_ldc_1 + c, nb, mx1[0], mx1[1]);
+    }
+}
+
+template <class P>
+const void* const* vpa(const P* const* p)
+{
+    return reinterpret_cast<const void* const*>(p);
+}
+
+} // namespace
+
+template <typename R>
+void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* m, int* n, int* k, std::complex<R> alpha,
+                  const R* const* A, int* lda, const std::complex<R>* const* B, int* ldb, std::complex<R> beta,
+                  std::complex<R>** C, int* ldc, int batch_count)
+{
+    mixed_vbatched<R>(h, "gemmVbatched", false, true, transA, transB, m, n, k, alpha, vpa(A), lda, vpa(B), ldb, beta,
+                      C, ldc, batch_count);
+}
+
+template <typename R>
+void gemmVbatched(const DdlaHandle_t& h, char transA, char transB, int* m, int* n, int* k, std::complex<R> alpha,
+                  const std::complex<R>* const* A, int* lda, const R* const* B, int* ldb, std::complex<R> beta,
+                  std::complex<R>** C, int* ldc, int batch_count)
+{
+    mixed_vbatched<R>(h, "gemmVbatched", true, false, transA, transB, m, n, k, alpha, vpa(A), lda, vpa(B), ldb, beta,
+                      C, ldc, batch_count);
+}
+
+#define DDLA_MIXED2S(A0T, B0T, AB1T, A0C, B0C, AB1C)                                                             \
+    template <typename R>                                                                                        \
+    void gemmVbatched2s(const DdlaHandle_t& h, char transA_0, char transB_0, int* m_0, int* n_0, int* k_0,      \
+                        std::complex<R> alpha_0, const A0T* const* A_0, int* lda_0, const B0T* const* B_0,       \
+                        int* ldb_0, std::complex<R> beta_0, std::complex<R>** C_0, int* ldc_0, char transA_1,   \
+                        char transB_1, int* m_1, int* n_1, int* k_1, std::complex<R> alpha_1,                   \
+                        const AB1T* const* AB_1, int* lda_1, int* ldb_1, std::complex<R> beta_1,                \
+                        std::complex<R>** C_1, int* ldc_1, bool C0_left, int batch_count,                       \
+                        const int* segment_sizes, int segment_count)                                             \
+    {                                                                                                            \
+        mixed_vbatched2s<R>(h, A0C, B0C, AB1C, transA_0, transB_0, m_0, n_0, k_0, alpha_0, vpa(A_0), lda_0,     \
+                            vpa(B_0), ldb_0, beta_0, C_0, ldc_0, transA_1, transB_1, m_1, n_1, k_1, alpha_1,    \
+                            vpa(AB_1), lda_1, ldb_1, beta_1, C_1, ldc_1, C0_left, batch_count, segment_sizes,   \
+                            segment_count);                                                                      \
+    }
+DDLA_MIXED2S(R, std::complex<R>, R, false, true, false)
+DDLA_MIXED2S(std::complex<R>, R, R, true, false, false)
+DDLA_MIXED2S(R, std::complex<R>, std::complex<R>, false, true, true)
+DDLA_MIXED2S(std::complex<R>, R, std::complex<R>, true, false, true)
+#undef DDLA_MIXED2S
 
 template <class T>
 void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work)
@@ -693,6 +858,39 @@ void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long 
                                               void*, std::size_t);
 DDLA_INSTANTIATE(float)
 DDLA_INSTANTIATE(double)
+DDLA_INSTANTIATE(std::complex<float>)
+DDLA_INSTANTIATE(std::complex<double>)
 #undef DDLA_INSTANTIATE
+
+#define DDLA_MIXED_INSTANTIATE(R)                                                                                \
+    template void gemmVbatched<R>(const DdlaHandle_t&, char, char, int*, int*, int*, std::complex<R>,           \
+                                  const R* const*, int*, const std::complex<R>* const*, int*, std::complex<R>,  \
+                                  std::complex<R>**, int*, int);                                                 \
+    template void gemmVbatched<R>(const DdlaHandle_t&, char, char, int*, int*, int*, std::complex<R>,           \
+                                  const std::complex<R>* const*, int*, const R* const*, int*, std::complex<R>,  \
+                                  std::complex<R>**, int*, int);                                                 \
+    template void gemmVbatched2s<R>(const DdlaHandle_t&, char, char, int*, int*, int*, std::complex<R>,         \
+                                    const R* const*, int*, const std::complex<R>* const*, int*, std::complex<R>, \
+                                    std::complex<R>**, int*, char, char, int*, int*, int*, std::complex<R>,      \
+                                    const R* const*, int*, int*, std::complex<R>, std::complex<R>**, int*, bool, \
+                                    int, const int*, int);                                                       \
+    template void gemmVbatched2s<R>(const DdlaHandle_t&, char, char, int*, int*, int*, std::complex<R>,         \
+                                    const std::complex<R>* const*, int*, const R* const*, int*, std::complex<R>, \
+                                    std::complex<R>**, int*, char, char, int*, int*, int*, std::complex<R>,      \
+                                    const R* const*, int*, int*, std::complex<R>, std::complex<R>**, int*, bool, \
+                                    int, const int*, int);                                                       \
+    template void gemmVbatched2s<R>(const DdlaHandle_t&, char, char, int*, int*, int*, std::complex<R>,         \
+                                    const R* const*, int*, const std::complex<R>* const*, int*, std::complex<R>, \
+                                    std::complex<R>**, int*, char, char, int*, int*, int*, std::complex<R>,      \
+                                    const std::complex<R>* const*, int*, int*, std::complex<R>,                 \
+                                    std::complex<R>**, int*, bool, int, const int*, int);                       \
+    template void gemmVbatched2s<R>(const DdlaHandle_t&, char, char, int*, int*, int*, std::complex<R>,         \
+                                    const std::complex<R>* const*, int*, const R* const*, int*, std::complex<R>, \
+                                    std::complex<R>**, int*, char, char, int*, int*, int*, std::complex<R>,      \
+                                    const std::complex<R>* const*, int*, int*, std::complex<R>,                 \
+                                    std::complex<R>**, int*, bool, int, const int*, int);
+DDLA_MIXED_INSTANTIATE(float)
+DDLA_MIXED_INSTANTIATE(double)
+#undef DDLA_MIXED_INSTANTIATE
 
 } // namespace ddla

@@ -5,7 +5,9 @@
 
 #include <ddla/ddla.h>
 
+#include <complex>
 #include <cstddef>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -122,6 +124,27 @@ void fill_int(hipStream_t s, int* p, int value);
 // 64 within [64, 768] (the split-product path caps it at 512).  Returns the
 // previous override (0 = none).
 int set_panel_width(int cols);
+
+template <class T> struct is_complex : std::false_type {};
+template <class R> struct is_complex<std::complex<R>> : std::true_type {};
+
+// Hermitian path of the complex types (complex.cpp); arguments already checked.
+namespace complex_impl {
+template <class R>
+std::size_t ppotrf_bytes(int n);
+template <class R>
+std::size_t ppotrs_bytes(char side, int n, int nrhs);
+template <class R>
+void ppotrf_run(const DdlaHandle_t& h, char uplo, int n, std::complex<R>* A, const int* descA, int& info, void* work,
+                std::size_t bytes);
+template <class R>
+void ppotrs_run(const DdlaHandle_t& h, char side, char uplo, int n, int nrhs, const std::complex<R>* A,
+                const int* descA, std::complex<R>* B, const int* descB, void* work, std::size_t bytes);
+template <class R>
+void potrf(const DdlaHandle_t& h, bool upper, int n, std::complex<R>* A, int lda, int* info);
+template <class R>
+void diag_logdet(const DdlaHandle_t& h, int n, const std::complex<R>* A, int lda, std::complex<R>& result);
+} // namespace complex_impl
 
 // pposv with the 64-row sweeps whatever the order (no staged-solve work, so
 // the workspace stays at about 512 bytes per row): its backend bytes and its

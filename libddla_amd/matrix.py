@@ -11,7 +11,8 @@ import numpy as np
 
 from ._capi import LLD_, check_status, lib
 
-_TYPE_CHARS = {np.dtype(np.float32): "s", np.dtype(np.float64): "d"}
+_TYPE_CHARS = {np.dtype(np.float32): "s", np.dtype(np.float64): "d", np.dtype(np.complex64): "c",
+               np.dtype(np.complex128): "z"}
 
 
 def descinit(handle, m, n, mb, nb, rsrc=0, csrc=0, lld=-1) -> np.ndarray:
@@ -27,7 +28,7 @@ class DistributedMatrix:
         self.handle = handle
         self.dtype = np.dtype(dtype)
         if self.dtype not in _TYPE_CHARS:
-            raise TypeError(f"unsupported dtype {self.dtype}; float32 and float64 are built")
+            raise TypeError(f"unsupported dtype {self.dtype}; float32, float64, complex64 and complex128 are built")
         self.type_char = _TYPE_CHARS[self.dtype]
         self.m, self.n, self.mb, self.nb = m, n, mb, nb
         self.desc = descinit(handle, m, n, mb, nb, rsrc, csrc, lld)
@@ -108,7 +109,8 @@ class DistributedMatrix:
 
         if not tensor.is_contiguous() or tensor.dim() != 2 or tensor.shape[0] != n:
             raise ValueError("expected a contiguous (n, lld) tensor holding a column-major matrix")
-        dtype = {torch.float32: np.float32, torch.float64: np.float64}[tensor.dtype]
+        dtype = {torch.float32: np.float32, torch.float64: np.float64, torch.complex64: np.complex64,
+                 torch.complex128: np.complex128}[tensor.dtype]
         return cls(handle, m, n, mb, nb, dtype, lld=tensor.shape[1], data_ptr=tensor.data_ptr(), owner=tensor)
 
     def __repr__(self):

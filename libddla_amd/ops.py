@@ -199,7 +199,7 @@ def pposv_mixed_run(A, B, uplo, backend_work: int, backend_bytes: int, factor: i
 def diag_logdet(A, n: int = None):
     """Principal log of the product of the diagonal: complex (sum log|d|, phase)."""
     n = A.m if n is None else n
-    res = np.zeros(2, dtype=A.dtype)
+    res = np.zeros(2, dtype=np.finfo(A.dtype).dtype)  # the real type of A's elements
     st = _fn(A, "diag_logdet")(A.handle.ptr, n, ctypes.c_void_p(A.data_ptr), A.lld,
                                ctypes.c_void_p(res.ctypes.data))
     check_status(st, f"ddla_{A.type_char}_diag_logdet")
@@ -209,7 +209,7 @@ def diag_logdet(A, n: int = None):
 def pdiag_logdet(A, n: int = None):
     """diag_logdet of the leading n x n block, through A's descriptor (the reference's pdiag_logdet)."""
     n = A.m if n is None else n
-    res = np.zeros(2, dtype=A.dtype)
+    res = np.zeros(2, dtype=np.finfo(A.dtype).dtype)
     st = _fn(A, "pdiag_logdet")(A.handle.ptr, n, ctypes.c_void_p(A.data_ptr), A.desc_ptr,
                                 ctypes.c_void_p(res.ctypes.data))
     check_status(st, f"ddla_{A.type_char}_pdiag_logdet")

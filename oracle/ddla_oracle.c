@@ -236,3 +236,174 @@ void NAME(char ta, char tb, int m, int n, int k, T alpha, const T* A, int lda,  
 }
 DEFINE_GEMM(oracle_dgemm, double, double)
 DEFINE_GEMM(oracle_sgemm, float, double)
+
+/* ======================================================================== */
+/* Complex (Hermitian) routines.  Matrices are interleaved (real, imaginary) */
+/* column-major arrays, the layout of std::complex<R> and of C99 complex.    */
+/* ======================================================================== */
+#include <complex.h>
+
+/* ------------------------------------------------------------------------ */
+/* Hermitian ?potrf: the unblocked Cholesky of LAPACK zpotf2 / cpotf2, which */
+/* the reference's CPU path calls for complex types (src/Cholesky/           */
+/* potrf.cpp:51-59 cpotrf_/zpotrf_).  uplo 'L': A = L L^H,                   */
+/*   L(j,j)^2 = Re A(j,j) - sum_{k<j} |L(j,k)|^2,                            */
+/*   L(i,j)   = (A(i,j) - sum_{k<j} L(i,k) conj(L(j,k))) / L(j,j),  i > j;   */
+/* uplo 'U': A = U^H U with U = L^H.  The imaginary part of a diagonal entry */
+/* is never read; the factor's diagonal is real.                             */
+/* ------------------------------------------------------------------------ */
+#define DEFINE_HPOTRF(NAME, R, C, SQRT)                                                 \
+void NAME(char uplo, int n, void* Av, int lda, int* info)                               \
+{                                                                                       \
+    C* A = (C*)Av;                                                                      \
+    *info = 0;                                                                          \
+    const int lower = (uplo == 'L' || uplo == 'l');                                     \
+    for (int j = 0; j < n; ++j) {                                                       \
+        R ajj = creal(A[j + (size_t)j * lda]);                                          \
+        for (int k = 0; k < j; ++k) {                                                   \
+            const C v = lower ? A[j + (size_t)k * lda] : A[k + (size_t)j * lda];        \
+            ajj -= creal(v) * creal(v) + cimag(v) * cimag(v);                           \
+        }                                                                               \
+        if (!(ajj > 0)) { A[j + (size_t)j * lda] = ajj; *info = j + 1; return; }        \
+        ajj = SQRT(ajj);                                                                \
+        A[j + (size_t)j * lda] = ajj;                                                   \
+        if (lower) {                                                                    \
+            for (int i = j + 1; i < n; ++i) {                                           \
+                C s = A[i + (size_t)j * lda];                                           \
+                for (int k = 0; k < j; ++k)                                             \
+                    s -= A[i + (size_t)k * lda] * conj(A[j + (size_t)k * lda]);         \
+                A[i + (size_t)j * lda] = s / ajj;                                       \
+            }                                                                           \
+        } else {                                                                        \
+            for (int c = j + 1; c < n; ++c) {                                           \
+                C s = A[j + (size_t)c * lda];                                           \
+                for (int k = 0; k < j; ++k)                                             \
+                    s -= conj(A[k + (size_t)j * lda]) * A[k + (size_t)c * lda];         \
+                A[j + (size_t)c * lda] = s / ajj;                                       \
+            }                                                                           \
+        }                                                                               \
+    }                                                                                   \
+}
+DEFINE_HPOTRF(oracle_zpotrf, double, double complex, sqrt)
+DEFINE_HPOTRF(oracle_cpotrf, float, float complex, sqrtf)
+
+/* ------------------------------------------------------------------------ */
+/* Hermitian ?potrs (src/Cholesky/ppotrs.cpp:44-61, two ?trsm).  With        */
+/* L(i,k) = F(i,k) (uplo 'L') or conj(F(k,i)) ('U'): L y = b, then           */
+/* L^H x = y.  Side 'R' (X A = B): A Hermitian gives A conj(x)^T =           */
+/* conj(b)^T for every row, solved as a left system and conjugated back.     */
+/* ------------------------------------------------------------------------ */
+#define DEFINE_HPOTRS(NAME, C)                                                          \
+static C NAME##_l(int lower, const C* F, int ldf, int i, int k)                         \
+{                                                                                       \
+    return lower ? F[i + (size_t)k * ldf] : conj(F[k + (size_t)i * ldf]);               \
+}                                                                                       \
+static void NAME##_vector(int lower, int n, const C* F, int ldf, C* x)                  \
+{                                                                                       \
+    for (int i = 0; i < n; ++i) {                                                       \
+        C s = x[i];                                                                     \
+        for (int k = 0; k < i; ++k) s -= NAME##_l(lower, F, ldf, i, k) * x[k];          \
+        x[i] = s / NAME##_l(lower, F, ldf, i, i);                                       \
+    }                                                                                   \
+    for (int i = n - 1; i >= 0; --i) {                                                  \
+        C s = x[i];                                                                     \
+        for (int k = i + 1; k < n; ++k) s -= conj(NAME##_l(lower, F, ldf, k, i)) * x[k]; \
+        x[i] = s / conj(NAME##_l(lower, F, ldf, i, i));                                 \
+    }                                                                                   \
+}                                                                                       \
+void NAME(char side, char uplo, int n, int nrhs, const void* Fv, int ldf, void* Bv, int ldb) \
+{                                                                                       \
+    const C* F = (const C*)Fv;                                                          \
+    C* B = (C*)Bv;                                                                      \
+    const int lower = (uplo == 'L' || uplo == 'l');                                     \
+    const int left = (side == 'L' || side == 'l');                                      \
+    if (n <= 0 || nrhs <= 0) return;                                                    \
+    if (left) {                                                                         \
+        for (int j = 0; j < nrhs; ++j) NAME##_vector(lower, n, F, ldf, B + (size_t)j * ldb); \
+        return;                                                                         \
+    }                                                                                   \
+    C* row = (C*)malloc(sizeof(C) * (size_t)n);                                         \
+    for (int r = 0; r < nrhs; ++r) {                                                    \
+        for (int i = 0; i < n; ++i) row[i] = conj(B[r + (size_t)i * ldb]);              \
+        NAME##_vector(lower, n, F, ldf, row);                                           \
+        for (int i = 0; i < n; ++i) B[r + (size_t)i * ldb] = conj(row[i]);              \
+    }                                                                                   \
+    free(row);                                                                          \
+}
+DEFINE_HPOTRS(oracle_zpotrs, double complex)
+DEFINE_HPOTRS(oracle_cpotrs, float complex)
+
+/* ------------------------------------------------------------------------ */
+/* diag_logdet for complex types: the term of src/diag_logdet_impl.h:50-94   */
+/* and the join of :42-48.  For d = x + iy: nonfinite part -> flag 1;        */
+/* d = 0 -> flag 2; otherwise log|d| = log(hi) + 0.5 log1p((lo/hi)^2)        */
+/* (hi, lo = the larger and smaller of |x|, |y|, so |d| never overflows),    */
+/* and arg d = atan2(y, x) added to the phase, except for a real d < 0,      */
+/* which flips the parity bit (its pi is added once, by finish_logdet).      */
+/* ------------------------------------------------------------------------ */
+#define DEFINE_HLOGDET(NAME, R)                                                         \
+void NAME(int n, const void* Av, int lda, double* magnitude, double* phase, int* flags) \
+{                                                                                       \
+    const R* A = (const R*)Av;                                                          \
+    double m = 0, p = 0;                                                                \
+    int f = 0, parity = 0;                                                              \
+    const double limit = 1.7976931348623157e308;                                        \
+    for (int i = 0; i < n; ++i) {                                                       \
+        const size_t at = 2 * ((size_t)i * lda + i);                                    \
+        const double x = (double)A[at], y = (double)A[at + 1];                          \
+        if (!(x <= limit && x >= -limit && y <= limit && y >= -limit)) { f |= 1; continue; } \
+        const double ax = fabs(x), ay = fabs(y);                                        \
+        const double hi = ax > ay ? ax : ay, lo = ax > ay ? ay : ax;                    \
+        if (hi == 0) { f |= 2; continue; }                                              \
+        const double ratio = lo / hi;                                                   \
+        m += log(hi) + 0.5 * log1p(ratio * ratio);                                      \
+        if (y == 0) { if (x < 0) parity ^= 1; }                                         \
+        else p += atan2(y, x);                                                          \
+    }                                                                                   \
+    *magnitude = m;                                                                     \
+    *phase = p;                                                                         \
+    *flags = f | (parity ? 4 : 0);                                                      \
+}
+DEFINE_HLOGDET(oracle_zdiag_logdet, double)
+DEFINE_HLOGDET(oracle_cdiag_logdet, float)
+
+/* ------------------------------------------------------------------------ */
+/* Complex GEMM, also with one real operand (the mixed real/complex          */
+/* gemmVbatched overloads, include/ddla/ddla.h:1878-1960): a_complex /        */
+/* b_complex 0 means that operand is a real R array, on which 'C' is 'T'.    */
+/* Accumulated in double complex.  alpha, beta: (real, imaginary).  k == 0   */
+/* or alpha == 0 only scales C; beta == 0 never reads C.                     */
+/* ------------------------------------------------------------------------ */
+#define DEFINE_ZGEMM(NAME, R, C)                                                        \
+static double complex NAME##_load(const void* X, int is_complex, size_t at, int conjugate) \
+{                                                                                       \
+    if (!is_complex) return (double)((const R*)X)[at];                                  \
+    const double complex v = (double complex)((const C*)X)[at];                         \
+    return conjugate ? conj(v) : v;                                                     \
+}                                                                                       \
+void NAME(char ta, char tb, int m, int n, int k, const R* alpha, const void* A, int a_complex, \
+          int lda, const void* B, int b_complex, int ldb, const R* beta, void* Cv, int ldc) \
+{                                                                                       \
+    C* Cm = (C*)Cv;                                                                     \
+    const int na = (ta == 'N' || ta == 'n'), nb = (tb == 'N' || tb == 'n');             \
+    const int ca = (ta == 'C' || ta == 'c'), cb = (tb == 'C' || tb == 'c');             \
+    const C al = alpha[0] + alpha[1] * I, be = beta[0] + beta[1] * I;                   \
+    const int alpha_zero = (alpha[0] == 0 && alpha[1] == 0);                            \
+    const int beta_zero = (beta[0] == 0 && beta[1] == 0);                               \
+    for (int j = 0; j < n; ++j) {                                                       \
+        for (int i = 0; i < m; ++i) {                                                   \
+            double complex s = 0;                                                       \
+            if (!alpha_zero)                                                            \
+                for (int p = 0; p < k; ++p) {                                           \
+                    const size_t ia = na ? i + (size_t)p * lda : p + (size_t)i * lda;   \
+                    const size_t ib = nb ? p + (size_t)j * ldb : j + (size_t)p * ldb;   \
+                    s += NAME##_load(A, a_complex, ia, ca) * NAME##_load(B, b_complex, ib, cb); \
+                }                                                                       \
+            const C prod = al * (C)s;                                                   \
+            C* c = Cm + i + (size_t)j * ldc;                                            \
+            *c = beta_zero ? prod : prod + be * *c;                                     \
+        }                                                                               \
+    }                                                                                   \
+}
+DEFINE_ZGEMM(oracle_zgemm, double, double complex)
+DEFINE_ZGEMM(oracle_cgemm, float, float complex)

@@ -29,6 +29,20 @@
  *                      tests/test_diag_logdet.cpp:76-111
  *   gemm               numpy matmul; zero-k / alpha / beta rules of
  *                      src/vbatched/gemm_template_kernel_vbatched.hpp:46-105
+ *   zpotrf / cpotrf    scipy LAPACK zpotrf/cpotrf golden fixtures (with a
+ *                      nonzero imaginary diagonal, which LAPACK ignores),
+ *                      the closed-form factor triangular_l_value of
+ *                      tests/api_grid_test_common.h:502-507, upper factor =
+ *                      conjugate transpose of the lower one, pivot info
+ *   zpotrs / cpotrs    scipy LAPACK zpotrs/cpotrs fixtures; closed forms
+ *                      hpd_value / x_value (tests/api_grid_test_common.h:
+ *                      490-512) on both sides
+ *   z/c diag_logdet    50-digit decimal log|z| and atan2 sums for the cases
+ *                      of tests/test_diag_logdet.cpp:78-110 (largest value,
+ *                      subnormals, polar(1, 0.8 pi) x 3, 1025 x -1), numpy
+ *                      log|z| / angle on random entries
+ *   z/c gemm           numpy matmul for every N/T/C pair, one real operand
+ *                      ('C' = 'T' there), the alpha / k / beta rules
  * No function is left "parity unpinned".
  */
 #ifndef DDLA_ORACLE_H
@@ -82,6 +96,35 @@ void oracle_dgemm(char ta, char tb, int m, int n, int k, double alpha, const dou
                   const double* B, int ldb, double beta, double* C, int ldc);
 void oracle_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float* A, int lda,
                   const float* B, int ldb, float beta, float* C, int ldc);
+
+/* Complex (Hermitian) routines; matrices are interleaved (real, imaginary)
+ * column-major arrays (std::complex<R> / C99 complex layout).
+ *
+ * Hermitian ?potrf (LAPACK zpotf2/cpotf2, src/Cholesky/potrf.cpp:51-59):
+ * A = L L^H (uplo 'L') or U^H U ('U'); only the uplo triangle is read and
+ * written, the imaginary part of the diagonal is not read; info = j+1 for
+ * the first pivot whose real value is not positive (or is NaN). */
+void oracle_zpotrf(char uplo, int n, void* A, int lda, int* info);
+void oracle_cpotrf(char uplo, int n, void* A, int lda, int* info);
+
+/* Hermitian ppotrs at 1x1: side 'L': B := A^-1 B; side 'R': B := B A^-1. */
+void oracle_zpotrs(char side, char uplo, int n, int nrhs, const void* F, int ldf, void* B, int ldb);
+void oracle_cpotrs(char side, char uplo, int n, int nrhs, const void* F, int ldf, void* B, int ldb);
+
+/* diag_logdet for complex types (src/diag_logdet_impl.h:50-94): magnitude
+ * sum of log|d_i|, phase sum of arg d_i over the entries with a nonzero
+ * imaginary part, flags 1 nonfinite | 2 zero | 4 odd count of negative
+ * real entries. */
+void oracle_zdiag_logdet(int n, const void* A, int lda, double* magnitude, double* phase, int* flags);
+void oracle_cdiag_logdet(int n, const void* A, int lda, double* magnitude, double* phase, int* flags);
+
+/* Complex GEMM, also with one real operand (a_complex / b_complex 0: that
+ * operand is a real array; 'C' on it is 'T'), the mixed gemmVbatched
+ * overloads of include/ddla/ddla.h:1878-1960.  alpha, beta: {re, im}. */
+void oracle_zgemm(char ta, char tb, int m, int n, int k, const double* alpha, const void* A, int a_complex,
+                  int lda, const void* B, int b_complex, int ldb, const double* beta, void* C, int ldc);
+void oracle_cgemm(char ta, char tb, int m, int n, int k, const float* alpha, const void* A, int a_complex,
+                  int lda, const void* B, int b_complex, int ldb, const float* beta, void* C, int ldc);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,16 @@ def _load():
     lib.oracle_sgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_float, fp, ctypes.c_int, fp, ctypes.c_int, ctypes.c_float, fp,
                                  ctypes.c_int]
+    for p in ("z", "c"):
+        getattr(lib, f"oracle_{p}potrf").argtypes = [ctypes.c_char, ctypes.c_int, dp, ctypes.c_int, ip]
+        getattr(lib, f"oracle_{p}potrs").argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, dp,
+                                                     ctypes.c_int, dp, ctypes.c_int]
+        getattr(lib, f"oracle_{p}diag_logdet").argtypes = [ctypes.c_int, dp, ctypes.c_int,
+                                                           ctypes.POINTER(ctypes.c_double),
+                                                           ctypes.POINTER(ctypes.c_double), ip]
+        getattr(lib, f"oracle_{p}gemm").argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_int, dp,
+                                                    ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
     return lib
 
 
@@ -47,7 +57,8 @@ lib = _load()
 
 
 def _pfx(a):
-    return "d" if a.dtype == np.float64 else "s"
+    return {np.dtype(np.float64): "d", np.dtype(np.float32): "s", np.dtype(np.complex128): "z",
+            np.dtype(np.complex64): "c"}[np.dtype(a.dtype)]
 
 
 def philox(ctr, key):
@@ -109,7 +120,15 @@ def potrs(side, uplo, factor, b):
 
 
 def diag_logdet(a, n=None):
-    """(magnitude, flags) with flags 1 nonfinite | 2 zero | 4 odd negative count."""
+    """(magnitude, flags) with flags 1 nonfinite | 2 zero | 4 odd negative count; a complex a gives
+    (magnitude, phase, flags), the phase summed over the entries that are not real."""
+    if np.iscomplexobj(a):
+        a = np.asfortranarray(a)
+        n = a.shape[1] if n is None else n
+        mag, phase, flags = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+        getattr(lib, f"oracle_{_pfx(a)}diag_logdet")(n, a.ctypes.data, max(1, a.shape[0]), ctypes.byref(mag),
+                                                     ctypes.byref(phase), ctypes.byref(flags))
+        return mag.value, phase.value, flags.value
     a = np.asfortranarray(a)
     n = a.shape[1] if n is None else n
     mag = ctypes.c_double(0)
@@ -119,22 +138,34 @@ def diag_logdet(a, n=None):
     return mag.value, flags.value
 
 
-def finish_logdet(mag, flags):
+def finish_logdet(mag, flags, phase=0.0):
     """The reference's result rules (src/diag_logdet_impl.h:104-113, src/diag_logdet.cpp:52-54)."""
     if flags & 1:
         raise ValueError("nonfinite diagonal")
     if flags & 2:
         return complex(-np.inf, 0.0)
-    phase = math.remainder(((flags >> 2) & 1) * math.pi, 2 * math.pi)
+    phase = math.remainder(phase + ((flags >> 2) & 1) * math.pi, 2 * math.pi)
     if phase <= -math.pi:
         phase = math.pi
     return complex(mag, phase)
 
 
 def gemm(ta, tb, m, n, k, alpha, a, b, beta, c):
+    """C := alpha op(A) op(B) + beta C on a copy of c.  A complex c may take one real operand."""
     c = np.array(c, order="F", copy=True)
     a = np.asfortranarray(a)
     b = np.asfortranarray(b)
+    if np.iscomplexobj(c):
+        rt = np.float64 if c.dtype == np.complex128 else np.float32
+        al = np.array([complex(alpha).real, complex(alpha).imag], dtype=rt)
+        be = np.array([complex(beta).real, complex(beta).imag], dtype=rt)
+        a = a.astype(c.dtype if np.iscomplexobj(a) else rt, order="F", copy=False)
+        b = b.astype(c.dtype if np.iscomplexobj(b) else rt, order="F", copy=False)
+        getattr(lib, f"oracle_{_pfx(c)}gemm")(ta.encode(), tb.encode(), m, n, k, al.ctypes.data, a.ctypes.data,
+                                              int(np.iscomplexobj(a)), max(1, a.shape[0]), b.ctypes.data,
+                                              int(np.iscomplexobj(b)), max(1, b.shape[0]), be.ctypes.data,
+                                              c.ctypes.data, max(1, c.shape[0]))
+        return c
     fn = lib.oracle_dgemm if c.dtype == np.float64 else lib.oracle_sgemm
     fn(ta.encode(), tb.encode(), m, n, k, alpha, a.ctypes.data, max(1, a.shape[0]), b.ctypes.data,
        max(1, b.shape[0]), beta, c.ctypes.data, max(1, c.shape[0]))

@@ -1,4 +1,5 @@
-"""Writes tests/golden/lapack_cases.npz: LAPACK ?potrf / ?potrs results on small inputs.
+"""Writes tests/golden/lapack_cases.npz: LAPACK ?potrf / ?potrs results on small inputs
+(real d/s and Hermitian z/c).
 
 The expected outputs come from scipy's bundled LAPACK (OpenBLAS 0.3.29), the
 library the reference's CPU handle calls at a 1 x 1 grid
@@ -33,6 +34,33 @@ def main():
                 assert info == 0
                 factor = tri(c)
                 b = rng.standard_normal((n, 3)).astype(dtype)
+                x, info2 = potrs(factor, b, lower=(uplo == "L"))
+                assert info2 == 0
+                key = f"{tag}_{uplo}_{n}_{pad}"
+                cases[f"potrf_in_{key}"] = full
+                cases[f"potrf_out_{key}"] = factor.astype(dtype)
+                cases[f"potrs_b_{key}"] = b
+                cases[f"potrs_x_{key}"] = x.astype(dtype)
+    # Hermitian cases (zpotrf / cpotrf / zpotrs / cpotrs, src/Cholesky/potrf.cpp:51-59), from a
+    # generator of their own so that the real cases above keep their values.
+    crng = np.random.RandomState(20261203)
+    for dtype, tag in ((np.complex128, "z"), (np.complex64, "c")):
+        potrf = lapack.zpotrf if tag == "z" else lapack.cpotrf
+        potrs = lapack.zpotrs if tag == "z" else lapack.cpotrs
+        for n, pad in ((1, 0), (2, 3), (5, 0), (17, 3), (33, 0), (64, 2), (97, 0)):
+            for uplo in ("L", "U"):
+                m = crng.standard_normal((n, n)) + 1j * crng.standard_normal((n, n))
+                a = (m @ m.conj().T / n + np.eye(n)).astype(dtype)
+                # A nonzero imaginary part on the diagonal, which zpotrf ignores.
+                a[np.diag_indices(n)] += 1j * crng.standard_normal(n).astype(dtype)
+                lda = n + pad
+                full = np.full((lda, n), np.nan + 1j * np.nan, dtype=dtype, order="F")
+                tri = np.tril if uplo == "L" else np.triu
+                full[:n, :n] = np.where(tri(np.ones((n, n), bool)), a, np.nan + 1j * np.nan)
+                c, info = potrf(a, lower=(uplo == "L"), clean=0)
+                assert info == 0
+                factor = tri(c)
+                b = (crng.standard_normal((n, 3)) + 1j * crng.standard_normal((n, 3))).astype(dtype)
                 x, info2 = potrs(factor, b, lower=(uplo == "L"))
                 assert info2 == 0
                 key = f"{tag}_{uplo}_{n}_{pad}"

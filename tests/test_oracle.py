@@ -70,7 +70,7 @@ def test_potrf_matches_lapack_fixture(key):
     assert info == 0
     want = GOLD[f"potrf_out_{key}"]
     got = _tri(uplo, f[:n, :n])
-    tol = 1e-13 if tag == "d" else 2e-6
+    tol = 1e-13 if tag in "dz" else 2e-6
     assert np.abs(got - want).max() <= tol * max(1.0, np.abs(want).max())
     # the other triangle and the padding are neither read nor written: they stay NaN
     other = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
@@ -85,7 +85,7 @@ def test_potrs_matches_lapack_fixture(key):
     b = GOLD[f"potrs_b_{key}"]
     x = orc.potrs("L", uplo, factor, b)
     want = GOLD[f"potrs_x_{key}"]
-    tol = 1e-12 if tag == "d" else 1e-4
+    tol = 1e-12 if tag in "dz" else 1e-4
     assert np.abs(x - want).max() <= tol * max(1.0, np.abs(want).max())
 
 
@@ -359,3 +359,224 @@ def test_potrf_bottom_right_reports_the_highest_failing_row(uplo):
     assert finfo == 4
     g, info = orc.potrf_bottom_right(uplo, np.array([[9.0]]))
     assert info == 0 and g[0, 0] == 3.0
+
+
+# --------------------------------------------------------------------------
+# Complex (Hermitian) oracle functions.
+# --------------------------------------------------------------------------
+def hpd_value(n):
+    """hpd_value of tests/api_grid_test_common.h:490-500 (Hermitian, diagonally dominant)."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    lo, hi = np.minimum(i, j), np.maximum(i, j)
+    val = 0.01 * ((lo + 2 * hi) % 5 - 2) + 1j * 0.006 * ((3 * lo + hi) % 7 - 3)
+    a = np.where(i < j, val, np.conj(val))
+    return np.where(i == j, 5.0 + 0.2 * n + 0.05 * i + 0j, a)
+
+
+def triangular_l_value(n):
+    """triangular_l_value of tests/api_grid_test_common.h:502-507 (lower, real positive diagonal)."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    off = 0.04 * ((i + j) % 4 + 1) + 1j * 0.01 * ((i - j) % 3)
+    return np.where(i < j, 0j, np.where(i == j, 2.0 + 0.05 * i + 0j, off))
+
+
+def x_value(rows, cols):
+    """x_value of tests/api_grid_test_common.h:509-512."""
+    i, j = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+    return (0.2 + 0.03 * i - 0.02 * j) + 1j * (0.04 * (i + 1) + 0.01 * j)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [1, 2, 40, 129])
+def test_hpotrf_recovers_known_factor(n, dtype):
+    """A = L L^H from the reference's closed-form L: the unique Cholesky factor with a
+    positive real diagonal is L itself (uplo 'L') and L^H (uplo 'U')."""
+    l = triangular_l_value(n)
+    a = (l @ l.conj().T).astype(dtype)
+    tol = 1e-13 if dtype == np.complex128 else 2e-6
+    scale = np.abs(l).max()
+    for uplo, want in (("L", l), ("U", l.conj().T)):
+        f, info = orc.potrf(uplo, a)
+        assert info == 0
+        assert np.abs(_tri(uplo, f) - want).max() <= tol * 10 * scale
+    # planted mistake: the factor of the transposed (conjugated) matrix is conj(L), not L
+    f, _ = orc.potrf("L", a.conj().copy())
+    if n > 1:
+        assert np.abs(np.tril(f) - l).max() > 1e-3
+
+
+def test_hpotrf_upper_is_conjugate_transpose_of_lower():
+    rng = np.random.RandomState(5)
+    m = rng.standard_normal((37, 37)) + 1j * rng.standard_normal((37, 37))
+    a = m @ m.conj().T / 37 + np.eye(37)
+    a = (a + a.conj().T) / 2  # exactly Hermitian
+    fl, _ = orc.potrf("L", a)
+    fu, _ = orc.potrf("U", a)
+    # conj commutes exactly with complex +, - and *: the two triangles agree bit for bit
+    assert np.array_equal(np.triu(fu), np.tril(fl).conj().T)
+
+
+def test_hpotrf_ignores_the_imaginary_part_of_the_diagonal():
+    l = triangular_l_value(9)
+    a = l @ l.conj().T
+    noisy = a + 1j * np.diag(np.linspace(-3, 3, 9))
+    f0, _ = orc.potrf("L", a)
+    f1, _ = orc.potrf("L", noisy)
+    assert np.array_equal(np.tril(f0), np.tril(f1))
+    assert (np.diag(f1).imag == 0).all()
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_hpotrf_pivot_info(uplo, dtype):
+    l = triangular_l_value(12)
+    a = (l @ l.conj().T).astype(dtype)
+    bad = a.copy()
+    bad[11, 11] = -1.0 + 5j  # the real part decides: -1 - sum |l|^2 < 0
+    assert orc.potrf(uplo, bad)[1] == 12
+    bad = a.copy()
+    bad[4, 4] = np.nan
+    assert orc.potrf(uplo, bad)[1] == 5
+    bad = a.copy()
+    bad[0, 0] = 0.0
+    f, info = orc.potrf(uplo, bad)
+    assert info == 1 and f[0, 0] == 0
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_hpotrs_closed_form_solution(side, uplo, dtype):
+    """B = A X (side L) or X A (side R) with the reference's hpd_value / x_value; the solve
+    recovers X (tests/api_grid_test_common.h:524-563 build_rhs_side / check_solution)."""
+    n, nrhs = 40, 5
+    a = hpd_value(n)
+    x_true = x_value(n, nrhs) if side == "L" else x_value(nrhs, n)
+    b = a @ x_true if side == "L" else x_true @ a
+    f, info = orc.potrf(uplo, a.astype(dtype))
+    assert info == 0
+    x = orc.potrs(side, uplo, f, b.astype(dtype))
+    tol = 1e-12 if dtype == np.complex128 else 2e-5
+    assert np.abs(x - x_true).max() < tol
+    # planted mistake: the transposed factor (conjugation dropped) misses the closed form
+    wrong = orc.potrs(side, uplo, f.T.copy(), b.astype(dtype))
+    assert np.abs(wrong - x_true).max() > 1e-4
+
+
+def complex_logdet_reference(d):
+    """tests/test_diag_logdet.cpp:21-30: sum of log hypot and of atan2, phase reduced by remainder.
+    The magnitude is summed in 50-digit decimal from the exact |z|^2, which neither overflows
+    at the largest float nor loses digits at subnormals (where a binary hypot would)."""
+    import decimal
+
+    ctx = decimal.Context(prec=50)
+    mag, phases = decimal.Decimal(0), []
+    for z in d:
+        x, y = decimal.Decimal(z.real), decimal.Decimal(z.imag)
+        r2 = ctx.add(ctx.multiply(x, x), ctx.multiply(y, y))
+        if r2 == 0:
+            return complex(-np.inf, 0)
+        mag = ctx.add(mag, ctx.divide(ctx.ln(r2), 2))
+        phases.append(math.atan2(z.imag, z.real))
+    # fsum: the correctly rounded sum of the terms (the C++ test sums them in long double)
+    return complex(float(mag), math.remainder(math.fsum(phases), 2 * math.pi))
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("case", ["2,3,5", "-2,3,5", "-2,-3,5", "0,-3,5", "max", "polar", "ones11", "neg1025",
+                                  "empty", "random"])
+def test_complex_logdet_cases(case, dtype):
+    """The cases of tests/test_diag_logdet.cpp:78-110 for complex types, on a NaN-poisoned
+    array with lda = n + 3."""
+    rt = np.float64 if dtype == np.complex128 else np.float32
+    fmax, tiny = np.finfo(rt).max, np.finfo(rt).smallest_subnormal
+    d = {"2,3,5": [2, 3, 5], "-2,3,5": [-2, 3, 5], "-2,-3,5": [-2, -3, 5], "0,-3,5": [0, -3, 5],
+         "max": [complex(fmax, fmax), complex(tiny, -tiny)],
+         "polar": [complex(math.cos(0.8 * math.pi), math.sin(0.8 * math.pi))] * 3,
+         "ones11": [1 + i % 3 for i in range(11)], "neg1025": [-1.0] * 1025, "empty": [],
+         "random": list(np.random.RandomState(3).standard_normal(50) + 1j * np.random.RandomState(4).standard_normal(50))}[case]
+    d = [complex(dtype(z)) for z in d]
+    n = len(d)
+    a = np.full((n + 3, max(n, 1)), np.nan + 1j * np.nan, dtype=dtype)
+    for i, z in enumerate(d):
+        a[i, i] = z
+    mag, phase, flags = orc.diag_logdet(a, n)
+    got = orc.finish_logdet(mag, flags, phase)
+    want = complex_logdet_reference(d)
+    if math.isinf(want.real):
+        assert math.isinf(got.real) and got.real < 0 and got.imag == 0
+    else:
+        assert abs(got.real - want.real) < 3e-12 * (1 + abs(want.real))
+        assert phase_close(got.imag, want.imag, 3e-12)
+    if case == "neg1025":
+        assert got == complex(0, math.pi)
+    if case == "random":
+        assert abs(got.real - np.log(np.abs(np.array(d))).sum()) < 1e-12
+        assert phase_close(got.imag, float(np.angle(np.array(d)).sum()), 1e-12)
+
+
+@pytest.mark.parametrize("bad", [complex(1, np.inf), complex(np.nan, 0), complex(-np.inf, 2)])
+def test_complex_logdet_nonfinite_flag(bad):
+    a = np.diag([1.0 + 0j, bad])
+    mag, phase, flags = orc.diag_logdet(a)
+    assert flags & 1
+    with pytest.raises(ValueError):
+        orc.finish_logdet(mag, flags, phase)
+
+
+OPS = ["N", "T", "C"]
+
+
+def _op(x, t):
+    return x if t == "N" else (x.T if t == "T" else x.conj().T)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("ta", OPS)
+@pytest.mark.parametrize("tb", OPS)
+def test_complex_gemm_against_numpy(ta, tb, dtype):
+    rng = np.random.RandomState(8)
+    m, n, k = 7, 5, 9
+    cplx = lambda *s: (rng.standard_normal(s) + 1j * rng.standard_normal(s)).astype(dtype)
+    a = cplx(*((m, k) if ta == "N" else (k, m)))
+    b = cplx(*((k, n) if tb == "N" else (n, k)))
+    c = cplx(m, n)
+    alpha, beta = 0.7 - 1.3j, -0.4 + 0.2j
+    got = orc.gemm(ta, tb, m, n, k, alpha, a, b, beta, c)
+    want = alpha * (_op(a.astype(np.complex128), ta) @ _op(b.astype(np.complex128), tb)) + beta * c
+    tol = 1e-13 if dtype == np.complex128 else 2e-6
+    assert np.abs(got - want).max() < tol * np.abs(want).max() * 10
+
+
+@pytest.mark.parametrize("real_side", ["A", "B"])
+@pytest.mark.parametrize("t", OPS)
+def test_mixed_real_complex_gemm(real_side, t):
+    """One real operand (include/ddla/ddla.h:1878-1960): 'C' on the real operand equals 'T'."""
+    rng = np.random.RandomState(9)
+    m, n, k = 6, 4, 11
+    c = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    if real_side == "A":
+        a = rng.standard_normal((m, k) if t == "N" else (k, m))
+        b = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+        got = orc.gemm(t, "N", m, n, k, 1.5j, a, b, 2.0, c)
+        want = 1.5j * (_op(a, t) @ b) + 2.0 * c
+    else:
+        a = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
+        b = rng.standard_normal((k, n) if t == "N" else (n, k))
+        got = orc.gemm("N", t, m, n, k, 1.5j, a, b, 2.0, c)
+        want = 1.5j * (a @ _op(b, t)) + 2.0 * c
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max() * 10
+
+
+def test_complex_gemm_special_cases():
+    """src/vbatched/gemm_template_kernel_vbatched.hpp:46-105: alpha == 0 or k == 0 never reads A, B;
+    beta == 0 never reads C."""
+    nanc = np.full((4, 4), np.nan + 1j * np.nan)
+    c = np.arange(16).reshape(4, 4) * (1 - 1j)
+    got = orc.gemm("N", "N", 4, 4, 4, 0, nanc, nanc, 2 - 1j, c)
+    assert np.array_equal(got, (2 - 1j) * c)
+    got = orc.gemm("N", "N", 4, 4, 0, 1, nanc, nanc, 0.5, c)
+    assert np.array_equal(got, 0.5 * c)
+    eye = np.eye(4) + 0j
+    got = orc.gemm("N", "C", 4, 4, 4, 1j, eye, eye, 0, nanc)
+    assert np.array_equal(got, 1j * eye)
