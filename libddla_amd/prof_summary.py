@@ -16,6 +16,7 @@ each way), so traffic is reported in bytes.
 
     python -m libddla_amd.prof_summary --trace DIR --after NAME [--per K] ...
     python -m libddla_amd.prof_summary --trace DIR --span START END ...
+    python -m libddla_amd.prof_summary --trace DIR --pmc DIR [DIR ...] --focus NAME ...
 
 summarize one window of the kernel trace instead: --after keeps the
 dispatches after the last one whose name contains NAME (through the last one
@@ -23,7 +24,10 @@ containing --until, if given); --span keeps those
 from the last dispatch containing START that precedes the last one
 containing END, through that END dispatch.  The window's wall time (first
 start to last end), busy time and per-kernel totals are listed, divided by
---per (e.g. the number of problems in the window).
+--per (e.g. the number of problems in the window).  --pmc lists, for each
+--focus kernel, the mean of every counter of the given `rocprofv3 --pmc`
+passes per dispatch, with the shader clock, the matrix cores' busy share and
+instruction ratios derived from them.
 """
 import argparse
 import csv
@@ -107,6 +111,57 @@ def counters(pmc_dir, counter):
             if r["Counter_Name"] == counter]
 
 
+def counter_table(pmc_dirs, trace_dir, focus, cus=256, xcds=8):
+    """Per focus kernel: the mean of every collected counter per dispatch, and derived rates.
+
+    GRBM_GUI_ACTIVE is summed over the XCDs, so GRBM_GUI_ACTIVE / xcds / (mean traced duration)
+    is the shader clock during the kernel; SQ_VALU_MFMA_BUSY_CYCLES counts busy cycles of every
+    SIMD (4 per CU), so its share of 4 * cus * GRBM_GUI_ACTIVE / xcds is the fraction of the
+    matrix cores kept busy."""
+    sums = {}
+    for d in pmc_dirs:
+        for r in _read(os.path.join(d, "*counter_collection.csv")):
+            for f in focus:
+                if f in r["Kernel_Name"]:
+                    key = (f, short_name(r["Kernel_Name"]))
+                    acc = sums.setdefault(key, {})
+                    v = acc.setdefault(r["Counter_Name"], [0.0, 0])
+                    v[0] += float(r["Counter_Value"])
+                    v[1] += 1
+    durations = {}
+    if trace_dir:
+        for st in kernel_stats(trace_dir):
+            durations[short_name(st["name"])] = st["avg_us"]
+    lines = []
+    for (f, name), acc in sorted(sums.items()):
+        mean = {c: v[0] / v[1] for c, v in acc.items()}
+        lines += ["", f"### `{name}`", "", "| counter | mean per dispatch | dispatches |", "|---|---:|---:|"]
+        for c in sorted(mean):
+            lines.append(f"| {c} | {mean[c]:.4g} | {acc[c][1]} |")
+        grbm = mean.get("GRBM_GUI_ACTIVE")
+        derived = []
+        if grbm:
+            per_xcd = grbm / xcds
+            if name in durations and durations[name] > 0:
+                derived.append(f"shader clock {per_xcd / (durations[name] * 1e-6) / 1e9:.2f} GHz "
+                               f"(GRBM_GUI_ACTIVE / {xcds} over the traced mean {durations[name]:.1f} us)")
+            if "SQ_VALU_MFMA_BUSY_CYCLES" in mean:
+                derived.append(f"matrix cores busy {mean['SQ_VALU_MFMA_BUSY_CYCLES'] / (4 * cus * per_xcd):.3f} "
+                               f"of {4 * cus} SIMDs")
+        if mean.get("SQ_INSTS_MFMA"):
+            if "SQ_INSTS_LDS" in mean:
+                derived.append(f"LDS instructions per MFMA {mean['SQ_INSTS_LDS'] / mean['SQ_INSTS_MFMA']:.3f}")
+            if "SQ_INSTS_VALU" in mean:
+                derived.append(f"VALU instructions per MFMA {mean['SQ_INSTS_VALU'] / mean['SQ_INSTS_MFMA']:.3f}")
+        if mean.get("SQ_ACTIVE_INST_ANY") and "SQ_WAIT_INST_ANY" in mean:
+            derived.append(f"SQ_WAIT_INST_ANY / SQ_ACTIVE_INST_ANY {mean['SQ_WAIT_INST_ANY'] / mean['SQ_ACTIVE_INST_ANY']:.2f}")
+        if "SQ_LDS_BANK_CONFLICT" in mean:
+            derived.append(f"LDS bank-conflict cycles {mean['SQ_LDS_BANK_CONFLICT']:.0f}")
+        if derived:
+            lines += ["", "Derived: " + "; ".join(derived) + "."]
+    return lines
+
+
 def traffic(fetch_dir, write_dir, n):
     """(fetch scale, write scale, per-launch fetch bytes, per-launch write bytes) of the trailing update."""
     fetch, write = counters(fetch_dir, "FETCH_SIZE"), counters(write_dir, "WRITE_SIZE")
@@ -180,11 +235,22 @@ def main():
                     help="summarize the last START .. END window of dispatches")
     ap.add_argument("--per", type=int, default=1, help="units (problems, calls) in the window")
     ap.add_argument("--title", default="rocprofv3 kernel summary")
+    ap.add_argument("--pmc", nargs="+", help="counter-collection output directories (rocprofv3 --pmc passes); "
+                                             "summarizes the --focus kernels' counters")
     args = ap.parse_args()
 
     lines = [f"# {args.title}", ""]
     if args.command:
         lines += ["Command:", "", "```", args.command, "```", ""]
+    if args.pmc:
+        lines += counter_table(args.pmc, args.trace, args.focus)
+        text = "\n".join(lines) + "\n"
+        if args.md:
+            with open(args.md, "w") as f:
+                f.write(text)
+        else:
+            print(text)
+        return
     if args.after or args.span:
         lines += window_lines(trace_window(args.trace, args.after, args.span, args.until), args.per)
         text = "\n".join(lines) + "\n"
