@@ -19,7 +19,8 @@ mixed: the same n and nrhs with seed 2 and delta in {4e-2, 4e-4, 4e-6}
 (condition numbers about 1e2, 1e4, 1e6), solved by pposv_mixed (FP32 factor
 on the matrix cores, FP64 residual and refinement on the GPU, tol 1e-14) with
 each trailing-update mode (FP32 MFMA, bf16x3 and bf16x6 split bfloat16
-products with FP32 accumulation) and, on the same matrix, by the FP64 pposv;
+products, fp16x3 split half-precision products of row-scaled panels, all
+with FP32 accumulation) and, on the same matrix, by the FP64 pposv;
 per delta and mode: refinement iterations,
 time of each, their ratio, the backward error max_j ||b_j - A x_j||_inf /
 (||A||_inf ||x_j||_inf) (FP64, on the GPU) and the forward error
@@ -68,7 +69,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TFLOP/s, % peak: FP64 vs mixed-prec pposv n=32768 at 1e-14 bwd err, 1 GPU"
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
-FACTOR_NAMES = {0: "fp32", 1: "bf16x3", 2: "bf16x6"}  # pposv_mixed factor argument
+FACTOR_NAMES = {0: "fp32", 1: "bf16x3", 2: "bf16x6", 3: "fp16x3"}  # pposv_mixed factor argument
 
 
 def log(msg):
@@ -481,8 +482,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--deltas", type=float, nargs="+", default=[4e-2, 4e-4, 4e-6])
     ap.add_argument("--tol", type=float, default=1e-14)
-    ap.add_argument("--factors", type=int, nargs="+", default=[0, 1, 2],
-                    help="pposv_mixed factor modes: 0 FP32, 1 bf16x3, 2 bf16x6")
+    ap.add_argument("--factors", type=int, nargs="+", default=[0, 1, 2, 3],
+                    help="pposv_mixed factor modes: 0 FP32, 1 bf16x3, 2 bf16x6, 3 fp16x3")
     ap.add_argument("--batched", type=int, default=4096, help="members of the batched leg (0 skips it)")
     ap.add_argument("--gp-problems", type=int, default=16, help="problems of the GP-likelihood leg (0 skips it)")
     ap.add_argument("--gp-n", type=int, default=4096)
@@ -602,7 +603,9 @@ def main():
             "forward_error_reference": "FP64 pposv solution of the same system on the GPU",
             "factors": {"fp32": "FP32 MFMA trailing updates",
                         "bf16x3": "bfloat16 split products x0y0 + x0y1 + x1y0, FP32 accumulation",
-                        "bf16x6": "bfloat16 split products with i + j <= 2 (three planes), FP32 accumulation"},
+                        "bf16x6": "bfloat16 split products with i + j <= 2 (three planes), FP32 accumulation",
+                        "fp16x3": "half-precision split products x0y0 + x0y1 + x1y0 of panel rows scaled by "
+                                  "powers of two, FP32 accumulation"},
             "per_delta": rows,
             "worst_time_ratio_vs_fp64": {FACTOR_NAMES[f]: max(r["time_ratio_vs_fp64"] for r in rows
                                                               if r["factor"] == FACTOR_NAMES[f])

@@ -211,7 +211,9 @@ int ddla_z_pposv_run(void* handle, char side, char uplo, char trans, int n, int 
 /* ---- mixed-precision pposv (no counterpart in the reference; LAPACK dsposv
  * semantics): FP64 A X = B, side 'L', factored in FP32 on the matrix cores
  * (factor_precision 0: FP32 trailing updates; 1: bf16x3 and 2: bf16x6 split
- * bfloat16 products with FP32 accumulation) and refined in FP64 on the GPU
+ * bfloat16 products with FP32 accumulation; 3: fp16x3, three half-precision
+ * split products of panels whose rows are scaled by powers of two, FP32
+ * accumulation) and refined in FP64 on the GPU
  * until every column has
  * ||r||_inf <= tol ||A||_inf ||x||_inf (at most max_iter corrections), else
  * the FP64 pposv runs.  *iter = corrections applied, -3 (FP32 factor failed)

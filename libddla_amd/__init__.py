@@ -31,6 +31,7 @@ from .matrix import DistributedMatrix, descinit
 from .ops import (
     FACTOR_BF16X3,
     FACTOR_BF16X6,
+    FACTOR_FP16X3,
     FACTOR_FP32,
     diag_logdet,
     pdiag_logdet,
@@ -56,7 +57,7 @@ __all__ = [
     "LIBRARY_PATH", "lib", "check_status", "Backend", "Handle", "DistributedMatrix", "descinit",
     "ppotrf", "ppotrs", "pposv", "ptrtrs", "trsm", "potrf_bottom_right", "ppotrf_bottom_right", "ppotrf_workspace_size", "ppotrf_run", "ppotrs_workspace_size", "ppotrs_run",
     "pposv_workspace_size", "pposv_run", "pposv_mixed", "pposv_mixed_workspace_size", "pposv_mixed_run",
-    "FACTOR_FP32", "FACTOR_BF16X3", "FACTOR_BF16X6",
+    "FACTOR_FP32", "FACTOR_BF16X3", "FACTOR_BF16X6", "FACTOR_FP16X3",
     "diag_logdet", "pdiag_logdet", "VBatch", "BatchedGraph", "pposv_vbatched", "pposv_vbatched_run", "pposv_vbatched_tune",
     "pposv_vbatched_workspace_size", "vbatched_schedule", "set_vbatched_schedule", "DdlaError", "DdlaInvalidHandleError",
     "DdlaInvalidValueError", "DdlaAllocFailedError", "DdlaBackendUnavailableError", "DdlaInternalError",

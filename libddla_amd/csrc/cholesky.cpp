@@ -87,14 +87,15 @@ void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k
     kern::panel_factor<T>(st, false, n, v.A, v.lda, k, kb, info, work);
 }
 
-// BF16 split-product trailing updates (FP32 factorization only): after panel
-// p is factored, its rows below the diagonal block are split into bfloat16
-// planes in buf[p % 3] (split_buffer_bytes(n, products) bytes each), and every
+// Split-product trailing updates (FP32 factorization only): after panel p is
+// factored, its rows below the diagonal block are split into bfloat16 or
+// half-precision planes in buf[p % 3] (split_buffer_bytes(n, mode) bytes
+// each), and every
 // trailing update of step p reads that buffer.  Three buffers: the split of
 // panel p + 3 waits (through the look-ahead chain) for step p's update of the
 // rest, the last reader of buf[p % 3].
 struct SplitPlan {
-    int products;  // 3 (bf16x3) or 6 (bf16x6)
+    int mode;  // 1 bf16x3, 2 bf16x6, 3 fp16x3
     void* buf[3];
 };
 
@@ -108,7 +109,7 @@ void split_update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& 
     if (M <= 0 || N <= 0) return;
     LaunchTimer timer(h, cls, flops, 0, st);
     if constexpr (std::is_same<T, float>::value) {
-        kern::split_update(st, sp.products, false, M, N, kb, buf, m_split, a_row, b_row, v.at(r + a_row, r + b_row),
+        kern::split_update(st, sp.mode, false, M, N, kb, buf, m_split, a_row, b_row, v.at(r + a_row, r + b_row),
                            v.lda, true, (long long)a_row - b_row, abort);
     } else {
         throw std::logic_error("split-product updates apply to the FP32 factorization");
@@ -129,7 +130,7 @@ void split_update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& 
 // panels are updated and factored beside them; concurrent launches write
 // disjoint columns and read only finished panels.  Returns with all work
 // ordered on the handle's stream.  split (FP32 only, optional): the trailing
-// updates run as BF16 split products.
+// updates run as split products.
 template <class T>
 void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work,
                    const SplitPlan* split = nullptr)
@@ -164,7 +165,7 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
         const long long r = (long long)k + kb;
         if (!split || r >= n) return;
         if constexpr (std::is_same<T, float>::value)
-            kern::split_panel(st, split->products, false, n - (int)r, kb, v.at(r, k), lda, split->buf[q % 3]);
+            kern::split_panel(st, split->mode, false, n - (int)r, kb, v.at(r, k), lda, split->buf[q % 3]);
     };
     // C_L(i, j) -= sum over panel p's columns of L(i, .) L(j, .) for columns
     // c_lo <= j < c_hi and rows i >= j (lower trapezoid), on stream st
@@ -821,9 +822,9 @@ template void potrf_blocked_dispatch<float>(DdlaStream*, bool, int, float*, long
 template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, long long, int*, void*);
 
 void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long lda, int* info, void* work,
-                          int products, void* const* bufs)
+                          int mode, void* const* bufs)
 {
-    const SplitPlan plan{products, {bufs[0], bufs[1], bufs[2]}};
+    const SplitPlan plan{mode, {bufs[0], bufs[1], bufs[2]}};
     potrf_blocked<float>(h, upper, n, A, lda, info, work, &plan);
 }
 

@@ -73,21 +73,24 @@ int* panel_error_word(int n, void* work);
 template <class T>
 void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, int kb, int* info, void* work);
 
-// BF16 split-product trailing update of the FP32 factorization (L view as for
-// panel_factor); products = 3 (bf16x3, two bfloat16 planes) or 6 (bf16x6,
-// three planes).  split_panel writes rows [0, m) and columns [0, kb) (kb <=
-// 512) of the L view starting at X -- the stored address of L-view element
-// (r, k) -- into a fragment-ordered buffer zero-padded to split_rows(m) rows;
-// split_buffer_bytes(n, products) bytes hold it for any m <= n.  split_update
+// Split-product trailing update of the FP32 factorization (L view as for
+// panel_factor); mode = 1 (bf16x3: two bfloat16 planes, three products), 2
+// (bf16x6: three planes, six products) or 3 (fp16x3: two half-precision
+// planes of row-scaled values, three products).  split_panel writes rows
+// [0, m) and columns [0, kb) (kb <= 512) of the L view starting at X -- the
+// stored address of L-view element (r, k) -- into a fragment-ordered buffer
+// zero-padded to split_rows(m) rows (with the row exponents after it for
+// fp16x3); split_buffer_bytes(n, mode) bytes hold it for any m <= n.  split_update
 // then applies, with the buffer of a split of m_split rows,
 //   C_L(i, j) -= sum_k P(a_row + i, k) P(b_row + j, k),  i < m, j < n2,
 // accumulated in FP32, writing only i + diag >= j when tri; C is the stored
-// address of L-view element C_L(0, 0).  a_row and b_row must be multiples of
+// address of L-view element C_L(0, 0), and upper must be false (the split
+// products update the lower view only).  a_row and b_row must be multiples of
 // 16 with every row the launch reads inside split_rows(m_split) (checked).
 int split_rows(int m);
-size_t split_buffer_bytes(int n, int products);
-void split_panel(hipStream_t s, int products, bool upper, int m, int kb, const float* X, long long ldx, void* out);
-void split_update(hipStream_t s, int products, bool upper, int m, int n2, int kb, const void* P, int m_split,
+size_t split_buffer_bytes(int n, int mode);
+void split_panel(hipStream_t s, int mode, bool upper, int m, int kb, const float* X, long long ldx, void* out);
+void split_update(hipStream_t s, int mode, bool upper, int m, int n2, int kb, const void* P, int m_split,
                   int a_row, int b_row, float* C, long long ldc, bool tri, long long diag, const int* abort);
 
 // Inverts every 64 x 64 diagonal block of the order-n factor (L view as for

@@ -533,68 +533,89 @@ int set_wait_limit(int spins)
     return g_wait_limit.exchange(spins > 0 ? spins : kDefaultWaitLimit, std::memory_order_relaxed);
 }
 
-int split_rows(int m) { return (m + split_detail::kTile - 1) / split_detail::kTile * split_detail::kTile + split_detail::kTile; }
-
-size_t split_buffer_bytes(int n, int products)
-{
-    const int planes = products == 6 ? 3 : 2;
-    return (size_t)split_rows(n) * 512 * planes * 2;
-}
+// rows of a split buffer: m rounded up to the 128-row column tiles, plus 256
+// zero rows, so that a 256-row tile that starts inside the rounded range reads
+// only written rows
+int split_rows(int m) { return (m + split_detail::kCols - 1) / split_detail::kCols * split_detail::kCols + split_detail::kRows; }
 
 namespace {
-void check_products(int products)
+void check_split_mode(int mode)
 {
-    if (products != 3 && products != 6) throw std::invalid_argument("split update: products must be 3 or 6");
+    if (mode != split_detail::kSplitBf16x3 && mode != split_detail::kSplitBf16x6 && mode != split_detail::kSplitFp16x3)
+        throw std::invalid_argument("split update: mode must be 1 (bf16x3), 2 (bf16x6) or 3 (fp16x3)");
 }
+
+// bytes of the fragment data of a split of `rows` rows and kb columns; the
+// fp16x3 row exponents follow it
+size_t split_fragment_bytes(int rows, int nks, int planes) { return (size_t)rows * nks * split_detail::kSlab * planes * 2; }
 } // namespace
 
-void split_panel(hipStream_t s, int products, bool upper, int m, int kb, const float* X, long long ldx, void* out)
+size_t split_buffer_bytes(int n, int mode)
 {
-    check_products(products);
+    check_split_mode(mode);
+    const int rows = split_rows(n), planes = split_detail::split_planes(mode);
+    return split_fragment_bytes(rows, 512 / split_detail::kSlab, planes) + (size_t)rows * sizeof(int);
+}
+
+void split_panel(hipStream_t s, int mode, bool upper, int m, int kb, const float* X, long long ldx, void* out)
+{
+    check_split_mode(mode);
     if (m <= 0 || kb <= 0) return;
     if (kb > 512) throw std::invalid_argument("split_panel: at most 512 columns");
     const int rows = split_rows(m), nks = (kb + split_detail::kSlab - 1) / split_detail::kSlab;
-    const dim3 grid((unsigned)((rows + 255) / 256), (unsigned)(4 * nks)), block(256);
+    const int planes = split_detail::split_planes(mode);
     auto* o = static_cast<split_detail::u4_t*>(out);
-    if (products == 6) split_detail::split_panel_kernel<3><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, o);
-    else split_detail::split_panel_kernel<2><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, o);
+    int* exps = reinterpret_cast<int*>(static_cast<char*>(out) + split_fragment_bytes(rows, nks, planes));
+    const dim3 grid((unsigned)((rows + 255) / 256), (unsigned)(4 * nks)), block(256);
+    if (mode == split_detail::kSplitFp16x3) {
+        split_detail::split_rowexp_kernel<<<(rows + 255) / 256, block, 0, s>>>(m, kb, rows, X, ldx, upper, exps);
+        check_launch("split_panel");
+        split_detail::split_panel_kernel<2, true><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, exps, o);
+    } else if (mode == split_detail::kSplitBf16x6) {
+        split_detail::split_panel_kernel<3, false><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, nullptr, o);
+    } else {
+        split_detail::split_panel_kernel<2, false><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, nullptr, o);
+    }
     check_launch("split_panel");
 }
 
-void split_update(hipStream_t s, int products, bool upper, int m, int n2, int kb, const void* P, int m_split,
+void split_update(hipStream_t s, int mode, bool upper, int m, int n2, int kb, const void* P, int m_split,
                   int a_row, int b_row, float* C, long long ldc, bool tri, long long diag, const int* abort)
 {
-    check_products(products);
+    check_split_mode(mode);
+    if (upper) throw std::invalid_argument("split_update: the split products update the lower view only");
     if (m <= 0 || n2 <= 0 || kb <= 0) return;
-    using split_detail::kTile;
-    const int planes = products == 6 ? 3 : 2, rows = split_rows(m_split);
-    const int ntm = (m + kTile - 1) / kTile, ntn = (n2 + kTile - 1) / kTile;
+    using split_detail::kCols;
+    using split_detail::kRows;
+    const int planes = split_detail::split_planes(mode), rows = split_rows(m_split);
+    const int nI = (m + kRows - 1) / kRows, nt = (n2 + kCols - 1) / kCols;
     if (kb > 512 || a_row % 16 != 0 || b_row % 16 != 0 || a_row < 0 || b_row < 0 ||
-        a_row + (long long)ntm * kTile > rows || b_row + (long long)ntn * kTile > rows)
+        a_row + (long long)nI * kRows > rows || b_row + (long long)nt * kCols > rows)
         throw std::invalid_argument("split_update: rows outside the split buffer");
-    if ((long long)kTile * ldc >= INT_MAX) throw std::invalid_argument("split_update: leading dimension too large");
+    if ((long long)kCols * ldc >= INT_MAX) throw std::invalid_argument("split_update: leading dimension too large");
     const int nks = (kb + split_detail::kSlab - 1) / split_detail::kSlab;
     const long long rb_units = (long long)nks * planes * split_detail::kFragU4;  // 16-byte units per 16-row block
     const auto* base = static_cast<const split_detail::u4_t*>(P);
+    const int* exps = reinterpret_cast<const int*>(static_cast<const char*>(P) + split_fragment_bytes(rows, nks, planes));
     split_detail::SplitProblem p{m, n2, nks, base + (a_row / 16) * rb_units, base + (b_row / 16) * rb_units, C,
-                                 ldc, tri ? 1 : 0, diag};
-    TileOrder o;
-    o.ntm = ntm;
-    o.ntn = ntn;
-    o.lower_enum = (tri && diag == 0 && m == n2) ? 1 : 0;
-    const long long total = o.lower_enum ? (long long)ntm * (ntm + 1) / 2 : (long long)ntm * ntn;
+                                 ldc, tri ? 1 : 0, diag, exps + a_row, exps + b_row};
+    split_detail::SplitOrder o;
+    o.nI = nI;
+    o.nt = nt;
+    o.lower = (tri && diag == 0 && m == n2) ? 1 : 0;
+    long long total = 0;
+    if (o.lower) {
+        for (int g = 0; g < (nt + 7) / 8; ++g) total += split_detail::split_group_tiles(g, nI, nt);
+    } else {
+        total = (long long)nI * nt;
+    }
     if (total > (long long)INT_MAX / 2) throw std::invalid_argument("split_update: problem too large for one launch");
     o.total = (int)total;
-    o.group = 8;
     o.per_xcd = (o.total + 7) / 8;
     const dim3 grid((unsigned)(o.per_xcd * 8)), block(256);
-    if (products == 3) {
-        if (upper) split_detail::split_syrk_kernel<3, true, 2><<<grid, block, 0, s>>>(p, o, abort);
-        else split_detail::split_syrk_kernel<3, false, 2><<<grid, block, 0, s>>>(p, o, abort);
-    } else {
-        if (upper) split_detail::split_syrk_kernel<6, true, 1><<<grid, block, 0, s>>>(p, o, abort);
-        else split_detail::split_syrk_kernel<6, false, 1><<<grid, block, 0, s>>>(p, o, abort);
-    }
+    if (mode == split_detail::kSplitBf16x3) split_detail::split_syrk_kernel<split_detail::kSplitBf16x3><<<grid, block, 0, s>>>(p, o, abort);
+    else if (mode == split_detail::kSplitBf16x6) split_detail::split_syrk_kernel<split_detail::kSplitBf16x6><<<grid, block, 0, s>>>(p, o, abort);
+    else split_detail::split_syrk_kernel<split_detail::kSplitFp16x3><<<grid, block, 0, s>>>(p, o, abort);
     check_launch("split_update");
 }
 

@@ -22,7 +22,7 @@ namespace ddla {
 template <class T>
 void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work);
 void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long lda, int* info, void* work,
-                          int products, void* const* bufs);
+                          int mode, void* const* bufs);
 
 namespace {
 
@@ -56,15 +56,15 @@ MixedLayout mixed_layout(int n, int nrhs)
     l.sync = put(kern::solve_work_bytes(nrhs));
     l.error = put(sizeof(int));
     l.stage = put(kern::stage_work_bytes(n, nrhs, sizeof(float)));
-    for (std::size_t& b : l.split) b = put(kern::split_buffer_bytes(n, 6));
+    for (std::size_t& b : l.split) b = put(kern::split_buffer_bytes(n, 2));  // the largest mode (three planes)
     l.total = off;
     return l;
 }
 
 void check_mixed_options(int factor, double tol, int max_iter)
 {
-    if (factor < 0 || factor > 2)
-        throw std::invalid_argument("pposv_mixed: factor must be 0 (FP32), 1 (bf16x3) or 2 (bf16x6)");
+    if (factor < 0 || factor > 3)
+        throw std::invalid_argument("pposv_mixed: factor must be 0 (FP32), 1 (bf16x3), 2 (bf16x6) or 3 (fp16x3)");
     if (!(tol > 0) || max_iter < 0) throw std::invalid_argument("pposv_mixed: tol must be > 0 and max_iter >= 0");
 }
 
@@ -128,7 +128,7 @@ void pposv_mixed_run(const DdlaHandle_t& h, char uplo, int n, int nrhs, T* A, co
         potrf_blocked_dispatch<float>(h, upper, n, a32, n, info32, base + l.panel);
     } else {
         void* const bufs[3] = {base + l.split[0], base + l.split[1], base + l.split[2]};
-        potrf_split_dispatch(h, upper, n, a32, n, info32, base + l.panel, factor == 1 ? 3 : 6, bufs);
+        potrf_split_dispatch(h, upper, n, a32, n, info32, base + l.panel, factor, bufs);
     }
     int host_info = 0, panel_error = 0;
     DDLA_HIP(hipMemcpyAsync(&host_info, info32, sizeof(int), hipMemcpyDeviceToHost, s));

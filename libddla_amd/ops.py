@@ -143,6 +143,7 @@ def pposv_run(A, B, uplo, backend_work: int, backend_bytes: int, side: str = "L"
 FACTOR_FP32 = 0
 FACTOR_BF16X3 = 1  # trailing updates as three bfloat16 split products (FP32 accumulation)
 FACTOR_BF16X6 = 2  # six split products, about FP32 accuracy
+FACTOR_FP16X3 = 3  # three half-precision split products of row-scaled panels (~2^-22), at bf16x3's cost
 
 
 def _mixed_args(A, B, uplo):
@@ -156,7 +157,8 @@ def pposv_mixed(A, B, uplo: str = "L", factor: int = FACTOR_FP32, tol: float = N
 
     factor selects the trailing updates of the FP32 factorization: FACTOR_FP32
     (FP32 MFMA), FACTOR_BF16X3 or FACTOR_BF16X6 (split bfloat16 products,
-    FP32 accumulation).
+    FP32 accumulation), FACTOR_FP16X3 (split half-precision products of
+    row-scaled panels, FP32 accumulation).
 
     Returns (B, iter, info): iter >= 0 is the number of corrections applied;
     -3 (FP32 factor failed) or -(max_iter + 1) (no convergence) mean the FP64

@@ -126,7 +126,7 @@ def test_mixed_config_at_full_size(handle, delta, kappa):
     a, b = a0.clone(), b0.clone()
     A = dd.DistributedMatrix.from_torch(handle, a, n, n, 512, 512)
     B = dd.DistributedMatrix.from_torch(handle, b, n, nrhs, 512, 512)
-    for factor in (dd.FACTOR_FP32, dd.FACTOR_BF16X3, dd.FACTOR_BF16X6):
+    for factor in (dd.FACTOR_FP32, dd.FACTOR_BF16X3, dd.FACTOR_BF16X6, dd.FACTOR_FP16X3):
         a.copy_(a0)
         b.copy_(b0)
         torch.cuda.synchronize()

@@ -46,7 +46,7 @@ def stored(a, uplo):
     return np.where(mask, a, np.nan)
 
 
-FACTORS = [dd.FACTOR_FP32, dd.FACTOR_BF16X3, dd.FACTOR_BF16X6]
+FACTORS = [dd.FACTOR_FP32, dd.FACTOR_BF16X3, dd.FACTOR_BF16X6, dd.FACTOR_FP16X3]
 
 
 @pytest.mark.parametrize("factor", FACTORS)
@@ -92,6 +92,37 @@ def test_split_factor_modes_three_panels(handle, uplo):
         its[factor] = it
     assert its[dd.FACTOR_BF16X6] <= its[dd.FACTOR_FP32] + 1, its
     assert its[dd.FACTOR_BF16X3] >= its[dd.FACTOR_BF16X6], its
+    assert its[dd.FACTOR_FP16X3] <= its[dd.FACTOR_BF16X3], its
+
+
+@pytest.mark.parametrize("factor", FACTORS)
+@pytest.mark.parametrize("kind", ["row_scaled", "block_diagonal"])
+def test_scaled_and_block_diagonal_systems(handle, kind, factor):
+    """n = 1100 (panels of 512, 512, 76 columns).  row_scaled: A = D S D with
+    D = diag(2^k), k uniform in [-24, 24] (exact in FP64 and FP32), so panel
+    rows span 2^48 in magnitude -- far outside half precision's range, which
+    fp16x3's per-row power-of-two scaling must absorb.  block_diagonal: two
+    SPD blocks of 700 and 400, so whole rows of the first panel below its
+    diagonal block are zero (their scale exponent is 0).  Every mode
+    converges to the 1e-14 backward error."""
+    n, nrhs, tol = 1100, 3, 1e-14
+    print(f"case mixed {kind} n={n} factor={factor}", flush=True)
+    s = orc.spd_matrix(n, 5, 0.1)
+    if kind == "row_scaled":
+        k = np.random.RandomState(6).randint(-24, 25, size=n)
+        d = np.ldexp(1.0, k)
+        a = s * d[:, None] * d[None, :]
+    else:
+        a = np.zeros((n, n))
+        a[:700, :700] = s[:700, :700]
+        a[700:, 700:] = s[700:, 700:]
+    b = orc.normal_matrix(5, 1, n, nrhs)
+    A, B = upload(handle, stored(a, "L")), upload(handle, b)
+    _, it, info = dd.pposv_mixed(A, B, "L", factor=factor, tol=tol)
+    # the normwise test can pass with no correction when the scaling makes
+    # the largest rows dominate
+    assert info == 0 and 0 <= it <= 30, (it, info)
+    assert backward_errors(a, B.to_global_matrix(), b).max() <= tol
 
 
 @pytest.mark.parametrize("uplo", ["L", "U"])
@@ -224,7 +255,7 @@ def test_workspace_api_matches_driver(handle):
 def test_invalid_arguments(handle):
     a = orc.spd_matrix(64, 1, 1.0)
     A, B = upload(handle, a), upload(handle, np.ones((64, 1)))
-    for factor in (3, -1):
+    for factor in (4, -1):
         with pytest.raises(dd.DdlaExceptionError):
             dd.pposv_mixed(A, B, factor=factor)
     with pytest.raises(dd.DdlaExceptionError):

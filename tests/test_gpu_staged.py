@@ -105,7 +105,7 @@ def test_fp32_staged_backward_error(handle):
     assert berr <= 0.05 * n * EPS32, berr
 
 
-@pytest.mark.parametrize("factor", [dd.FACTOR_FP32, dd.FACTOR_BF16X3])
+@pytest.mark.parametrize("factor", [dd.FACTOR_FP32, dd.FACTOR_BF16X3, dd.FACTOR_FP16X3])
 def test_mixed_refinement_through_staged_solve(handle, factor):
     import torch
 
