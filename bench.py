@@ -400,8 +400,9 @@ def gp_leg(h, dd, args):
         xt = torch.tensor(x, device=dev)
         prepared.append((xt, torch.tensor(y, device=dev)))
 
-    def timed(loss_fn):
-        """Seconds for forward + backward of every problem (K formed outside), and problem 0's gradients."""
+    def timed(loss_fn, handles=None):
+        """Seconds for forward + backward of every problem (K formed outside), and problem 0's gradients;
+        with handles, the problems run concurrently on them (one host thread and stream each)."""
         times, grads = [], []
         for rep in range(args.warmup + 1):
             ks = []
@@ -412,9 +413,12 @@ def gp_leg(h, dd, args):
                 ks.append((l, K, Kd, yt))
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            for l, K, Kd, yt in ks:
-                loss = loss_fn(Kd, yt, h)
-                loss.backward()
+            if handles:
+                ag.backward_concurrently([(Kd, yt) for l, K, Kd, yt in ks], loss_fn, handles)
+            else:
+                for l, K, Kd, yt in ks:
+                    loss = loss_fn(Kd, yt, h)
+                    loss.backward()
             torch.cuda.synchronize()
             t = time.perf_counter() - t0
             if rep == args.warmup:
@@ -424,11 +428,15 @@ def gp_leg(h, dd, args):
                     grads.append((Kd.grad.detach().cpu().numpy(), float(l.grad), K.detach().cpu().numpy()))
         return times[0], grads[0]
 
-    t, (kbar, dl, k) = timed(ag.gp_negative_log_likelihood)
+    workers = [dd.Handle(dd.Backend.GPU) for _ in range(max(0, args.gp_workers))] if args.gp_workers > 1 else None
+    t, (kbar, dl, k) = timed(ag.gp_negative_log_likelihood, workers)
+    t_seq, _ = timed(ag.gp_negative_log_likelihood) if workers else (t, None)
     if args.no_gp_composed:
         t_comp, kbar_c, dl_c = None, None, None
     else:
-        t_comp, (kbar_c, dl_c, _) = timed(ag.gp_negative_log_likelihood_composed)
+        t_comp, (kbar_c, dl_c, _) = timed(ag.gp_negative_log_likelihood_composed, workers)
+    for w in workers or []:
+        w.destroy()
 
     def lower_product_flops(m):
         # tril(X^T Y) of lower-triangular X, Y in four k blocks [k0, k1) of growing order, masked
@@ -455,12 +463,17 @@ def gp_leg(h, dd, args):
     dl_ref = float((kbar_ref * (k - 1e-2 * np.eye(n)) * d2 / 0.2 ** 3).sum())
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     return {"problems": problems, "n": n, "ms": round(t * 1e3, 2), "tflops": round(flops / t / 1e12, 3),
+            "handles": max(1, args.gp_workers),
+            "sequential_ms": round(t_seq * 1e3, 2),
             "flops_convention": "forward n^3/3 + 4n^2; backward K^-1 from the factor (recursive triangular "
                                 "inverse n^3/2 + tril(L^-T L^-1) in four k blocks, about n^2/2 * n) + 2n^2",
             "dK_rel_err_vs_fp64_closed_form": rel(kbar, kbar_ref),
             "dl_rel_err_vs_fp64_closed_form": abs(dl - dl_ref) / abs(dl_ref),
             "timed": "forward + backward of all problems through torch.autograd, K given: one Function "
-                     "(ppotrf, ppotrs, diag_logdet; backward (K^-1 - alpha alpha^T) / 2 with K^-1 from potri)",
+                     "(ppotrf, ppotrs, diag_logdet; backward (K^-1 - alpha alpha^T) / 2 with K^-1 from potri); "
+                     "the problems run concurrently on `handles` handles (libddla_amd.autograd."
+                     "backward_concurrently: one host thread and stream each); sequential_ms is one problem "
+                     "after another on one handle",
             "composed": None if t_comp is None else {
                 "ms": round(t_comp * 1e3, 2), "tflops": round(flops_comp / t_comp / 1e12, 3),
                 "dK_rel_err_vs_fp64_closed_form": rel(kbar_c, kbar_ref),
@@ -487,6 +500,8 @@ def main():
     ap.add_argument("--batched", type=int, default=4096, help="members of the batched leg (0 skips it)")
     ap.add_argument("--gp-problems", type=int, default=16, help="problems of the GP-likelihood leg (0 skips it)")
     ap.add_argument("--gp-n", type=int, default=4096)
+    ap.add_argument("--gp-workers", type=int, default=4,
+                    help="handles (host threads and streams) the GP problems run on concurrently")
     ap.add_argument("--no-gp-composed", action="store_true", help="time only the fused GP likelihood Function")
     ap.add_argument("--no-bottom-right", action="store_true", help="skip the ppotrf_bottom_right leg")
     ap.add_argument("--uplo", choices=["L", "U"], default="L", help="triangle of the headline pposv")

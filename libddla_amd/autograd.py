@@ -13,7 +13,8 @@ Backward passes (ddla_{s,d}_*_backward):
 gp_negative_log_likelihood(K, y, h) is one Function whose backward is the
 closed form (K^-1 - alpha alpha^T) / 2 with K^-1 from ddla_{s,d}_potri;
 gp_negative_log_likelihood_composed builds the same loss from the three
-Functions above.
+Functions above; backward_concurrently runs independent problems' forward
+and backward passes on several handles at once.
 The library works on column-major storage; tensors are passed through
 column-major copies (or views, when they already are), and torch's current
 stream is synchronized before each call (every call returns synchronized).
@@ -236,6 +237,51 @@ def gp_negative_log_likelihood(K, y, h):
     """0.5 y^T K^-1 y + 0.5 log det K (the constant n/2 log 2 pi omitted),
     with the fused closed-form backward."""
     return GPNegativeLogLikelihood.apply(K, y, h)
+
+
+def backward_concurrently(problems, loss_fn, handles):
+    """Forward and backward of independent losses at once.
+
+    Problem i (an argument tuple for loss_fn(*args, h)) runs on
+    handles[i % len(handles)]; each handle is driven by a host thread of its
+    own, with torch's work for its problems enqueued on the handle's compute
+    stream (torch.cuda.ExternalStream), so different problems' kernels overlap
+    on the GPU -- one problem's latency-bound panel factorization beside
+    another's products.  Gradients accumulate into the inputs' .grad as
+    loss.backward() would.  The caller's current stream is synchronized first
+    (the inputs are ready) and every handle's stream before returning.  Returns
+    the detached losses in problem order; the first exception raised by a
+    worker is re-raised."""
+    import threading
+
+    workers = len(handles)
+    if workers < 1:
+        raise ValueError("backward_concurrently needs at least one handle")
+    torch.cuda.current_stream().synchronize()
+    losses = [None] * len(problems)
+    errors = []
+
+    def work(w):
+        h = handles[w]
+        try:
+            stream = torch.cuda.ExternalStream(h.stream)
+            with torch.cuda.stream(stream):
+                for i in range(w, len(problems), workers):
+                    loss = loss_fn(*problems[i], h)
+                    loss.backward()
+                    losses[i] = loss.detach()
+            stream.synchronize()
+        except BaseException as e:  # re-raised on the calling thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(w,)) for w in range(workers)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return losses
 
 
 def gp_negative_log_likelihood_composed(K, y, h):

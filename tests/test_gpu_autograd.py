@@ -172,6 +172,40 @@ def test_gp_likelihood_gradient_n4096(handle, which):
         assert abs(fd - want) <= 1e-7 * scale, (i, j, fd, want)
 
 
+@pytest.mark.parametrize("workers", [1, 3])
+def test_gp_backward_concurrently_matches_sequential(handle, workers):
+    """Five independent GP problems (n = 700) on `workers` handles, each from
+    its own host thread and stream: losses and dK bit for bit those of one
+    problem after another on one handle."""
+    import libddla_amd as dd
+
+    print(f"case gp concurrent workers={workers}", flush=True)
+    n, count = 700, 5
+    rng = np.random.RandomState(9)
+    ks, ys = [], []
+    for q in range(count):
+        x = torch.tensor(rng.uniform(size=(n, 3)), device=DEV)
+        l = torch.tensor(0.2 + 0.01 * q, dtype=torch.float64, device=DEV)
+        ks.append(gp_kernel(x, l)[0].detach())
+        ys.append(torch.tensor(rng.standard_normal(n), device=DEV))
+    seq = []
+    for K, y in zip(ks, ys):
+        Kd = K.clone().requires_grad_()
+        loss = ag.gp_negative_log_likelihood(Kd, y, handle)
+        loss.backward()
+        seq.append((float(loss.detach()), Kd.grad.cpu().numpy()))
+    handles = [dd.Handle(dd.Backend.GPU) for _ in range(workers)]
+    try:
+        kds = [K.clone().requires_grad_() for K in ks]
+        losses = ag.backward_concurrently(list(zip(kds, ys)), ag.gp_negative_log_likelihood, handles)
+        for (l0, g0), l1, Kd in zip(seq, losses, kds):
+            assert float(l1) == l0
+            assert np.array_equal(Kd.grad.cpu().numpy(), g0)
+    finally:
+        for h in handles:
+            h.destroy()
+
+
 def test_autograd_rejects_host_tensors_and_non_pd(handle):
     with pytest.raises(ValueError):
         ag.cholesky(torch.eye(3, dtype=torch.float64), handle)
