@@ -322,6 +322,7 @@ def batched_leg(h, dd, args):
         t0 = time.perf_counter()
         occ, ordered = dd.pposv_vbatched_tune(vb, "L")
         tune_s = time.perf_counter() - t0
+        sched = dd.vbatched_schedule(h, dtype)
         g = dd.BatchedGraph(vb, "L")
         times = []
         for rep in range(args.warmup + args.steps):
@@ -372,7 +373,8 @@ def batched_leg(h, dd, args):
             "max_logdet_diff_vs_loop": float(np.abs(logdet - loop_logdet).max()),
             "max_rel_solution_diff_vs_loop": dx,
             "tuned": {"workgroups_per_cu": occ, "largest_first": ordered,
-                      "blocked_path_above_order": 64 * dd.vbatched_schedule(h, dtype)["split_blocks"],
+                      "kernel": "left-looking" if sched["left_looking"] else "right-looking",
+                      "blocked_path_above_order": None if sched["left_looking"] else 64 * sched["split_blocks"],
                       "tune_s": round(tune_s, 2)},
         }
         log(f"batched {name}: {t * 1e3:.3f} ms ({flops / t / 1e12:.2f} TF/s), loop {t_loop * 1e3:.0f} ms")
@@ -428,8 +430,34 @@ def gp_leg(h, dd, args):
                     grads.append((Kd.grad.detach().cpu().numpy(), float(l.grad), K.detach().cpu().numpy()))
         return times[0], grads[0]
 
+    def timed_batched():
+        """The same problems stacked (K: problems x n x n) through the batched Function, whose every launch
+        serves all problems; seconds for forward + backward (K formed outside), problem 0's gradients."""
+        result = None
+        for rep in range(args.warmup + 1):
+            l = torch.tensor(0.2, dtype=torch.float64, device=dev, requires_grad=True)
+            parts = [kernel(xt, l) for xt, _ in prepared]
+            K = torch.stack([p[0] for p in parts])
+            Kd = K.detach().requires_grad_()
+            Y = torch.stack([yt for _, yt in prepared])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            losses = ag.gp_negative_log_likelihood_batched(Kd, Y, h)
+            losses.sum().backward()
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if rep == args.warmup:
+                K.backward(Kd.grad)  # dLoss/dl (summed over the problems) outside the timed region
+                g0 = Kd.grad[0].detach().cpu().numpy()
+                # problem 0's dl alone: its dK contracted with dK_0/dl
+                dk0 = (parts[0][0].detach() - 1e-2 * torch.eye(n, dtype=torch.float64, device=dev)) * parts[0][1] / 0.2 ** 3
+                dl0 = float((Kd.grad[0] * dk0).sum())
+                result = (t, g0, dl0, parts[0][0].detach().cpu().numpy())
+        return result
+
+    t_b, kbar_b, dl_b, k = timed_batched()
     workers = [dd.Handle(dd.Backend.GPU) for _ in range(max(0, args.gp_workers))] if args.gp_workers > 1 else None
-    t, (kbar, dl, k) = timed(ag.gp_negative_log_likelihood, workers)
+    t, (kbar, dl, _) = timed(ag.gp_negative_log_likelihood, workers)
     t_seq, _ = timed(ag.gp_negative_log_likelihood) if workers else (t, None)
     if args.no_gp_composed:
         t_comp, kbar_c, dl_c = None, None, None
@@ -448,6 +476,22 @@ def gp_leg(h, dd, args):
                 total += k1 * (k1 + 1.0) * (k1 - k0)
         return total
 
+    def lower_inverse_flops(m):
+        # L^-1 by merging inverted 64 x 64 diagonal blocks: X21 = -X22 (L21 X11), two products per merge
+        f, hh = 0.0, 64
+        while hh < m:
+            full = m // (2 * hh)
+            o = full * 2 * hh
+            f += full * 4.0 * hh ** 3
+            if o + hh < m:
+                n2 = m - o - hh
+                f += 2.0 * n2 * hh * hh + 2.0 * n2 * hh * n2
+            hh *= 2
+        return f
+
+    # batched: forward potrf n^3/3 + L^-1 (lower_inverse_flops) + z = L^-1 y (2n^2); backward tril(X^T X)
+    # (lower_product_flops) + a = X^T z (2n^2) + the gradient (n^2)
+    flops_b = problems * (n ** 3 / 3 + lower_inverse_flops(n) + 2.0 * n * n + lower_product_flops(n) + 3.0 * n * n)
     # forward: potrf n^3/3, two one-column sweeps 2n^2; backward (fused): K^-1 from the factor --
     # tril(L^-1) by recursive inversion (n^3/2) and tril(L^-T L^-1) (lower_product_flops) -- and the
     # rank-1 update (2n^2)
@@ -462,18 +506,27 @@ def gp_leg(h, dd, args):
     d2 = ((data[0][0][:, None, :] - data[0][0][None, :, :]) ** 2).sum(-1)
     dl_ref = float((kbar_ref * (k - 1e-2 * np.eye(n)) * d2 / 0.2 ** 3).sum())
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    return {"problems": problems, "n": n, "ms": round(t * 1e3, 2), "tflops": round(flops / t / 1e12, 3),
-            "handles": max(1, args.gp_workers),
-            "sequential_ms": round(t_seq * 1e3, 2),
-            "flops_convention": "forward n^3/3 + 4n^2; backward K^-1 from the factor (recursive triangular "
-                                "inverse n^3/2 + tril(L^-T L^-1) in four k blocks, about n^2/2 * n) + 2n^2",
-            "dK_rel_err_vs_fp64_closed_form": rel(kbar, kbar_ref),
-            "dl_rel_err_vs_fp64_closed_form": abs(dl - dl_ref) / abs(dl_ref),
-            "timed": "forward + backward of all problems through torch.autograd, K given: one Function "
-                     "(ppotrf, ppotrs, diag_logdet; backward (K^-1 - alpha alpha^T) / 2 with K^-1 from potri); "
-                     "the problems run concurrently on `handles` handles (libddla_amd.autograd."
-                     "backward_concurrently: one host thread and stream each); sequential_ms is one problem "
-                     "after another on one handle",
+    return {"problems": problems, "n": n, "ms": round(t_b * 1e3, 2), "tflops": round(flops_b / t_b / 1e12, 3),
+            "flops_convention": "per problem: forward n^3/3 (potrf) + L^-1 by merging inverted 64 x 64 diagonal "
+                                "blocks (2/3 n^3 at n = 4096) + 2n^2; backward tril(L^-T L^-1) in four k blocks "
+                                "(about n^2/2 * n) + 3n^2",
+            "dK_rel_err_vs_fp64_closed_form": rel(kbar_b, kbar_ref),
+            "dl_rel_err_vs_fp64_closed_form": abs(dl_b - dl_ref) / abs(dl_ref),
+            "timed": "forward + backward of all problems through torch.autograd, K given: the batched Function "
+                     "(libddla_amd.autograd.gp_negative_log_likelihood_batched, K stacked problems x n x n): "
+                     "every launch -- each panel and trailing update of the factorization, each merge level of "
+                     "L^-1, the products -- serves all problems",
+            "per_problem": {
+                "ms": round(t * 1e3, 2), "tflops": round(flops / t / 1e12, 3),
+                "handles": max(1, args.gp_workers), "sequential_ms": round(t_seq * 1e3, 2),
+                "flops_convention": "forward n^3/3 + 4n^2; backward K^-1 from the factor (recursive triangular "
+                                    "inverse n^3/2 + tril(L^-T L^-1) in four k blocks, about n^2/2 * n) + 2n^2",
+                "dK_rel_err_vs_fp64_closed_form": rel(kbar, kbar_ref),
+                "dl_rel_err_vs_fp64_closed_form": abs(dl - dl_ref) / abs(dl_ref),
+                "timed": "one fused Function per problem (ppotrf, ppotrs, diag_logdet; backward (K^-1 - alpha "
+                         "alpha^T) / 2 with K^-1 from potri), the problems concurrently on `handles` handles "
+                         "(libddla_amd.autograd.backward_concurrently: one host thread and stream each); "
+                         "sequential_ms is one problem after another on one handle"},
             "composed": None if t_comp is None else {
                 "ms": round(t_comp * 1e3, 2), "tflops": round(flops_comp / t_comp / 1e12, 3),
                 "dK_rel_err_vs_fp64_closed_form": rel(kbar_c, kbar_ref),

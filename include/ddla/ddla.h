@@ -466,6 +466,25 @@ void logdet_backward(const DdlaHandle_t& handle, int n, const T* L, int ldl, dou
 template <typename T>
 void potri(const DdlaHandle_t& handle, char uplo, int n, const T* L, int ldl, T* Ainv, int lda);
 
+// Batched Gaussian-process negative log-likelihood (no counterpart in the
+// reference): for `batch` independent problems of order n (FP64, device
+// memory), loss_b = y_b^T K_b^-1 y_b / 2 + log det K_b / 2 from the lower
+// triangle of K_b (column-major, leading dimension n, problem b at
+// K + b * k_stride), y_b at y + b * y_stride.  Every launch serves all
+// problems at once.  work holds batch * gpBatchedWorkspaceSize(n) bytes
+// (256-byte aligned) and carries the factor, L^-1 and L^-1 y from the
+// forward call to the backward call.  The forward call writes loss[b] and
+// info[b] (device; info as for potrf, the loss is meaningless where it is
+// nonzero) and returns synchronized; the backward call writes, for the
+// upstream gradients g[b] (device), Kbar_b = g_b (K_b^-1 - a a^T) / 2 (full
+// symmetric, a = K_b^-1 y_b, at Kbar + b * kbar_stride) and, when ybar is not
+// null, ybar_b = g_b a, asynchronously on the handle's stream.
+std::size_t gpBatchedWorkspaceSize(int n);
+void gpBatchedForward(const DdlaHandle_t& handle, int batch, int n, const double* K, long long k_stride,
+                      const double* y, long long y_stride, void* work, double* loss, int* info);
+void gpBatchedBackward(const DdlaHandle_t& handle, int batch, int n, void* work, const double* g, double* Kbar,
+                       long long kbar_stride, double* ybar, long long ybar_stride);
+
 namespace cholesky_impl {
 // Three components per routine: <r>_check (validation), <r>_workspace_size
 // (backend and host bytes; no allocation) and <r>_run on caller scratch (no

@@ -80,6 +80,16 @@ int ddla_c_set_wait_limit(int spins);
  * depend on it beyond rounding; it exists so that tests can run the wide
  * panels of large orders on small matrices. */
 int ddla_c_set_panel_width(int cols);
+/* The stream schedule of the blocked factorization for order n and panel
+ * width panel_width (<= 0: the FP64 rule), without a GPU: *count steps, the
+ * first min(*count, capacity) written to steps as 5 ints each, (op, stream,
+ * a, b, c) in the order the host issues them: op 0 resets the panel
+ * counters, 1 factors panel a, 2 applies panel a to columns [b, c) (rows from
+ * b), all three launched on stream 0 (the handle's stream), 1 (side) or 2
+ * (look-ahead); op 3 records event a on the stream, op 4 makes the stream
+ * wait for event a (0 panel, 1 look-ahead, 2 rest).  No counterpart in the
+ * reference; it lets a caller check the schedule's ordering on the host. */
+int ddla_c_potrf_schedule(int n, int panel_width, int* steps, int capacity, int* count);
 /* ddlaSetStream / ddlaGetStream (include/ddla/ddla_handle_t.h:300, :337);
  * no C entry in the reference. */
 int ddla_c_set_stream(void* handle, void* stream);
@@ -389,8 +399,18 @@ int ddla_c_graph_destroy(void* graph);
 int ddla_c_vbatched_schedule(void* handle, int element_bytes, int* tuned, int* occupancy, int* ordered,
                              int* split_blocks);
 /* Pins that schedule without timing (occupancy 1 or 2; split_blocks in
- * [1, 16], used only with ordered != 0); marks it tuned. */
+ * [1, 16], used only with ordered != 0) on the right-looking kernels; marks it
+ * tuned. */
 int ddla_c_vbatched_set_schedule(void* handle, int element_bytes, int occupancy, int ordered, int split_blocks);
+/* Which persistent kernel the schedule runs: *left_looking = 1 for the
+ * left-looking kernel (every member in one launch, each block column formed
+ * once by one MFMA product; split_blocks unused), 0 for the right-looking
+ * kernel with the optional blocked path.  No counterpart in the reference
+ * (it has no batched Cholesky). */
+int ddla_c_vbatched_kernel(void* handle, int element_bytes, int* left_looking);
+/* Selects that kernel (0 or 1) for the schedule held (an untuned handle
+ * first takes 1 workgroup per CU, largest first); marks it tuned. */
+int ddla_c_vbatched_set_kernel(void* handle, int element_bytes, int left_looking);
 
 /* ---- reverse-mode adjoints (no counterpart in the reference), factor of
  * A = L L^T in the lower triangle (uplo 'L'); each returns synchronized:
@@ -417,6 +437,25 @@ int ddla_d_logdet_backward(void* handle, int n, const void* L, int ldl, double g
  * Returns synchronized. */
 int ddla_s_potri(void* handle, char uplo, int n, const void* L, int ldl, void* Ainv, int lda);
 int ddla_d_potri(void* handle, char uplo, int n, const void* L, int ldl, void* Ainv, int lda);
+
+/* ---- batched Gaussian-process likelihood (no counterpart in the reference;
+ * callers that evaluate many independent GP likelihoods of one order at once).
+ * For problem b < batch (FP64, device memory): K_b column-major n x n (leading
+ * dimension n, lower triangle read) at K + b * k_stride, y_b at y + b *
+ * y_stride; loss[b] = y_b^T K_b^-1 y_b / 2 + log det K_b / 2.  Each launch
+ * serves every problem.  work: batch * (*bytes_per_problem) bytes, 256-byte
+ * aligned, kept between the forward and the backward call (it carries the
+ * factor, L^-1 and L^-1 y).  forward writes loss[b] and info[b] (potrf's
+ * info; the loss is meaningless where it is nonzero) and returns
+ * synchronized; backward writes Kbar_b = g[b] (K_b^-1 - a a^T) / 2 (full
+ * symmetric, at Kbar + b * kbar_stride) and, unless ybar is null, ybar_b =
+ * g[b] a (a = K_b^-1 y_b) for the device upstream gradients g, asynchronously
+ * on the handle's stream.  batch in [0, 65535], n >= 1. */
+int ddla_d_gp_batched_workspace_size(int n, long long* bytes_per_problem);
+int ddla_d_gp_batched_forward(void* handle, int batch, int n, const void* K, long long k_stride, const void* y,
+                              long long y_stride, void* work, void* loss, int* info);
+int ddla_d_gp_batched_backward(void* handle, int batch, int n, void* work, const void* g, void* Kbar,
+                               long long kbar_stride, void* ybar, long long ybar_stride);
 
 /* ---- harness helpers (no counterpart in the reference) ------------------
  * Philox4x64-10 + Box-Muller normals of the project's seed convention

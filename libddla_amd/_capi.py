@@ -122,6 +122,12 @@ _SIGNATURES = {
     "ddla_c_graph_destroy": (_i, [_vp]),
     "ddla_c_vbatched_schedule": (_i, [_vp, _i, _ip, _ip, _ip, _ip]),
     "ddla_c_vbatched_set_schedule": (_i, [_vp, _i, _i, _i, _i]),
+    "ddla_c_vbatched_kernel": (_i, [_vp, _i, _ip]),
+    "ddla_c_potrf_schedule": (_i, [_i, _i, _ip, _i, _ip]),
+    "ddla_d_gp_batched_workspace_size": (_i, [_i, _llp]),
+    "ddla_d_gp_batched_forward": (_i, [_vp, _i, _i, _vp, _ll, _vp, _ll, _vp, _vp, _vp]),  # info: device array
+    "ddla_d_gp_batched_backward": (_i, [_vp, _i, _i, _vp, _vp, _vp, _ll, _vp, _ll]),
+    "ddla_c_vbatched_set_kernel": (_i, [_vp, _i, _i]),
     "ddla_c_profile": (_i, [_vp, _i]),
     "ddla_c_profile_read": (_i, [_vp, _i, _ip, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_double)]),

@@ -12,6 +12,9 @@ Backward passes (ddla_{s,d}_*_backward):
     cholesky_logdet: Lbar = 2 g diag(1 / L_ii)
 gp_negative_log_likelihood(K, y, h) is one Function whose backward is the
 closed form (K^-1 - alpha alpha^T) / 2 with K^-1 from ddla_{s,d}_potri;
+gp_negative_log_likelihood_batched(K, y, h) takes a stack of independent
+problems (K: batch x n x n, y: batch x n, FP64) and runs every launch over
+all of them (ddla_d_gp_batched_forward / _backward);
 gp_negative_log_likelihood_composed builds the same loss from the three
 Functions above; backward_concurrently runs independent problems' forward
 and backward passes on several handles at once.
@@ -237,6 +240,68 @@ def gp_negative_log_likelihood(K, y, h):
     """0.5 y^T K^-1 y + 0.5 log det K (the constant n/2 log 2 pi omitted),
     with the fused closed-form backward."""
     return GPNegativeLogLikelihood.apply(K, y, h)
+
+
+class GPNegativeLogLikelihoodBatched(torch.autograd.Function):
+    """loss[b] = y_b^T K_b^-1 y_b / 2 + log det K_b / 2 for a stack of
+    independent problems of one order (K: batch x n x n symmetric, y: batch x
+    n, float64 CUDA tensors), with each kernel launch serving every problem:
+    the blocked factorization (one panel launch and one trailing update per
+    step for all of them), L^-1 by merging inverted diagonal blocks, z = L^-1 y
+    and loss = z^T z / 2 + sum log L_ii.  Backward: dloss/dK_b = g_b (K_b^-1 -
+    a a^T) / 2 with K_b^-1 = L^-T L^-1 and a = K_b^-1 y_b (Rasmussen &
+    Williams 2006, eq. 5.9, the symmetric-input convention), dloss/dy_b = g_b a.
+    The device workspace made by forward is kept for backward."""
+
+    @staticmethod
+    def forward(ctx, K, y, h):
+        if K.dtype != torch.float64 or y.dtype != torch.float64:
+            raise TypeError("the batched GP likelihood is built for float64")
+        if not (K.is_cuda and y.is_cuda):
+            raise ValueError("the batched GP likelihood runs on the GPU: pass CUDA tensors")
+        if K.dim() != 3 or K.shape[1] != K.shape[2] or y.shape != K.shape[:2]:
+            raise ValueError("K must be batch x n x n and y batch x n")
+        batch, n = int(K.shape[0]), int(K.shape[1])
+        # K_b is symmetric, so torch's row-major K_b is the column-major one
+        Kc = K.detach().contiguous()
+        yc = y.detach().contiguous()
+        per = ctypes.c_longlong()
+        check_status(lib.ddla_d_gp_batched_workspace_size(n, ctypes.byref(per)), "ddla_d_gp_batched_workspace_size")
+        # device workspace, 256-byte aligned inside a torch allocation
+        raw = torch.empty(batch * per.value + 256, dtype=torch.uint8, device=K.device)
+        off = (-raw.data_ptr()) % 256
+        work = ctypes.c_void_p(raw.data_ptr() + off)
+        loss = torch.empty(batch, dtype=torch.float64, device=K.device)
+        info = torch.empty(batch, dtype=torch.int32, device=K.device)
+        _sync()
+        check_status(lib.ddla_d_gp_batched_forward(h.ptr, batch, n, _ptr(Kc), n * n, _ptr(yc), n, work, _ptr(loss),
+                                                   _ptr(info)), "ddla_d_gp_batched_forward")
+        bad = torch.nonzero(info).flatten().tolist()
+        if bad:
+            raise RuntimeError(f"gp likelihood: problem {bad[0]} is not positive definite "
+                               f"(leading minor of order {int(info[bad[0]])})")
+        ctx.h, ctx.raw, ctx.work, ctx.shape = h, raw, work, (batch, n)
+        return loss
+
+    @staticmethod
+    def backward(ctx, g):
+        batch, n = ctx.shape
+        gc = g.detach().to(torch.float64).contiguous()
+        Kbar = torch.empty((batch, n, n), dtype=torch.float64, device=gc.device)
+        ybar = torch.empty((batch, n), dtype=torch.float64, device=gc.device)
+        _sync()
+        check_status(lib.ddla_d_gp_batched_backward(ctx.h.ptr, batch, n, ctx.work, _ptr(gc), _ptr(Kbar), n * n,
+                                                    _ptr(ybar), n), "ddla_d_gp_batched_backward")
+        ctx.h.synchronize()
+        ctx.raw = None
+        return Kbar, ybar, None
+
+
+def gp_negative_log_likelihood_batched(K, y, h):
+    """Per-problem 0.5 y_b^T K_b^-1 y_b + 0.5 log det K_b (the constant n/2 log
+    2 pi omitted) for K of shape batch x n x n and y of shape batch x n, as a
+    tensor of batch losses; every kernel launch serves all problems."""
+    return GPNegativeLogLikelihoodBatched.apply(K, y, h)
 
 
 def backward_concurrently(problems, loss_fn, handles):

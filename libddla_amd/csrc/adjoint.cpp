@@ -66,13 +66,13 @@ double lower_product_flops(int n)
 }
 
 template <class T>
-void lower_product(hipStream_t s, int n, const T* Lz, const T* Bz, T* M)
+void lower_product(hipStream_t s, int n, const T* Lz, const T* Bz, T* M, const kern::Batch& batch = kern::Batch())
 {
     for (int q = 3, k0, k1; q >= 0; --q) {
         k_block(n, q, k0, k1);
         if (k1 <= k0) continue;
         kern::gemm<T>(s, 'T', 'N', k1, k1, k1 - k0, T(1), Lz + k0, n, Bz + k0, n, q == 3 ? T(0) : T(1), M, n,
-                      kern::kTriLower, 0);
+                      kern::kTriLower, 0, nullptr, false, batch);
     }
 }
 
@@ -130,6 +130,14 @@ void lower_inverse(hipStream_t s, int n, const T* L, long long ldl, const T* din
 }
 
 } // namespace
+
+// M = tril(X^T X) for lower-triangular X (order n, leading dimension n) of
+// each problem of the batch (the batched Gaussian-process likelihood).
+void lower_gram_batched(hipStream_t s, int n, const double* X, double* M, const kern::Batch& batch)
+{
+    lower_product<double>(s, n, X, X, M, batch);
+}
+double lower_gram_flops(int n) { return lower_product_flops(n); }
 
 template <class T>
 void potri(const DdlaHandle_t& h, char uplo, int n, const T* L, int ldl, T* Ainv, int lda)

@@ -562,11 +562,332 @@ __global__ void blocked_prep_kernel(BlockedArgs<T> a, int j, StepArrays<T> s)
     s.ld64[b] = kB;
 }
 
+// ---------------------------------------------------------------------------
+// Left-looking persistent kernel: one 256-thread workgroup per member, every
+// member in the same launch.  Block column j (64 columns) is formed once:
+//   C = A(j0:n, j) - L(j0:n, 0:j0) L(j, 0:j0)^T      (one MFMA product, the
+//       gemm_tile engine over 128 x 64 tiles, lower triangle of the diagonal
+//       block masked; the accumulators stay in registers across all of k),
+//   L(j, j), D_j = L(j, j)^-1 from the diagonal block (potf2_tile.h),
+//   L(j0 + 64:n, j) = C(j0 + 64:n, j) D_j^T            (in place, MFMA).
+// So each block column is written twice in all, where the right-looking
+// kernels read and write the whole trailing matrix at every step.  The
+// solve then runs as column sweeps (solve_member_ll) and the log-determinant
+// is the fixed-order sum of the blocks' butterflies.  Uplo 'U' runs the same
+// steps on the stored upper triangle (transposed products).
+// ---------------------------------------------------------------------------
+
+// LDS of the left-looking kernel, in elements of T: the larger of the
+// products' double-buffered slabs (128 x 64 tiles), the diagonal block's S,
+// X and rdiag, and the solve's kLLCols-column right-hand-side buffer.
+constexpr int kLLCols = 4;
+template <class T>
+constexpr int ll_lds_elems()
+{
+    constexpr int g = gemm_detail::lds_elems<128, 64, 16>();
+    constexpr int f = 2 * kB * kL + kB;
+    constexpr int s = kMaxBlocks * kB * kLLCols;
+    return g > f ? (g > s ? g : s) : (f > s ? f : s);
+}
+
+// The value of p (v) in the workgroup's first lane, held in scalar registers.
+template <class P>
+__device__ inline P* uniform_ptr(P* p)
+{
+    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return reinterpret_cast<P*>(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ inline long long uniform_ll(long long v)
+{
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// One product of the left-looking kernel, tile after tile: BN == 64 runs
+// down the rows (128 x 64 tiles), BM == 64 along the columns (64 x 128
+// tiles).  Out of line, with its arguments made uniform again: inlined into
+// the persistent loop, the address arithmetic of every product would be
+// hoisted out of that loop and stay live throughout (the kernel then spilled
+// about 400 registers).  OCC: the calling kernel's workgroups per CU, so that
+// each instantiation is reached from one kernel and compiled to its register
+// budget.
+template <class T, int OCC, int BM, int BN, bool TA, bool TB>
+__device__ __attribute__((noinline)) void ll_product(gemm_detail::GemmProblem<T> q, T* smem)
+{
+    gemm_detail::GemmProblem<T> p;
+    p.m = __builtin_amdgcn_readfirstlane(q.m);
+    p.n = __builtin_amdgcn_readfirstlane(q.n);
+    p.k = __builtin_amdgcn_readfirstlane(q.k);
+    p.alpha = q.alpha;
+    p.beta = q.beta;
+    p.A = uniform_ptr(q.A);
+    p.B = uniform_ptr(q.B);
+    p.C = uniform_ptr(q.C);
+    p.lda = uniform_ll(q.lda);
+    p.ldb = uniform_ll(q.ldb);
+    p.ldc = uniform_ll(q.ldc);
+    p.tri = __builtin_amdgcn_readfirstlane(q.tri);
+    p.diag = uniform_ll(q.diag);
+    p.vec_ab = __builtin_amdgcn_readfirstlane(q.vec_ab);
+    static_assert(BM == 64 || BN == 64, "one tile dimension spans the block column");
+    if (BN == 64) {
+        for (int m0 = 0; m0 < p.m; m0 += BM) gemm_detail::gemm_tile<T, BM, BN, 16, TA, TB>(p, m0, 0, smem);
+    } else {
+        for (int n0 = 0; n0 < p.n; n0 += BN) gemm_detail::gemm_tile<T, BM, BN, 16, TA, TB>(p, 0, n0, smem);
+    }
+}
+
+// Solve A X = B in place on B (n x nrhs, leading dimension ldb) from the
+// member's factor (L view v) and its diagonal-block inverses D_j at dinv +
+// j * 64 * 64 (column-major, zeros above the diagonal), NC right-hand-side
+// columns per pass held in LDS (y, n x NC).  Forward: for each block j,
+// y_j = D_j y_j, then y_i -= L(i, j) y_j for every row i below; backward: for
+// j from the last block, x_j = D_j^T y_j, then y_i -= L(j, i)^T x_j for every
+// row i above.  So each sweep takes one step per block, and every access to
+// L runs along a column (lower rows i: consecutive lanes read consecutive
+// rows) or along a row of L (each lane reads one contiguous piece).
+template <class T, int NC, int OCC>
+__device__ __attribute__((noinline)) void solve_member_ll(const T* A, View v, int n, int nb, T* Bm, long long ldb,
+                                                          int nrhs, const T* dinv, T* y)
+{
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    A = uniform_ptr(A);
+    Bm = uniform_ptr(Bm);
+    dinv = uniform_ptr(dinv);
+    v.lda = uniform_ll(v.lda);
+    v.upper = __builtin_amdgcn_readfirstlane(v.upper);
+    n = __builtin_amdgcn_readfirstlane(n);
+    nb = __builtin_amdgcn_readfirstlane(nb);
+    ldb = uniform_ll(ldb);
+    nrhs = __builtin_amdgcn_readfirstlane(nrhs);
+    const long long lda = v.lda;
+    for (int c0 = 0; c0 < nrhs; c0 += NC) {
+        const int nc = min(NC, nrhs - c0);
+        for (int e = tid; e < n * NC; e += kThreads) {
+            const int i = e / NC, c = e % NC;
+            y[e] = c < nc ? Bm[i + (long long)(c0 + c) * ldb] : T(0);
+        }
+        __syncthreads();
+        for (int j = 0; j < nb; ++j) {
+            const int j0 = kB * j, wj = min(kB, n - j0);
+            const T* D = dinv + (long long)j * kB * kB;
+            T z = T(0);
+            if (r < wj && q < nc) {
+#pragma unroll 8
+                for (int k = 0; k < wj; ++k) z = fma(D[r + kB * k], y[(j0 + k) * NC + q], z);
+            }
+            __syncthreads();
+            if (r < wj && q < nc) y[(j0 + r) * NC + q] = z;
+            __syncthreads();
+            for (int i = j0 + wj + tid; i < n; i += kThreads) {
+                T acc[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[c] = y[i * NC + c];
+                const T* li = v.upper ? A + j0 + (long long)i * lda : A + i + (long long)j0 * lda;
+                const long long ks = v.upper ? 1 : lda;
+#pragma unroll 16
+                for (int k = 0; k < wj; ++k) {
+                    const T l = li[k * ks];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) acc[c] = fma(-l, y[(j0 + k) * NC + c], acc[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) y[i * NC + c] = acc[c];
+            }
+            __syncthreads();
+        }
+        for (int j = nb - 1; j >= 0; --j) {
+            const int j0 = kB * j, wj = min(kB, n - j0);
+            const T* D = dinv + (long long)j * kB * kB;
+            T z = T(0);
+            if (r < wj && q < nc) {
+#pragma unroll 8
+                for (int k = 0; k < wj; ++k) z = fma(D[k + kB * r], y[(j0 + k) * NC + q], z);
+            }
+            __syncthreads();
+            if (r < wj && q < nc) y[(j0 + r) * NC + q] = z;
+            __syncthreads();
+            for (int i = tid; i < j0; i += kThreads) {
+                T acc[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[c] = y[i * NC + c];
+                // L(j0 + k, i)
+                const T* li = v.upper ? A + i + (long long)j0 * lda : A + j0 + (long long)i * lda;
+                const long long ks = v.upper ? lda : 1;
+#pragma unroll 16
+                for (int k = 0; k < wj; ++k) {
+                    const T l = li[k * ks];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) acc[c] = fma(-l, y[(j0 + k) * NC + c], acc[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) y[i * NC + c] = acc[c];
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < n * NC; e += kThreads) {
+            const int i = e / NC, c = e % NC;
+            if (c < nc) Bm[i + (long long)(c0 + c) * ldb] = y[e];
+        }
+        __syncthreads();
+    }
+}
+
+// Block column j of the member (L view v, order n): the left-looking product
+// C = A(j0:n, j) - L(j0:n, 0:j0) L(j, 0:j0)^T in place (j > 0), the diagonal
+// block (factor_block_column: L(j, j), D_j to dj, log sum to logpart), then
+// the panel L(j0 + 64:n, j) = C D_j^T in place.  Returns 0 or the failed
+// pivot (1-based).  smem: ll_lds_elems<T>() elements.
+template <class T, int OCC>
+__device__ inline int ll_block_column(T* A, View v, int n, int nb, int j, T* dj, T* smem, int* s_flag,
+                                      double& logpart)
+{
+    using gemm_detail::GemmProblem;
+    const long long lda = v.lda;
+    const int j0 = kB * j, wj = min(kB, n - j0), m = n - j0;
+    constexpr int VW = Mfma<T>::kVec;
+    const bool aligned = ((uintptr_t)A % 16 == 0) && lda % VW == 0;
+    if (j > 0) {
+        GemmProblem<T> p;
+        p.k = j0;
+        p.alpha = T(-1);
+        p.beta = T(1);
+        p.lda = p.ldb = p.ldc = lda;
+        p.diag = 0;
+        p.vec_ab = aligned;
+        if (!v.upper) {
+            // C_L(i, jj) -= sum_k L(j0 + i, k) L(j0 + jj, k), i >= jj
+            p.m = m;
+            p.n = wj;
+            p.A = A + j0;
+            p.B = A + j0;
+            p.C = A + j0 + (long long)j0 * lda;
+            p.tri = gemm_detail::kTriLowerD;
+            ll_product<T, OCC, 128, 64, false, true>(p, smem);
+        } else {
+            // stored U(j0 + jj, j0 + i) -= sum_k U(k, j0 + jj) U(k, j0 + i), jj <= i
+            p.m = wj;
+            p.n = m;
+            p.A = A + (long long)j0 * lda;
+            p.B = A + (long long)j0 * lda;
+            p.C = A + j0 + (long long)j0 * lda;
+            p.tri = gemm_detail::kTriUpperD;
+            ll_product<T, OCC, 64, 128, true, false>(p, smem);
+        }
+        __syncthreads();
+    }
+    T* S = smem;
+    T* X = smem + kB * kL;
+    T* rdiag = smem + 2 * kB * kL;
+    const int f = factor_block_column<T, (OCC > 1)>(A, v, n, nb, j, dj, S, X, rdiag, s_flag, logpart, false);
+    if (f != 0) return f;
+    __syncthreads();
+    if (m > kB) {
+        GemmProblem<T> p;
+        p.k = kB;
+        p.alpha = T(1);
+        p.beta = T(0);
+        p.diag = 0;
+        p.tri = gemm_detail::kTriNoneD;
+        p.vec_ab = aligned;
+        if (!v.upper) {
+            // L(j0 + 64 + i, j0 + jj) = sum_k C(j0 + 64 + i, j0 + k) D_j(jj, k)
+            p.m = m - kB;
+            p.n = kB;
+            p.A = A + j0 + kB + (long long)j0 * lda;
+            p.lda = lda;
+            p.B = dj;
+            p.ldb = kB;
+            p.C = A + j0 + kB + (long long)j0 * lda;
+            p.ldc = lda;
+            ll_product<T, OCC, 128, 64, false, true>(p, smem);
+        } else {
+            // stored U(j0 + jj, j0 + 64 + i) = sum_k D_j(jj, k) U(j0 + k, j0 + 64 + i)
+            p.m = kB;
+            p.n = m - kB;
+            p.A = dj;
+            p.lda = kB;
+            p.B = A + j0 + (long long)(j0 + kB) * lda;
+            p.ldb = lda;
+            p.C = A + j0 + (long long)(j0 + kB) * lda;
+            p.ldc = lda;
+            ll_product<T, OCC, 64, 128, false, false>(p, smem);
+        }
+        __syncthreads();
+    }
+    return 0;
+}
+
+// OCC = resident workgroups per CU the register allocation is sized for.
+template <class T, int OCC>
+__global__ __launch_bounds__(kThreads, OCC) void batched_ll_kernel(BatchArgs<T> a)
+{
+    __shared__ T smem[ll_lds_elems<T>()];
+    __shared__ int s_item, s_flag;
+    const int tid = threadIdx.x;
+    T* const slot = a.xinv + (long long)blockIdx.x * a.max_blocks * kB * kB;
+
+    for (;;) {
+        if (tid == 0) {
+            const int q = atomicAdd(a.queue, 1);
+            s_item = q < a.batch ? a.order[q] : -1;
+        }
+        __syncthreads();
+        // uniform over the workgroup: every per-member value below then lives
+        // in scalar registers (read back from LDS it would be per lane)
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item < 0) return;
+
+        const int n = __builtin_amdgcn_readfirstlane(a.n[item]);
+        const long long lda = __builtin_amdgcn_readfirstlane(a.lda[item]);
+        const long long ldb = a.nrhs > 0 ? __builtin_amdgcn_readfirstlane(a.ldb[item]) : 1;
+        const int nb = (n + kB - 1) / kB;
+        if (n < 0 || n > a.max_blocks * kB || lda < (n > 1 ? n : 1) || (a.nrhs > 0 && ldb < (n > 1 ? n : 1))) {
+            if (tid == 0) {
+                a.info[item] = -1;
+                a.logdet[item] = __builtin_nan("");
+            }
+            continue;
+        }
+        T* A = uniform_ptr(a.A[item]);
+        const View v{lda, a.upper};
+        int fail = 0;
+        double logsum = 0.0;  // meaningful in thread 0
+        for (int j = 0; j < nb; ++j) {
+            double l = 0.0;
+            fail = ll_block_column<T, OCC>(A, v, n, nb, j, slot + (long long)j * kB * kB, smem, &s_flag, l);
+            if (fail != 0) break;
+            logsum += l;
+        }
+        __syncthreads();
+        if (fail != 0) {
+            if (tid == 0) {
+                a.info[item] = fail;
+                a.logdet[item] = __builtin_nan("");
+            }
+            continue;
+        }
+        if (a.nrhs == 1) solve_member_ll<T, 1, OCC>(A, v, n, nb, uniform_ptr(a.B[item]), ldb, 1, slot, smem);
+        else if (a.nrhs > 1)
+            solve_member_ll<T, kLLCols, OCC>(A, v, n, nb, uniform_ptr(a.B[item]), ldb, a.nrhs, slot, smem);
+        if (tid == 0) {
+            a.info[item] = 0;
+            a.logdet[item] = 2.0 * logsum;
+        }
+    }
+}
+
 // Solves and log-determinants of the members factored on the blocked path.
 template <class T>
 __global__ __launch_bounds__(kThreads) void blocked_solve_kernel(BlockedArgs<T> a, T* const* B, double* logdet)
 {
-    __shared__ T part[kSlab * 4 * kB], V[kSlab * kB];
+    __shared__ T part[kSlab * 4 * kB];
+    static_assert(kSlab * 4 * kB >= kMaxBlocks * kB * kLLCols, "the solve's right-hand sides must fit");
     const int b = blockIdx.x;
     if (b >= *a.count) return;
     const int item = a.order[b];
@@ -576,9 +897,10 @@ __global__ __launch_bounds__(kThreads) void blocked_solve_kernel(BlockedArgs<T> 
         if (threadIdx.x == 0) logdet[item] = __builtin_nan("");
         return;
     }
-    if (a.nrhs > 0)
-        solve_member<T>(a.A[item], View{a.lda[item], a.upper}, n, nb, B[item], a.ldb[item], a.nrhs,
-                        a.dinv + (long long)b * a.max_blocks * kB * kB, part, V);
+    const View v{a.lda[item], a.upper};
+    const T* dinv = a.dinv + (long long)b * a.max_blocks * kB * kB;
+    if (a.nrhs == 1) solve_member_ll<T, 1, 1>(a.A[item], v, n, nb, B[item], a.ldb[item], 1, dinv, part);
+    else if (a.nrhs > 1) solve_member_ll<T, kLLCols, 1>(a.A[item], v, n, nb, B[item], a.ldb[item], a.nrhs, dinv, part);
     if (threadIdx.x == 0) {
         double sum = 0.0;
         for (int jj = 0; jj < nb; ++jj) sum += a.logparts[(long long)item * kMaxBlocks + jj];

@@ -200,6 +200,15 @@ int ddla_c_communication_available(void* handle, int communication)
 int ddla_c_set_wait_limit(int spins) { return ddla::kern::set_wait_limit(spins); }
 int ddla_c_set_panel_width(int cols) { return ddla::set_panel_width(cols); }
 
+int ddla_c_potrf_schedule(int n, int panel_width, int* steps, int capacity, int* count)
+{
+    return guarded([&]() -> int {
+        if (count == nullptr || capacity < 0 || (capacity > 0 && steps == nullptr)) return DDLA_C_STATUS_INVALID_VALUE;
+        *count = ddla::potrf_schedule_steps(n, panel_width, steps, capacity);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
 // One process: the communicator helpers describe a world of one rank.
 int ddla_c_default_comm() { return 0; }
 int ddla_c_comm_rank(int) { return 0; }
@@ -557,6 +566,35 @@ DDLA_C_ADJOINTS(s, float)
 DDLA_C_ADJOINTS(d, double)
 #undef DDLA_C_ADJOINTS
 
+int ddla_d_gp_batched_workspace_size(int n, long long* bytes_per_problem)
+{
+    return guarded([&]() -> int {
+        if (bytes_per_problem == nullptr) return DDLA_C_STATUS_INVALID_VALUE;
+        *bytes_per_problem = (long long)ddla::gpBatchedWorkspaceSize(n);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_d_gp_batched_forward(void* handle, int batch, int n, const void* K, long long k_stride, const void* y,
+                              long long y_stride, void* work, void* loss, int* info)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        ddla::gpBatchedForward(h, batch, n, static_cast<const double*>(K), k_stride, static_cast<const double*>(y),
+                               y_stride, work, static_cast<double*>(loss), info);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_d_gp_batched_backward(void* handle, int batch, int n, void* work, const void* g, void* Kbar,
+                               long long kbar_stride, void* ybar, long long ybar_stride)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        ddla::gpBatchedBackward(h, batch, n, work, static_cast<const double*>(g), static_cast<double*>(Kbar),
+                                kbar_stride, static_cast<double*>(ybar), ybar_stride);
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
 // ---- batched SPD solves --------------------------------------------------------
 #define DDLA_C_BATCHED(P, T)                                                                                   \
     int ddla_##P##_pposv_vbatched(void* handle, char uplo, const int* d_n, void* const* d_A, const int* d_lda, \
@@ -660,7 +698,28 @@ int ddla_c_vbatched_set_schedule(void* handle, int element_bytes, int occupancy,
             split_blocks > ddla::kern::kBatchedMaxBlocks)
             return DDLA_C_STATUS_INVALID_VALUE;
         h->batched_tuning[element_bytes == 8 ? 1 : 0] =
-            ddla::DdlaStream::BatchedTuning{true, occupancy, ordered != 0, split_blocks};
+            ddla::DdlaStream::BatchedTuning{true, occupancy, ordered != 0, split_blocks, false};
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_vbatched_kernel(void* handle, int element_bytes, int* left_looking)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if (element_bytes != 4 && element_bytes != 8) return DDLA_C_STATUS_INVALID_VALUE;
+        if (left_looking) *left_looking = h->batched_tuning[element_bytes == 8 ? 1 : 0].left_looking ? 1 : 0;
+        return DDLA_C_STATUS_SUCCESS;
+    });
+}
+
+int ddla_c_vbatched_set_kernel(void* handle, int element_bytes, int left_looking)
+{
+    return run(handle, [&](ddla::DdlaHandle_t h) -> int {
+        if ((element_bytes != 4 && element_bytes != 8) || (left_looking != 0 && left_looking != 1))
+            return DDLA_C_STATUS_INVALID_VALUE;
+        auto& t = h->batched_tuning[element_bytes == 8 ? 1 : 0];
+        if (!t.tuned) t = ddla::DdlaStream::BatchedTuning{true, 1, true, ddla::kern::kBatchedMaxBlocks, false};
+        t.left_looking = left_looking != 0;
         return DDLA_C_STATUS_SUCCESS;
     });
 }

@@ -67,24 +67,25 @@ struct LView {
 template <class T>
 void update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& v, int M, int N, int K, T alpha,
                   long long ai, long long ak, long long bj, long long bk, T beta, long long ci, long long cj,
-                  bool mask, const int* abort, double flops)
+                  bool mask, const int* abort, double flops, const kern::Batch& batch = kern::Batch())
 {
     if (M <= 0 || N <= 0) return;
-    LaunchTimer timer(h, cls, flops, 0, st);
+    LaunchTimer timer(h, cls, flops * batch.count, 0, st);
     const bool lookahead = cls == kClassLookahead;
     kern::gemm<T>(st, 'N', 'T', M, N, K, alpha, v.at(ai, ak), v.lda, v.at(bj, bk), v.lda, beta, v.at(ci, cj), v.lda,
-                  mask ? kTriLower : kTriNone, ci - cj, abort, lookahead);
+                  mask ? kTriLower : kTriNone, ci - cj, abort, lookahead, batch);
 }
 
 // Factors the panel of columns [k, k + kb) of the order-n L view (its
 // diagonal block and everything below it), assuming every earlier trailing
 // update has been applied to these columns: one persistent launch.
 template <class T>
-void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k, int kb, int* info, void* work)
+void factor_panel(DdlaStream* h, hipStream_t st, const LView<T>& v, int n, int k, int kb, int* info, void* work,
+                  const kern::Batch& batch = kern::Batch())
 {
     const double below = (double)n - k - kb;
-    LaunchTimer timer(h, kClassPanel, (double)kb * kb * kb / 3.0 + below * kb * kb, 0, st);
-    kern::panel_factor<T>(st, false, n, v.A, v.lda, k, kb, info, work);
+    LaunchTimer timer(h, kClassPanel, ((double)kb * kb * kb / 3.0 + below * kb * kb) * batch.count, 0, st);
+    kern::panel_factor<T>(st, false, n, v.A, v.lda, k, kb, info, work, batch);
 }
 
 // Split-product trailing updates (FP32 factorization only): after panel p is
@@ -116,6 +117,69 @@ void split_update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& 
     }
 }
 
+} // namespace
+
+namespace sched {
+
+// The stream schedule of potrf_blocked for order n and panel width NB: panel
+// p holds columns [p NB, (p + 1) NB) (clipped to n).  After panel 0 is
+// factored, step p (panel p factored) applies panel p to the columns of
+// panel p + 1 on the look stream, which the side stream then factors; to
+// panel p + 2's columns on the look stream, after step p - 1's update of the
+// rest (which also wrote them); and to every column from panel p + 3 on the
+// main stream.  Three events order the streams; the schedule ends with all
+// work ordered on the main stream.
+std::vector<Step> potrf_schedule(int n, int NB)
+{
+    std::vector<Step> s;
+    if (n <= 0 || NB <= 0) return s;
+    const int np = (n + NB - 1) / NB;
+    auto col = [&](int q) { return std::min<long long>(n, (long long)q * NB); };
+    auto launch = [&](int op, int stream, int a, int b = 0, int c = 0) { s.push_back(Step{op, stream, a, b, c}); };
+    auto update = [&](int stream, int p, long long lo, long long hi) {
+        launch(kUpdate, stream, p, (int)lo, (int)hi);
+    };
+    launch(kReset, kMain, 0);
+    launch(kFactor, kMain, 0);
+    launch(kRecord, kMain, kEvPanel);
+    for (int p = 0; p + 1 < np; ++p) {
+        launch(kWait, kLook, kEvPanel);
+        launch(kWait, kMain, kEvPanel);
+        update(kLook, p, col(p + 1), col(p + 2));
+        launch(kRecord, kLook, kEvLook);
+        launch(kWait, kSide, kEvLook);
+        launch(kFactor, kSide, p + 1);
+        launch(kRecord, kSide, kEvPanel);
+        if (p + 2 < np) {
+            if (p > 0) launch(kWait, kLook, kEvRest);
+            update(kLook, p, col(p + 2), col(p + 3));
+        }
+        update(kMain, p, col(p + 3), n);
+        launch(kRecord, kMain, kEvRest);
+    }
+    launch(kRecord, kLook, kEvLook);
+    launch(kWait, kMain, kEvLook);
+    launch(kWait, kMain, kEvPanel);
+    return s;
+}
+
+} // namespace sched
+
+int potrf_schedule_steps(int n, int panel_width, int* steps, int capacity)
+{
+    const int NB = panel_width > 0 ? panel_width : outer_block<double>(n, false);
+    const std::vector<sched::Step> s = sched::potrf_schedule(n, NB);
+    const int count = (int)s.size();
+    for (int i = 0; i < count && i < capacity; ++i) {
+        const sched::Step& t = s[i];
+        const int v[5] = {t.op, t.stream, t.a, t.b, t.c};
+        std::copy(v, v + 5, steps + 5 * i);
+    }
+    return count;
+}
+
+namespace {
+
 // Blocked right-looking Cholesky of the order-n L view with a look-ahead of
 // two panels; *info (device) must be zero on entry and receives the 1-based
 // index of the first failed pivot; work holds kern::panel_work_bytes<T>(n)
@@ -130,11 +194,16 @@ void split_update_lower(DdlaStream* h, hipStream_t st, int cls, const LView<T>& 
 // panels are updated and factored beside them; concurrent launches write
 // disjoint columns and read only finished panels.  Returns with all work
 // ordered on the handle's stream.  split (FP32 only, optional): the trailing
-// updates run as split products.
+// updates run as split products.  batch (uplo L, no split): batch.count
+// independent problems of order n at a common stride (A, info and work of
+// problem b sit b * batch.stride bytes after problem 0's), factored in step
+// with each launch serving all of them.
 template <class T>
 void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* info, void* work,
-                   const SplitPlan* split = nullptr)
+                   const SplitPlan* split = nullptr, const kern::Batch& batch = kern::Batch())
 {
+    if (batch.count != 1 && (upper || split != nullptr || batch.inner != 1))
+        throw std::logic_error("potrf_blocked: a batch of problems is factored in the lower view, without splits");
     if (upper) {
         // The factorization's kernels are tuned for the lower view; run in
         // the upper view (transposed operand access throughout) the same
@@ -161,7 +230,7 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
     // factors panel q and splits its rows below the diagonal block into buf[q % 3]
     auto factor = [&](hipStream_t st, int q) {
         const int k = col(q), kb = col(q + 1) - k;
-        factor_panel<T>(h, st, v, n, k, kb, info, work);
+        factor_panel<T>(h, st, v, n, k, kb, info, work, batch);
         const long long r = (long long)k + kb;
         if (!split || r >= n) return;
         if constexpr (std::is_same<T, float>::value)
@@ -180,30 +249,21 @@ void potrf_blocked(DdlaStream* h, bool upper, int n, T* A, long long lda, int* i
             split_update_lower<T>(h, st, cls, v, *split, split->buf[p % 3], n - (int)r, M, N, kb, r, c_lo - (int)r,
                                   c_lo - (int)r, info, flops);
         else
-            update_lower<T>(h, st, cls, v, M, N, kb, T(-1), c_lo, k, c_lo, k, T(1), c_lo, c_lo, true, info, flops);
+            update_lower<T>(h, st, cls, v, M, N, kb, T(-1), c_lo, k, c_lo, k, T(1), c_lo, c_lo, true, info, flops,
+                            batch);
     };
-    kern::panel_reset<T>(main, n, work);
-    factor(main, 0);
-    DDLA_HIP(hipEventRecord(h->ev_panel, main));
-    for (int p = 0; p + 1 < np; ++p) {
-        // panel p is factored
-        DDLA_HIP(hipStreamWaitEvent(look, h->ev_panel, 0));
-        DDLA_HIP(hipStreamWaitEvent(main, h->ev_panel, 0));
-        update(look, p, col(p + 1), col(p + 2));
-        DDLA_HIP(hipEventRecord(h->ev_look, look));
-        DDLA_HIP(hipStreamWaitEvent(side, h->ev_look, 0));
-        factor(side, p + 1);
-        DDLA_HIP(hipEventRecord(h->ev_panel, side));
-        if (p + 2 < np) {
-            if (p > 0) DDLA_HIP(hipStreamWaitEvent(look, h->ev_rest, 0));
-            update(look, p, col(p + 2), col(p + 3));
+    (void)np;
+    const hipStream_t streams[3] = {main, side, look};
+    const hipEvent_t events[3] = {h->ev_panel, h->ev_look, h->ev_rest};
+    for (const sched::Step& st : sched::potrf_schedule(n, NB)) {
+        switch (st.op) {
+        case sched::kReset: kern::panel_reset<T>(streams[st.stream], n, work, batch); break;
+        case sched::kFactor: factor(streams[st.stream], st.a); break;
+        case sched::kUpdate: update(streams[st.stream], st.a, st.b, st.c); break;
+        case sched::kRecord: DDLA_HIP(hipEventRecord(events[st.a], streams[st.stream])); break;
+        case sched::kWait: DDLA_HIP(hipStreamWaitEvent(streams[st.stream], events[st.a], 0)); break;
         }
-        update(main, p, col(p + 3), n);
-        DDLA_HIP(hipEventRecord(h->ev_rest, main));
     }
-    DDLA_HIP(hipEventRecord(h->ev_look, look));
-    DDLA_HIP(hipStreamWaitEvent(main, h->ev_look, 0));
-    DDLA_HIP(hipStreamWaitEvent(main, h->ev_panel, 0));
 }
 
 // Left solve A X = B with A = L L^T given by its L view; B is n x nrhs: the
@@ -820,6 +880,13 @@ void potrf_blocked_dispatch(DdlaStream* h, bool upper, int n, T* A, long long ld
 }
 template void potrf_blocked_dispatch<float>(DdlaStream*, bool, int, float*, long long, int*, void*);
 template void potrf_blocked_dispatch<double>(DdlaStream*, bool, int, double*, long long, int*, void*);
+
+template <class T>
+void potrf_batched_dispatch(DdlaStream* h, int n, T* A, long long lda, int* info, void* work, const kern::Batch& batch)
+{
+    potrf_blocked<T>(h, false, n, A, lda, info, work, nullptr, batch);
+}
+template void potrf_batched_dispatch<double>(DdlaStream*, int, double*, long long, int*, void*, const kern::Batch&);
 
 void potrf_split_dispatch(DdlaStream* h, bool upper, int n, float* A, long long lda, int* info, void* work,
                           int mode, void* const* bufs)

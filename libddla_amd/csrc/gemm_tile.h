@@ -65,7 +65,26 @@ struct GemmProblem {
     int tri;
     long long diag;
     int vec_ab;  // A, B and their leading dimensions allow 16-byte loads along the contiguous dimension
+    // launches over independent problems (grid z = problem * inner + sub-block):
+    // problem b moves every pointer by b * bstride bytes, sub-block i by
+    // i * istride bytes more (the abort word only by the problem's offset)
+    int inner = 1;
+    long long bstride = 0, istride = 0;
 };
+
+// Moves p's operands, result and the abort word to grid z's problem and
+// sub-block (nothing for a single-problem launch).
+template <class T>
+__device__ inline void batch_offset(GemmProblem<T>& p, const int*& abort)
+{
+    if (gridDim.z == 1) return;
+    const long long z = blockIdx.z, prob = z / p.inner, sub = z - prob * p.inner;
+    const long long off = prob * p.bstride + sub * p.istride;
+    p.A = reinterpret_cast<const T*>(reinterpret_cast<const char*>(p.A) + off);
+    p.B = reinterpret_cast<const T*>(reinterpret_cast<const char*>(p.B) + off);
+    p.C = reinterpret_cast<T*>(reinterpret_cast<char*>(p.C) + off);
+    if (abort != nullptr) abort = reinterpret_cast<const int*>(reinterpret_cast<const char*>(abort) + prob * p.bstride);
+}
 
 // Tile schedule of one launch (see map_tile).
 struct TileOrder {
@@ -406,6 +425,7 @@ __device__ __forceinline__ void gemm_tile(const GemmProblem<T>& p, int m0, int n
 template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
 __global__ __launch_bounds__(256, WAVES_PER_EU) void gemm_kernel(GemmProblem<T> p, TileOrder o, const int* abort)
 {
+    batch_offset(p, abort);
     if (abort != nullptr && *abort != 0) return;
     int ti, tj;
     if (!map_tile(o, blockIdx.x, ti, tj)) return;
@@ -420,6 +440,7 @@ template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU, i
 __global__ __launch_bounds__(256, WAVES_PER_EU) void syrk_update_kernel(GemmProblem<T> p, TileOrder o,
                                                                          const int* abort)
 {
+    batch_offset(p, abort);
     if (abort != nullptr && *abort != 0) return;
     int ti, tj;
     if (!map_tile(o, blockIdx.x, ti, tj)) return;
@@ -434,6 +455,7 @@ template <class T, int BM, int BN, int BK, bool TA, bool TB, int WAVES_PER_EU>
 __global__ __launch_bounds__(256, WAVES_PER_EU) void panel_update_kernel(GemmProblem<T> p, TileOrder o,
                                                                           const int* abort)
 {
+    batch_offset(p, abort);
     if (abort != nullptr && *abort != 0) return;
     int ti, tj;
     if (!map_tile(o, blockIdx.x, ti, tj)) return;

@@ -57,6 +57,7 @@ public:
         int occupancy = 1;
         bool ordered = true;
         int split_blocks = 16;  // members of more blocks take the blocked path (16: none)
+        bool left_looking = false;  // every member on the left-looking persistent kernel
     };
     BatchedTuning batched_tuning[2];  // [0] float, [1] double
 
@@ -124,6 +125,25 @@ void fill_int(hipStream_t s, int* p, int value);
 // 64 within [64, 768] (the split-product path caps it at 512).  Returns the
 // previous override (0 = none).
 int set_panel_width(int cols);
+
+// The stream schedule of the blocked factorization: the host issues these
+// steps in order.  op kReset / kFactor (a = panel) / kUpdate (panel a applied
+// to columns [b, c), rows from b) launch on `stream` (kMain = the handle's
+// stream, kSide, kLook); kRecord / kWait record event a on, or make `stream`
+// wait for, one of the three events.
+namespace sched {
+enum Op { kReset = 0, kFactor = 1, kUpdate = 2, kRecord = 3, kWait = 4 };
+enum Stream { kMain = 0, kSide = 1, kLook = 2 };
+enum Event { kEvPanel = 0, kEvLook = 1, kEvRest = 2 };
+struct Step {
+    int op, stream, a, b, c;
+};
+std::vector<Step> potrf_schedule(int n, int NB);
+} // namespace sched
+// Writes min(count, capacity) steps of the schedule for order n and panel
+// width panel_width (<= 0: the FP64 rule) as (op, stream, a, b, c) ints;
+// returns the step count.
+int potrf_schedule_steps(int n, int panel_width, int* steps, int capacity);
 
 template <class T> struct is_complex : std::false_type {};
 template <class R> struct is_complex<std::complex<R>> : std::true_type {};

@@ -25,6 +25,19 @@ int set_wait_limit(int spins);  // returns the previous limit; spins <= 0 restor
 //   kTriNone, or kTriLower and i + diag >= j, or kTriUpper and i + diag <= j.
 enum Tri { kTriNone = 0, kTriLower = 1, kTriUpper = 2 };
 
+// Independent problems in one launch.  Problem b's buffers sit b * stride
+// bytes after problem 0's: every pointer of the call (operands, result, info,
+// work) moves by the same amount, so the caller lays out each problem's data
+// at one common stride.  A product may also repeat over `inner` equally
+// spaced sub-blocks of each problem, inner_stride bytes apart (operands and
+// result alike).  The default is one problem.
+struct Batch {
+    int count = 1;
+    long long stride = 0;
+    int inner = 1;
+    long long inner_stride = 0;
+};
+
 // C := alpha * op(A) * op(B) + beta * C, op(X) = X or X^T ('N' / 'T' / 'C').
 // alpha == 0 or k == 0 only scales C; beta == 0 never reads C.  If abort is
 // non-null and *abort != 0 when a workgroup starts, the workgroup does nothing
@@ -36,7 +49,7 @@ enum Tri { kTriNone = 0, kTriLower = 1, kTriUpper = 2 };
 template <class T>
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, long long lda,
           const T* B, long long ldb, T beta, T* C, long long ldc, Tri tri = kTriNone, long long diag = 0,
-          const int* abort = nullptr, bool lookahead = false);
+          const int* abort = nullptr, bool lookahead = false, const Batch& batch = Batch());
 
 // Variable-size batched GEMM over device arrays of sizes, leading dimensions
 // and pointers; max_m / max_n size the grid; tri masks every C (diagonal
@@ -67,11 +80,12 @@ void vbatched_validate(hipStream_t s, int na, int nb, const int* m, const int* n
 template <class T>
 size_t panel_work_bytes(int n);
 template <class T>
-void panel_reset(hipStream_t s, int n, void* work);
+void panel_reset(hipStream_t s, int n, void* work, const Batch& batch = Batch());
 template <class T>
 int* panel_error_word(int n, void* work);
 template <class T>
-void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, int kb, int* info, void* work);
+void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, int kb, int* info, void* work,
+                  const Batch& batch = Batch());
 
 // Split-product trailing update of the FP32 factorization (L view as for
 // panel_factor); mode = 1 (bf16x3: two bfloat16 planes, three products), 2
@@ -97,7 +111,8 @@ void split_update(hipStream_t s, int mode, bool upper, int m, int n2, int kb, co
 // panel_factor) into inv[b * 4096 ...], identity-padded for a ragged last
 // block; unit: the diagonal is taken as ones (the stored one is not used).
 template <class T>
-void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv, bool unit = false);
+void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv, bool unit = false,
+                       const Batch& batch = Batch());
 // M := alpha M (m x n, leading dimension ld); alpha == 0 writes zeros
 // without reading M.
 template <class T>
@@ -144,13 +159,15 @@ void stage_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
 // of them in one gemm_vbatched launch per step, then their solves -- and the
 // persistent launch takes the rest.  work holds batched_work_bytes(sizeof(T),
 // batch, grid, max_blocks, blocked) bytes, blocked = whether split_blocks may be
-// used.
+// used.  left_looking: every member on the left-looking persistent kernel
+// instead (batched_ll_kernel: block columns formed once by one MFMA product
+// each; split_blocks is then ignored).
 constexpr int kBatchedMaxBlocks = 16;
 size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks, bool blocked);
 template <class T>
 void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
                    T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
-                   int occupancy, int split_blocks, void* work);
+                   int occupancy, int split_blocks, void* work, bool left_looking = false);
 // Batched copy of matrix i (rows n[i], columns cols or n[i] when cols < 0)
 // from src[i] (leading dimension lds[i]) to dst[i] (leading dimension ldd[i]).
 template <class T>
@@ -201,6 +218,25 @@ void diag_adjoint(hipStream_t s, int n, const T* L, long long ldl, double g, T* 
 // diag_logdet reduction over A[i * (lda + 1)], i < n: work must hold
 // logdet_work_bytes(); out (device) receives {sum log|d|, flags} where flags
 // = 1 nonfinite | 2 zero | 4 odd number of negative entries.
+// Batched Gaussian-process likelihood kernels (gp_kernels.hip).  Problem b's working
+// region sits b * stride bytes after problem 0's (pointers below are problem
+// 0's); the caller's K, y, Kbar, ybar, loss, info and g have their own
+// strides (elements) or one entry per problem.
+//   gp_load:       L_b = K_b (n x n, leading dimension n), y copied, info_b = 0;
+//   gp_place_diag: X_b = the block-diagonal matrix of the inverted 64 x 64
+//                  diagonal blocks dinv_b, zero elsewhere;
+//   gp_loss:       loss[b] = z_b^T z_b / 2 + sum_i log L_b(i, i) (also kept at
+//                  loss_slot), info[b] = info_b;
+//   gp_grad:       Kbar_b(i, j) = g[b] (M_b(max, min) - a_i a_j) / 2 from the
+//                  lower triangle M_b, ybar_b = g[b] a_b (when ybar is not null).
+void gp_load(hipStream_t s, int batch, int n, const double* K, long long k_stride, const double* y, long long y_stride,
+             double* L, double* ybuf, int* info, long long stride);
+void gp_place_diag(hipStream_t s, int batch, int n, const double* dinv, double* X, long long stride);
+void gp_loss(hipStream_t s, int batch, int n, const double* L, const double* z, const int* info, double* loss_slot,
+             long long stride, double* loss_out, int* info_out);
+void gp_grad(hipStream_t s, int batch, int n, const double* M, const double* a, long long stride, const double* g,
+             double* Kbar, long long kbar_stride, double* ybar, long long ybar_stride);
+
 struct LogdetResult { double magnitude; int flags; int pad; };
 size_t logdet_work_bytes();
 template <class T>

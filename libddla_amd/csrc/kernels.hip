@@ -78,10 +78,15 @@ constexpr int kIB = 64;
 // inv(block b) = L(b, b)^-1 for every 64 x 64 diagonal block of a factor
 // (unit: the diagonal taken as ones, the stored one never used).
 template <class T>
-__global__ __launch_bounds__(256) void trtri_kernel(bool upper, bool unit, int n, const T* A, long long lda, T* inv)
+__global__ __launch_bounds__(256) void trtri_kernel(bool upper, bool unit, int n, const T* A, long long lda, T* inv,
+                                                    long long bstride)
 {
     using namespace potf2_detail;
     __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
+    if (gridDim.y > 1) {  // problem blockIdx.y of a batch: A and inv bstride bytes apart
+        A = reinterpret_cast<const T*>(reinterpret_cast<const char*>(A) + blockIdx.y * bstride);
+        inv = reinterpret_cast<T*>(reinterpret_cast<char*>(inv) + blockIdx.y * bstride);
+    }
     const long long j0 = (long long)blockIdx.x * kIB;
     const int w = (int)min((long long)kIB, n - j0);
     load_block<T>(upper, w, A + j0 + j0 * lda, lda, S);
@@ -441,7 +446,8 @@ bool is_n(char op) { return op == 'N' || op == 'n'; }
 
 // Builds the tile schedule of one launch (see map_tile) and launches it.
 template <class T, int BM, int BN, int BK, int WPE>
-void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, const int* abort, bool lookahead)
+void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, const int* abort, bool lookahead,
+                     int zcount)
 {
     TileOrder o;
     o.ntm = (p.m + BM - 1) / BM;
@@ -452,7 +458,7 @@ void launch_gemm_cfg(hipStream_t s, bool ta, bool tb, const GemmProblem<T>& p, c
     o.total = (int)total;
     o.group = 8;
     o.per_xcd = (o.total + 7) / 8;
-    const dim3 grid((unsigned)(o.per_xcd * 8)), block(256);
+    const dim3 grid((unsigned)(o.per_xcd * 8), 1, (unsigned)zcount), block(256);
     if (p.tri != kTriNoneD && !lookahead) {
         // the trailing updates: C (N T) or its mirror (T N)
         if (!ta && tb) syrk_update_kernel<T, BM, BN, BK, false, true, WPE><<<grid, block, 0, s>>>(p, o, abort);
@@ -495,21 +501,27 @@ void launch_vbatched_cfg(hipStream_t s, bool ta, bool tb, const VBatch<T>& v, in
 
 template <class T>
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, long long lda, const T* B,
-          long long ldb, T beta, T* C, long long ldc, Tri tri, long long diag, const int* abort, bool lookahead)
+          long long ldb, T beta, T* C, long long ldc, Tri tri, long long diag, const int* abort, bool lookahead,
+          const Batch& batch)
 {
-    if (m <= 0 || n <= 0) return;
+    if (m <= 0 || n <= 0 || batch.count <= 0 || batch.inner <= 0) return;
     constexpr int VW = 16 / sizeof(T);
-    const int vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % VW == 0 && ldb % VW == 0;
-    GemmProblem<T> p{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, (int)tri, diag, vec};
+    // 16-byte loads need every problem's and sub-block's operands aligned too
+    const int vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % VW == 0 && ldb % VW == 0 &&
+                    (batch.count == 1 || batch.stride % 16 == 0) && (batch.inner == 1 || batch.inner_stride % 16 == 0);
+    const long long zcount = (long long)batch.count * batch.inner;
+    if (zcount > 65535) throw std::invalid_argument("gemm: more than 65535 problems x sub-blocks in one launch");
+    GemmProblem<T> p{m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, (int)tri, diag, vec, batch.inner, batch.stride,
+                     batch.inner_stride};
     const bool a_t = !is_n(ta), b_t = !is_n(tb);
     // 128 x 128 tiles unless there are fewer than 512 of them (two per CU)
     // and 64 x 64 tiles give at least twice as many
     const long long t128 = (long long)((m + 127) / 128) * ((n + 127) / 128);
     const long long t64 = (long long)((m + 63) / 64) * ((n + 63) / 64);
     if (m >= 128 && n >= 128 && (t128 >= 512 || t64 < 2 * t128))
-        launch_gemm_cfg<T, 128, 128, 16, 2>(s, a_t, b_t, p, abort, lookahead);
+        launch_gemm_cfg<T, 128, 128, 16, 2>(s, a_t, b_t, p, abort, lookahead, (int)zcount);
     else
-        launch_gemm_cfg<T, 64, 64, 16, 2>(s, a_t, b_t, p, abort, lookahead);
+        launch_gemm_cfg<T, 64, 64, 16, 2>(s, a_t, b_t, p, abort, lookahead, (int)zcount);
 }
 
 template <class T>
@@ -651,34 +663,53 @@ int* panel_error_word(int n, void* work)
     return panel_sync<T>(work) + (size_t)((n + kB - 1) / kB) * kSyncInts;
 }
 
+namespace {
+// p[0, count) := 0 for each problem y (problem y's words stride bytes after problem 0's)
+__global__ void zero_ints_batched_kernel(int* p, int count, long long stride)
+{
+    int* q = reinterpret_cast<int*>(reinterpret_cast<char*>(p) + blockIdx.y * stride);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) q[i] = 0;
+}
+} // namespace
+
 template <class T>
-void panel_reset(hipStream_t s, int n, void* work)
+void panel_reset(hipStream_t s, int n, void* work, const Batch& batch)
 {
     using namespace panel_detail;
     const int count = ((n + kB - 1) / kB) * kSyncInts + 1;
-    solve_detail::zero_ints_kernel<<<grid_for(count, 256), 256, 0, s>>>(panel_sync<T>(work), count);
+    if (batch.count == 1) {
+        solve_detail::zero_ints_kernel<<<grid_for(count, 256), 256, 0, s>>>(panel_sync<T>(work), count);
+    } else if (batch.count > 1) {
+        const dim3 grid((unsigned)grid_for(count, 256), (unsigned)batch.count);
+        zero_ints_batched_kernel<<<grid, 256, 0, s>>>(panel_sync<T>(work), count, batch.stride);
+    }
     check_launch("panel_reset");
 }
 
 template <class T>
-void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, int kb, int* info, void* work)
+void panel_factor(hipStream_t s, bool upper, int n, T* A, long long lda, int k, int kb, int* info, void* work,
+                  const Batch& batch)
 {
     using namespace panel_detail;
     if (kb <= 0 || k < 0 || k + kb > n) return;
     if (kb > kMaxCols * kB) throw std::invalid_argument("panel_factor: panel wider than 768 columns");
     if (k % kB != 0) throw std::invalid_argument("panel_factor: panel must start on a 64-column boundary");
     PanelArgs<T> a{A, lda, upper ? 1 : 0, n, k, kb, info, static_cast<T*>(work),
-                   panel_sync<T>(work) + (size_t)(k / kB) * kSyncInts, panel_error_word<T>(n, work), wait_limit()};
+                   panel_sync<T>(work) + (size_t)(k / kB) * kSyncInts, panel_error_word<T>(n, work), wait_limit(),
+                   batch.stride};
     const int rows = (n - k + kB - 1) / kB;
-    panel_kernel<T><<<rows, 256, 0, s>>>(a);
+    if (batch.count <= 0) return;
+    panel_kernel<T><<<dim3((unsigned)rows, (unsigned)batch.count), 256, 0, s>>>(a);
     check_launch("panel_factor");
 }
 
 template <class T>
-void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv, bool unit)
+void trtri_diag_blocks(hipStream_t s, bool upper, int n, const T* A, long long lda, T* inv, bool unit,
+                       const Batch& batch)
 {
-    if (n <= 0) return;
-    trtri_kernel<T><<<(n + kIB - 1) / kIB, 256, 0, s>>>(upper, unit, n, A, lda, inv);
+    if (n <= 0 || batch.count <= 0) return;
+    const dim3 grid((unsigned)((n + kIB - 1) / kIB), (unsigned)batch.count);
+    trtri_kernel<T><<<grid, 256, 0, s>>>(upper, unit, n, A, lda, inv, batch.stride);
     check_launch("trtri_diag_blocks");
 }
 
@@ -733,7 +764,7 @@ __global__ __launch_bounds__(256) void batched_copy_kernel(int batch, const int*
 template <class T>
 void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
                    T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
-                   int occupancy, int split_blocks, void* work)
+                   int occupancy, int split_blocks, void* work, bool left_looking)
 {
     using namespace batched_detail;
     if (batch <= 0) return;
@@ -751,6 +782,16 @@ void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n,
     }
     check_launch("batched_pposv");
     // blocked path for the members of more than split_blocks blocks (needs the sorted order)
+    if (left_looking) {
+        solve_detail::zero_ints_kernel<<<1, 64, 0, s>>>(queue, 1);
+        check_launch("batched_pposv");
+        BatchArgs<T> a{batch, nrhs, upper ? 1 : 0, max_blocks, n, A, lda, B, ldb, info, logdet, order, queue, xinv,
+                       nullptr};
+        if (occupancy == 2) batched_ll_kernel<T, 2><<<grid, kThreads, 0, s>>>(a);
+        else batched_ll_kernel<T, 1><<<grid, kThreads, 0, s>>>(a);
+        check_launch("batched_pposv");
+        return;
+    }
     const int* first = nullptr;
     if (ordered && split_blocks >= 1 && split_blocks < max_blocks) {
         char* bw = w + fused_bytes(sizeof(T), batch, grid, max_blocks);
@@ -1297,19 +1338,19 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
 
 #define DDLA_KERN_INSTANTIATE(T)                                                                              \
     template void gemm<T>(hipStream_t, char, char, int, int, int, T, const T*, long long, const T*, long long, \
-                          T, T*, long long, Tri, long long, const int*, bool);                                \
+                          T, T*, long long, Tri, long long, const int*, bool, const Batch&);                  \
     template void gemm_vbatched<T>(hipStream_t, char, char, const int*, const int*, const int*, T,            \
                                    const T* const*, const int*, const T* const*, const int*, T, T* const*,    \
                                    const int*, int, int, int, Tri);                                           \
     template size_t panel_work_bytes<T>(int);                                                                 \
-    template void panel_reset<T>(hipStream_t, int, void*);                                                    \
+    template void panel_reset<T>(hipStream_t, int, void*, const Batch&);                                      \
     template int* panel_error_word<T>(int, void*);                                                            \
-    template void panel_factor<T>(hipStream_t, bool, int, T*, long long, int, int, int*, void*);              \
+    template void panel_factor<T>(hipStream_t, bool, int, T*, long long, int, int, int*, void*, const Batch&); \
     template void batched_pposv<T>(hipStream_t, bool, int, int, const int*, T* const*, const int*, T* const*,  \
-                                   const int*, int*, double*, int, int, bool, int, int, void*);                \
+                                   const int*, int*, double*, int, int, bool, int, int, void*, bool);          \
     template void batched_copy<T>(hipStream_t, int, const int*, int, T* const*, const int*, T* const*,         \
                                   const int*, int);                                                            \
-    template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*, bool);                \
+    template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*, bool, const Batch&);  \
     template void scale_matrix<T>(hipStream_t, int, int, T, T*, long long);                                   \
     template void diag_logdet<T>(hipStream_t, int, const T*, long long, void*, LogdetResult*);                \
     template void potrs_sweeps<T>(hipStream_t, bool, int, int, const T*, long long, const T*, T*, long long,   \

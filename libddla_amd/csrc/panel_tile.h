@@ -56,6 +56,8 @@ struct PanelArgs {
     int* sync;       // this panel's kSyncInts counters, zero on entry
     int* error;      // set to 1 when a wait exceeds spin_limit
     int spin_limit;
+    long long bstride;  // grid y = independent problems: problem b's A, info, inv, sync and error sit
+                        // b * bstride bytes after problem 0's
 };
 
 template <class T, bool COHERENT>
@@ -181,6 +183,14 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> a)
     __shared__ T S[kB * kL], X[kB * kL], rdiag[kB];
     __shared__ int s_ticket, s_flag;
     const int tid = threadIdx.x;
+    if (gridDim.y > 1) {
+        const long long off = (long long)blockIdx.y * a.bstride;
+        a.A = reinterpret_cast<T*>(reinterpret_cast<char*>(a.A) + off);
+        a.info = reinterpret_cast<int*>(reinterpret_cast<char*>(a.info) + off);
+        a.inv = reinterpret_cast<T*>(reinterpret_cast<char*>(a.inv) + off);
+        a.sync = reinterpret_cast<int*>(reinterpret_cast<char*>(a.sync) + off);
+        a.error = reinterpret_cast<int*>(reinterpret_cast<char*>(a.error) + off);
+    }
     int* const progress = a.sync + 1;
     int* const failed = a.sync + 1 + kMaxCols;
 

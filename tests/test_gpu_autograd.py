@@ -212,3 +212,106 @@ def test_autograd_rejects_host_tensors_and_non_pd(handle):
     a = torch.tensor([[1.0, 2.0], [2.0, 1.0]], dtype=torch.float64, device=DEV)
     with pytest.raises(RuntimeError):
         ag.cholesky(a, handle)
+
+
+def gp_closed_form(k, y):
+    """loss, dloss/dK, dloss/dy of 0.5 y^T K^-1 y + 0.5 log det K in FP64 on the CPU."""
+    kinv = np.linalg.inv(k)
+    alpha = kinv @ y
+    sign, logdet = np.linalg.slogdet(k)
+    assert sign > 0
+    return 0.5 * y @ alpha + 0.5 * logdet, 0.5 * (kinv - np.outer(alpha, alpha)), alpha
+
+
+@pytest.mark.parametrize("n,batch", [(1, 2), (64, 3), (100, 1), (700, 3), (1000, 2), (4288, 2)])
+def test_gp_batched_matches_closed_form(handle, n, batch):
+    """The batched Function on `batch` RBF problems of order n (orders off the
+    64 grid, a short last block, a last unpaired block in the inverse's merges
+    at 4288 = 67 blocks): every loss, dK and dy against the FP64 closed form
+    with distinct upstream weights, to 1e-10 relative; dK exactly symmetric."""
+    print(f"case gp batched n={n} batch={batch}", flush=True)
+    rng = np.random.RandomState(100 + n + batch)
+    ks, ys = [], []
+    for q in range(batch):
+        x = torch.tensor(rng.uniform(size=(n, 3)), device=DEV)
+        ks.append(gp_kernel(x, torch.tensor(0.2 + 0.02 * q, dtype=torch.float64, device=DEV))[0].detach())
+        ys.append(rng.standard_normal(n))
+    K = torch.stack(ks).requires_grad_()
+    Y = torch.tensor(np.stack(ys), device=DEV, requires_grad=True)
+    w = torch.tensor(rng.uniform(0.5, 2.0, size=batch), device=DEV)
+    losses = ag.gp_negative_log_likelihood_batched(K, Y, handle)
+    assert losses.shape == (batch,)
+    (w * losses).sum().backward()
+    kbar = K.grad.cpu().numpy()
+    ybar = Y.grad.cpu().numpy()
+    for q in range(batch):
+        k = ks[q].cpu().numpy()
+        loss_ref, kbar_ref, alpha = gp_closed_form(k, ys[q])
+        assert abs(float(losses[q]) - loss_ref) <= 1e-10 * max(1.0, abs(loss_ref)), (q, float(losses[q]), loss_ref)
+        wq = float(w[q])
+        assert rel(kbar[q], wq * kbar_ref) < 1e-10, q
+        assert np.array_equal(kbar[q], kbar[q].T)
+        assert rel(ybar[q], wq * alpha) < 1e-10, q
+
+
+def test_gp_batched_config_n4096(handle):
+    """Four problems of the GP configuration's order (n = 4096, l = 0.2, s = 1,
+    1e-2 jitter): dK and dl against the FP64 closed form to 1e-10, the batched
+    losses and gradients against the per-problem fused Function to 1e-11, and
+    a central difference on l (step 1e-6) for problem 0."""
+    print("case gp batched n=4096", flush=True)
+    n, batch = 4096, 4
+    rng = np.random.RandomState(4)
+    xs = [torch.tensor(rng.uniform(size=(n, 3)), device=DEV) for _ in range(batch)]
+    ys = [rng.standard_normal(n) for _ in range(batch)]
+    l = torch.tensor(0.2, dtype=torch.float64, device=DEV, requires_grad=True)
+    parts = [gp_kernel(x, l) for x in xs]
+    K = torch.stack([p[0] for p in parts])
+    K.retain_grad()
+    Y = torch.tensor(np.stack(ys), device=DEV)
+    losses = ag.gp_negative_log_likelihood_batched(K, Y, handle)
+    losses.sum().backward()
+    kbar = K.grad.detach().cpu().numpy()
+    dl = float(l.grad)
+    dl_ref = 0.0
+    for q in range(batch):
+        k = parts[q][0].detach().cpu().numpy()
+        loss_ref, kbar_ref, _ = gp_closed_form(k, ys[q])
+        assert rel(kbar[q], kbar_ref) < 1e-10, q
+        assert abs(float(losses[q]) - loss_ref) <= 1e-10 * abs(loss_ref)
+        dl_ref += float((kbar_ref * (k - 1e-2 * np.eye(n)) * parts[q][1].cpu().numpy() / 0.2 ** 3).sum())
+        # the per-problem fused Function (ppotrf, ppotrs, diag_logdet, potri)
+        Kd = parts[q][0].detach().clone().requires_grad_()
+        single = ag.gp_negative_log_likelihood(Kd, torch.tensor(ys[q], device=DEV), handle)
+        single.backward()
+        assert abs(float(single.detach()) - float(losses[q])) <= 1e-11 * abs(float(single.detach()))
+        assert rel(kbar[q], Kd.grad.cpu().numpy()) < 1e-11, q
+    assert abs(dl - dl_ref) / abs(dl_ref) < 1e-10
+    step = 1e-6
+
+    def f_of_l(lv):
+        with torch.no_grad():
+            Kp = gp_kernel(xs[0], torch.tensor(lv, dtype=torch.float64, device=DEV))[0].unsqueeze(0)
+            return float(ag.gp_negative_log_likelihood_batched(Kp, Y[:1], handle)[0])
+
+    fd = (f_of_l(0.2 + step) - f_of_l(0.2 - step)) / (2 * step)
+    dl0 = float((kbar[0] * (parts[0][0].detach().cpu().numpy() - 1e-2 * np.eye(n)) * parts[0][1].cpu().numpy()
+                 / 0.2 ** 3).sum())
+    assert abs(fd - dl0) <= 1e-7 * abs(dl0)
+
+
+def test_gp_batched_rejects_bad_input_and_non_pd(handle):
+    K = torch.eye(4, dtype=torch.float64, device=DEV).repeat(3, 1, 1)
+    Y = torch.ones((3, 4), dtype=torch.float64, device=DEV)
+    with pytest.raises(TypeError):
+        ag.gp_negative_log_likelihood_batched(K.float(), Y.float(), handle)
+    with pytest.raises(ValueError):
+        ag.gp_negative_log_likelihood_batched(K.cpu(), Y.cpu(), handle)
+    with pytest.raises(ValueError):
+        ag.gp_negative_log_likelihood_batched(K, Y[:, :3], handle)
+    Kb = K.clone()
+    Kb[1, 2, 2] = -1.0  # problem 1 fails at its third pivot; the others are fine
+    with pytest.raises(RuntimeError, match="problem 1 .*order 3"):
+        ag.gp_negative_log_likelihood_batched(Kb, Y, handle)
+    losses = ag.gp_negative_log_likelihood_batched(K, Y, handle)  # the handle still works
+    assert torch.allclose(losses, torch.full((3,), 2.0, dtype=torch.float64, device=DEV))

@@ -2,7 +2,8 @@
 
 Schedules: the persistent fused kernel for every member, or the blocked path
 (64-column steps with gemmVbatched trailing updates) for the members of more
-than split_blocks blocks; each test pins the schedule it runs.
+than split_blocks blocks, or the left-looking persistent kernel for every
+member; each test pins the schedule it runs.
 Every member is checked against the oracle on its own: the factor against
 oracle potrf, the solution against oracle potrs, the log-determinant against
 2 * the oracle's diag_logdet of its factor; the unused triangle (NaN-poisoned)
@@ -66,8 +67,9 @@ def check_member(vb, i, a, b, raw, dtype, uplo):
     assert abs(ld - 2 * mag) <= ltol * max(1.0, n), (i, n, ld, 2 * mag)
 
 
-# (workgroups per CU, largest first, split_blocks): fused only; blocked above 1 and 3 blocks
-SCHEDULES = [(1, True, 16), (2, False, 16), (1, True, 1), (2, True, 3)]
+# (workgroups per CU, largest first, split_blocks[, left-looking]): fused only; blocked above 1 and 3
+# blocks; the left-looking kernel for every member
+SCHEDULES = [(1, True, 16), (2, False, 16), (1, True, 1), (2, True, 3), (1, True, 16, True), (2, False, 16, True)]
 
 
 @pytest.mark.parametrize("schedule", SCHEDULES)
@@ -85,14 +87,16 @@ def test_batched_members_match_oracle(handle, dtype, uplo, nrhs, schedule):
         check_member(vb, i, mats[i], rhs[i], raws[i], dtype, uplo)
 
 
-@pytest.mark.parametrize("split", [16, 1])
+@pytest.mark.parametrize("schedule", [(1, True, 16), (1, True, 1), (1, True, 16, True), (2, True, 16, True)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_batched_failures_are_per_member(handle, dtype, split):
+def test_batched_failures_are_per_member(handle, dtype, schedule):
     """A non-PD member reports its pivot (B untouched, logdet NaN); an order
     above max_n reports -1 and is not touched; the others are solved.  With
     split 1 the members above 64 take the blocked path (the failing pivot is in
-    the third block column)."""
-    dd.set_vbatched_schedule(handle, dtype, 1, True, split)
+    the third block column); the last two schedules run the left-looking
+    kernel."""
+    print(f"case batched failures {np.dtype(dtype).name} schedule={schedule}", flush=True)
+    dd.set_vbatched_schedule(handle, dtype, *schedule)
     vb, mats, rhs, raws = make_batch(handle, [130, 70, 5, 40], dtype, "L", 2)
     # member 0: final pivot negative
     raw0 = raws[0].copy()
@@ -175,6 +179,14 @@ def test_batched_invalid_arguments(handle):
     for bad in ((3, True, 4), (1, True, 0), (1, True, 17)):
         with pytest.raises(dd.DdlaError):
             dd.set_vbatched_schedule(handle, np.float64, *bad)
+    with pytest.raises(dd.DdlaError):
+        dd.check_status(dd.lib.ddla_c_vbatched_set_kernel(handle.ptr, 8, 2), "ddla_c_vbatched_set_kernel")
+    dd.set_vbatched_schedule(handle, np.float64, 2, False, 5, True)
+    sched = dd.vbatched_schedule(handle, np.float64)
+    assert sched == {"tuned": True, "workgroups_per_cu": 2, "largest_first": False, "split_blocks": 5,
+                     "left_looking": True}
+    dd.set_vbatched_schedule(handle, np.float64, 1, True, 16)  # back to the right-looking kernels
+    assert not dd.vbatched_schedule(handle, np.float64)["left_looking"]
     vb = dd.VBatch(handle, [4, 4], nrhs=1)
     with pytest.raises(dd.DdlaExceptionError):
         dd.pposv_vbatched(vb, "X")
@@ -184,13 +196,16 @@ def test_batched_invalid_arguments(handle):
     dd.pposv_vbatched(empty, "L")  # no-op
 
 
+@pytest.mark.parametrize("left_looking", [False, True])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_full_batch_4096_members(handle, dtype):
+def test_full_batch_4096_members(handle, dtype, left_looking):
     """4096 members, orders uniform in [32, 512] (seed 3), A_i = G G^T / n + 0.5 I,
     one right-hand side each: every member succeeds; a sample of 64 members has
     backward error ||b - A x||_inf / (||A||_inf ||x||_inf) <= 64 n eps and a
     log-determinant within 1e-9 n (FP64) / 1e-3 n (FP32) of numpy's slogdet."""
-    dd.set_vbatched_schedule(handle, dtype, 1, True, 4)  # blocked path above order 256
+    print(f"case batched full {np.dtype(dtype).name} left_looking={left_looking}", flush=True)
+    # blocked path above order 256, or the left-looking kernel for every member
+    dd.set_vbatched_schedule(handle, dtype, 1, True, 4, left_looking)
     rng = np.random.RandomState(3)
     sizes = rng.randint(32, 513, size=4096)
     vb = dd.VBatch(handle, sizes, nrhs=1, dtype=dtype)
@@ -213,3 +228,22 @@ def test_full_batch_4096_members(handle, dtype):
         sign, ref = np.linalg.slogdet(a)
         assert sign > 0
         assert abs(logdet[i] - ref) <= (1e-9 if dtype == np.float64 else 1e-3) * sizes[i], (i, logdet[i], ref)
+
+
+@pytest.mark.parametrize("occupancy", [1, 2])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_left_looking_large_orders(handle, dtype, uplo, occupancy):
+    """The left-looking kernel up to the largest order (1024 = 16 block
+    columns), orders off the 64 grid and a short last block, with 17
+    right-hand sides (passes of 4 columns and a last one of 1) and with one:
+    every member against the oracle, the unused triangle and padding bitwise."""
+    print(f"case batched left-looking large {np.dtype(dtype).name} uplo={uplo} occ={occupancy}", flush=True)
+    dd.set_vbatched_schedule(handle, dtype, occupancy, True, 16, True)
+    sizes = [1024, 1000, 513, 700, 960, 2, 191]
+    for nrhs in (17, 1):
+        vb, mats, rhs, raws = make_batch(handle, sizes, dtype, uplo, nrhs, seed0=40 + nrhs)
+        dd.pposv_vbatched(vb, uplo, max_n=1024)
+        assert (vb.get_info() == 0).all(), vb.get_info()
+        for i in range(len(sizes)):
+            check_member(vb, i, mats[i], rhs[i], raws[i], dtype, uplo)
