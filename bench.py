@@ -478,11 +478,14 @@ def gp_leg(h, dd, args):
 
     def lower_inverse_flops(m):
         # L^-1 by merging inverted 64 x 64 diagonal blocks: X21 = -X22 (L21 X11), two products per merge
+        # (a merge of order-h blocks in 4 / 2 / 1 triangle-aware slices from h = 1024 / 256 / 64:
+        # 2 h^3 (1 + 1 / slices))
         f, hh = 0.0, 64
         while hh < m:
             full = m // (2 * hh)
             o = full * 2 * hh
-            f += full * 4.0 * hh ** 3
+            slices = 4 if hh >= 1024 else 2 if hh >= 256 else 1
+            f += full * 2.0 * hh ** 3 * (1.0 + 1.0 / slices)
             if o + hh < m:
                 n2 = m - o - hh
                 f += 2.0 * n2 * hh * hh + 2.0 * n2 * hh * n2
@@ -508,8 +511,8 @@ def gp_leg(h, dd, args):
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     return {"problems": problems, "n": n, "ms": round(t_b * 1e3, 2), "tflops": round(flops_b / t_b / 1e12, 3),
             "flops_convention": "per problem: forward n^3/3 (potrf) + L^-1 by merging inverted 64 x 64 diagonal "
-                                "blocks (2/3 n^3 at n = 4096) + 2n^2; backward tril(L^-T L^-1) in four k blocks "
-                                "(about n^2/2 * n) + 3n^2",
+                                "blocks in triangle-aware slices (0.42 n^3 at n = 4096) + 2n^2; backward "
+                                "tril(L^-T L^-1) in four k blocks (about n^2/2 * n) + 3n^2",
             "dK_rel_err_vs_fp64_closed_form": rel(kbar_b, kbar_ref),
             "dl_rel_err_vs_fp64_closed_form": abs(dl_b - dl_ref) / abs(dl_ref),
             "timed": "forward + backward of all problems through torch.autograd, K given: the batched Function "

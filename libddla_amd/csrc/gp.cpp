@@ -69,11 +69,18 @@ void check_args(const DdlaHandle_t& h, const char* who, int batch, int n, const 
         throw std::invalid_argument(std::string(who) + ": work must be a 256-byte aligned device buffer");
 }
 
+// Chunks of a merge of order-h blocks: X11 and X22 are lower triangular,
+// so T = L21 X11 is formed in `chunks` column slices (slice c needs rows c w..
+// of X11) and X21 = -X22 T in as many row slices (slice r needs columns ..
+// (r + 1) w of X22): 2 h^3 (1 + 1 / chunks) flops instead of 4 h^3.
+int merge_chunks(long long h) { return h >= 1024 ? 4 : h >= 256 ? 2 : 1; }
+
 // X = L^-1 for every problem, X holding the inverted 64 x 64 diagonal blocks
 // on entry: at level h (64, 128, ...) each aligned pair of order-h blocks
 // becomes one of order 2h by X21 = -X22 (L21 X11), T = L21 X11 staged in M;
 // the pairs of a level are equally spaced (2 h (n + 1) elements apart), so
-// one launch takes them all, plus one for a last, shorter pair.
+// one launch per slice takes them all, plus the products of a last, shorter
+// pair.
 void lower_inverse_batched(hipStream_t s, int n, const double* L, double* X, double* M, const kern::Batch& b)
 {
     const long long ld = n;
@@ -82,10 +89,19 @@ void lower_inverse_batched(hipStream_t s, int n, const double* L, double* X, dou
         const int full = (int)(n / (2 * h));
         if (full > 0) {
             const kern::Batch bi{b.count, b.stride, full, 2 * h * (ld + 1) * (long long)sizeof(double)};
-            kern::gemm<double>(s, 'N', 'N', hh, hh, hh, 1.0, L + h, ld, X, ld, 0.0, M + h, ld, kern::kTriNone, 0,
-                               nullptr, false, bi);
-            kern::gemm<double>(s, 'N', 'N', hh, hh, hh, -1.0, X + h + h * ld, ld, M + h, ld, 0.0, X + h, ld,
-                               kern::kTriNone, 0, nullptr, false, bi);
+            const int S = merge_chunks(h);
+            const long long w = h / S;
+            for (int c = 0; c < S; ++c) {  // T(:, c w..) = L21(:, c w..) X11(c w.., c w..)
+                const long long k0 = c * w;
+                kern::gemm<double>(s, 'N', 'N', hh, (int)w, (int)(h - k0), 1.0, L + h + k0 * ld, ld,
+                                   X + k0 + k0 * ld, ld, 0.0, M + h + k0 * ld, ld, kern::kTriNone, 0, nullptr, false,
+                                   bi);
+            }
+            for (int r = 0; r < S; ++r) {  // X21(r w.., :) = -X22(r w.., ..(r + 1) w) T(..(r + 1) w, :)
+                const long long r0 = r * w;
+                kern::gemm<double>(s, 'N', 'N', (int)w, hh, (int)(r0 + w), -1.0, X + h + r0 + h * ld, ld, M + h, ld,
+                                   0.0, X + h + r0, ld, kern::kTriNone, 0, nullptr, false, bi);
+            }
         }
         const long long o = (long long)full * 2 * h;
         if (o + h < n) {
@@ -104,7 +120,7 @@ double lower_inverse_flops(int n)
     double f = 0;
     for (long long h = kIB; h < n; h *= 2) {
         const long long full = n / (2 * h), o = full * 2 * h;
-        f += full * 4.0 * h * h * h;
+        f += full * 2.0 * h * h * h * (1.0 + 1.0 / merge_chunks(h));
         if (o + h < n) {
             const double n2 = (double)(n - o - h);
             f += 2.0 * n2 * h * h + 2.0 * n2 * h * n2;

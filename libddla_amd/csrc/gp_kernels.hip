@@ -27,7 +27,8 @@ __device__ inline P* moved(P* p, long long bytes)
     return reinterpret_cast<P*>(reinterpret_cast<char*>(const_cast<void*>(static_cast<const void*>(p))) + bytes);
 }
 
-// L_b = K_b (n x n, both column-major with leading dimension n), y copied,
+// L_b = the lower triangle of K_b (n x n, both column-major with leading
+// dimension n; L_b's strict upper triangle is left as it is), y copied,
 // info_b = 0.
 __global__ __launch_bounds__(256) void gp_load_kernel(int n, const double* K, long long k_stride, const double* y,
                                                       long long y_stride, double* L, double* ybuf, int* info,
@@ -36,8 +37,9 @@ __global__ __launch_bounds__(256) void gp_load_kernel(int n, const double* K, lo
     const long long b = blockIdx.y;
     const double* Kb = K + b * k_stride;
     double* Lb = moved(L, b * stride);
-    const long long total = (long long)n * n;
-    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (long long)gridDim.x * 256) Lb[t] = Kb[t];
+    // the lower triangle, column by column (the factorization reads nothing else)
+    for (long long j = blockIdx.x; j < n; j += gridDim.x)
+        for (long long i = j + threadIdx.x; i < n; i += 256) Lb[i + j * n] = Kb[i + j * n];
     if (blockIdx.x == 0) {
         double* yb = moved(ybuf, b * stride);
         for (int i = threadIdx.x; i < n; i += 256) yb[i] = y[b * y_stride + i];
@@ -94,23 +96,34 @@ __global__ __launch_bounds__(256) void gp_loss_kernel(int n, const double* L, co
 }
 
 // Kbar_b(i, j) = g_b (M(max(i, j), min(i, j)) - a_i a_j) / 2 and ybar_b = g_b a
-// (M = tril(K^-1), a = K^-1 y).
+// (M = tril(K^-1), a = K^-1 y).  Workgroup (bi, bj, b) writes the 32 x 32
+// tile (bi, bj) of Kbar_b from the lower tile (max, min) of M_b staged in LDS,
+// so both the reads and the writes run along columns.
 __global__ __launch_bounds__(256) void gp_grad_kernel(int n, const double* M, const double* a, long long stride,
                                                       const double* g, double* Kbar, long long kbar_stride,
                                                       double* ybar, long long ybar_stride)
 {
-    const long long b = blockIdx.y;
+    __shared__ double tile[32][33];
+    const long long b = blockIdx.z;
     const double* Mb = moved(M, b * stride);
     const double* ab = moved(a, b * stride);
     const double gb = g[b];
     double* Kb = Kbar + b * kbar_stride;
-    const long long total = (long long)n * n;
-    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
-        const int i = (int)(t % n), j = (int)(t / n);
-        const double m = i >= j ? Mb[t] : Mb[j + (long long)i * n];
-        Kb[t] = 0.5 * gb * (m - ab[i] * ab[j]);
+    const int bi = blockIdx.x, bj = blockIdx.y;
+    const long long si = (long long)max(bi, bj) * 32, sj = (long long)min(bi, bj) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8)
+        if (si + tx < n && sj + r < n) tile[r][tx] = Mb[(si + tx) + (sj + r) * (long long)n];  // tile[col][row]
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const long long i = (long long)bi * 32 + tx, j = (long long)bj * 32 + r;
+        if (i < n && j < n) {
+            const bool mirrored = bi < bj || (bi == bj && tx < r);
+            const double m = mirrored ? tile[tx][r] : tile[r][tx];
+            Kb[i + j * n] = 0.5 * gb * (m - ab[i] * ab[j]);
+        }
     }
-    if (blockIdx.x == 0 && ybar != nullptr)
+    if (bi == 0 && bj == 0 && ybar != nullptr)
         for (int i = threadIdx.x; i < n; i += 256) ybar[b * ybar_stride + i] = gb * ab[i];
 }
 
@@ -122,8 +135,8 @@ void gp_load(hipStream_t s, int batch, int n, const double* K, long long k_strid
              double* L, double* ybuf, int* info, long long stride)
 {
     if (batch <= 0 || n <= 0) return;
-    gp_load_kernel<<<dim3(grid_x((long long)n * n), (unsigned)batch), 256, 0, s>>>(n, K, k_stride, y, y_stride, L,
-                                                                                   ybuf, info, stride);
+    gp_load_kernel<<<dim3((unsigned)std::min(n, 1024), (unsigned)batch), 256, 0, s>>>(n, K, k_stride, y, y_stride,
+                                                                                      L, ybuf, info, stride);
     check_launch("gp_load");
 }
 
@@ -146,8 +159,9 @@ void gp_grad(hipStream_t s, int batch, int n, const double* M, const double* a, 
              double* Kbar, long long kbar_stride, double* ybar, long long ybar_stride)
 {
     if (batch <= 0 || n <= 0) return;
-    gp_grad_kernel<<<dim3(grid_x((long long)n * n), (unsigned)batch), 256, 0, s>>>(n, M, a, stride, g, Kbar,
-                                                                                   kbar_stride, ybar, ybar_stride);
+    const unsigned tiles = (unsigned)((n + 31) / 32);
+    gp_grad_kernel<<<dim3(tiles, tiles, (unsigned)batch), 256, 0, s>>>(n, M, a, stride, g, Kbar, kbar_stride, ybar,
+                                                                      ybar_stride);
     check_launch("gp_grad");
 }
 

@@ -222,7 +222,8 @@ void diag_adjoint(hipStream_t s, int n, const T* L, long long ldl, double g, T* 
 // region sits b * stride bytes after problem 0's (pointers below are problem
 // 0's); the caller's K, y, Kbar, ybar, loss, info and g have their own
 // strides (elements) or one entry per problem.
-//   gp_load:       L_b = K_b (n x n, leading dimension n), y copied, info_b = 0;
+//   gp_load:       L_b = the lower triangle of K_b (n x n, leading dimension n),
+//                  y copied, info_b = 0;
 //   gp_place_diag: X_b = the block-diagonal matrix of the inverted 64 x 64
 //                  diagonal blocks dinv_b, zero elsewhere;
 //   gp_loss:       loss[b] = z_b^T z_b / 2 + sum_i log L_b(i, i) (also kept at

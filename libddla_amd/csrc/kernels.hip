@@ -516,8 +516,9 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
     const bool a_t = !is_n(ta), b_t = !is_n(tb);
     // 128 x 128 tiles unless there are fewer than 512 of them (two per CU)
     // and 64 x 64 tiles give at least twice as many
-    const long long t128 = (long long)((m + 127) / 128) * ((n + 127) / 128);
-    const long long t64 = (long long)((m + 63) / 64) * ((n + 63) / 64);
+    // (counting every problem and sub-block of a batched launch)
+    const long long t128 = (long long)((m + 127) / 128) * ((n + 127) / 128) * zcount;
+    const long long t64 = (long long)((m + 63) / 64) * ((n + 63) / 64) * zcount;
     if (m >= 128 && n >= 128 && (t128 >= 512 || t64 < 2 * t128))
         launch_gemm_cfg<T, 128, 128, 16, 2>(s, a_t, b_t, p, abort, lookahead, (int)zcount);
     else
