@@ -375,6 +375,7 @@ def batched_leg(h, dd, args):
             "tuned": {"workgroups_per_cu": occ, "largest_first": ordered,
                       "kernel": "left-looking" if sched["left_looking"] else "right-looking",
                       "blocked_path_above_order": None if sched["left_looking"] else 64 * sched["split_blocks"],
+                      "blocked_step_columns": None if sched["left_looking"] else sched["step_columns"],
                       "tune_s": round(tune_s, 2)},
         }
         log(f"batched {name}: {t * 1e3:.3f} ms ({flops / t / 1e12:.2f} TF/s), loop {t_loop * 1e3:.0f} ms")

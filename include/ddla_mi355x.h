@@ -402,15 +402,17 @@ int ddla_c_vbatched_schedule(void* handle, int element_bytes, int* tuned, int* o
  * [1, 16], used only with ordered != 0) on the right-looking kernels; marks it
  * tuned. */
 int ddla_c_vbatched_set_schedule(void* handle, int element_bytes, int occupancy, int ordered, int split_blocks);
-/* Which persistent kernel the schedule runs: *left_looking = 1 for the
- * left-looking kernel (every member in one launch, each block column formed
- * once by one MFMA product; split_blocks unused), 0 for the right-looking
- * kernel with the optional blocked path.  No counterpart in the reference
- * (it has no batched Cholesky). */
-int ddla_c_vbatched_kernel(void* handle, int element_bytes, int* left_looking);
-/* Selects that kernel (0 or 1) for the schedule held (an untuned handle
+/* Which kernels the schedule runs: *kernel = 0 for the right-looking
+ * persistent kernel with the optional blocked path in steps of one 64-column
+ * block column, 2 for the same with steps of two (its trailing updates take
+ * k = 128, half as many), 1 for the left-looking persistent kernel (every
+ * member in one launch, each block column formed once by one MFMA product;
+ * split_blocks unused).  No counterpart in the reference (it has no batched
+ * Cholesky). */
+int ddla_c_vbatched_kernel(void* handle, int element_bytes, int* kernel);
+/* Selects that choice (0, 1 or 2) for the schedule held (an untuned handle
  * first takes 1 workgroup per CU, largest first); marks it tuned. */
-int ddla_c_vbatched_set_kernel(void* handle, int element_bytes, int left_looking);
+int ddla_c_vbatched_set_kernel(void* handle, int element_bytes, int kernel);
 
 /* ---- reverse-mode adjoints (no counterpart in the reference), factor of
  * A = L L^T in the lower triangle (uplo 'L'); each returns synchronized:

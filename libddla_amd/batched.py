@@ -186,28 +186,33 @@ def pposv_vbatched_tune(vb: VBatch, uplo: str = "L", max_n: int = None):
 
 def vbatched_schedule(handle, dtype=np.float64):
     """The batched schedule the handle holds for dtype: {"tuned", "workgroups_per_cu", "largest_first",
-    "split_blocks", "left_looking"} (members of more than split_blocks 64-row blocks take the blocked path with
-    gemmVbatched trailing updates, 16 meaning none; left_looking: every member runs on the left-looking
-    persistent kernel instead, and split_blocks is unused)."""
-    t, occ, ordered, split, ll = (ctypes.c_int() for _ in range(5))
+    "split_blocks", "left_looking", "step_columns"} (members of more than split_blocks 64-row blocks take the
+    blocked path with gemmVbatched trailing updates, 16 meaning none, in steps of step_columns = 64 or 128
+    columns; left_looking: every member runs on the left-looking persistent kernel instead, and split_blocks
+    is unused)."""
+    t, occ, ordered, split, kernel = (ctypes.c_int() for _ in range(5))
     item = np.dtype(dtype).itemsize
     check_status(lib.ddla_c_vbatched_schedule(handle.ptr, item, ctypes.byref(t), ctypes.byref(occ),
                                               ctypes.byref(ordered), ctypes.byref(split)),
                  "ddla_c_vbatched_schedule")
-    check_status(lib.ddla_c_vbatched_kernel(handle.ptr, item, ctypes.byref(ll)), "ddla_c_vbatched_kernel")
+    check_status(lib.ddla_c_vbatched_kernel(handle.ptr, item, ctypes.byref(kernel)), "ddla_c_vbatched_kernel")
     return {"tuned": bool(t.value), "workgroups_per_cu": occ.value, "largest_first": bool(ordered.value),
-            "split_blocks": split.value, "left_looking": bool(ll.value)}
+            "split_blocks": split.value, "left_looking": kernel.value == 1,
+            "step_columns": 128 if kernel.value == 2 else 64}
 
 
 def set_vbatched_schedule(handle, dtype=np.float64, workgroups_per_cu: int = 1, largest_first: bool = True,
-                          split_blocks: int = 16, left_looking: bool = False):
+                          split_blocks: int = 16, left_looking: bool = False, step_columns: int = 64):
     """Pins the batched schedule for dtype without timing (see vbatched_schedule)."""
+    if step_columns not in (64, 128) or (left_looking and step_columns != 64):
+        raise ValueError("step_columns is 64 or 128, and 64 with the left-looking kernel")
     item = np.dtype(dtype).itemsize
     check_status(lib.ddla_c_vbatched_set_schedule(handle.ptr, item, workgroups_per_cu, 1 if largest_first else 0,
                                                   split_blocks),
                  "ddla_c_vbatched_set_schedule")
-    if left_looking:
-        check_status(lib.ddla_c_vbatched_set_kernel(handle.ptr, item, 1), "ddla_c_vbatched_set_kernel")
+    kernel = 1 if left_looking else (2 if step_columns == 128 else 0)
+    if kernel:
+        check_status(lib.ddla_c_vbatched_set_kernel(handle.ptr, item, kernel), "ddla_c_vbatched_set_kernel")
 
 
 class BatchedGraph:

@@ -42,11 +42,11 @@ bool blocked_possible(int max_n) { return max_blocks_for(max_n) > 1; }
 template <class T>
 void launch_run(const DdlaHandle_t& h, hipStream_t s, char uplo, const int* d_n, T* const* d_A, const int* d_lda,
                 int nrhs, T* const* d_B, const int* d_ldb, int* d_info, double* d_logdet, int batch, int max_n,
-                int occupancy, bool ordered, int split_blocks, bool left_looking, void* work)
+                int occupancy, bool ordered, int split_blocks, bool left_looking, int step_blocks, void* work)
 {
     kern::batched_pposv<T>(s, uplo == 'U', batch, nrhs, d_n, d_A, d_lda, d_B, d_ldb, d_info, d_logdet,
                            grid_for_batch(h, batch, occupancy), max_blocks_for(max_n), ordered, occupancy,
-                           split_blocks, work, left_looking);
+                           split_blocks, work, left_looking, step_blocks);
 }
 
 } // namespace
@@ -91,7 +91,8 @@ void pposv_vbatched_run(const DdlaHandle_t& h, char uplo, const int* d_n, T* con
     const auto& t = tuning_of<T>(h);
     launch_run<T>(h, h->stream, uplo, d_n, d_A, d_lda, nrhs, d_B, d_ldb, d_info, d_logdet, batch, max_n,
                   t.tuned ? t.occupancy : 1, t.tuned ? t.ordered : true,
-                  t.tuned ? t.split_blocks : kDefaultSplitBlocks, t.tuned && t.left_looking, backend_work);
+                  t.tuned ? t.split_blocks : kDefaultSplitBlocks, t.tuned && t.left_looking,
+                  t.tuned ? t.step_blocks : 1, backend_work);
 }
 
 } // namespace cholesky_impl
@@ -111,8 +112,8 @@ void pposvVbatched(const DdlaHandle_t& h, char uplo, const int* d_n, T* const* d
 
 // Times every schedule (kernel variant for 1 or 2 workgroups per CU, largest
 // first or index order, and -- largest first only -- the blocked path for the
-// members of more than 2, 3, 4 or 6 blocks, or the left-looking kernel for
-// every member) on copies of the caller's
+// members of more than 2, 3, 4 or 6 blocks in steps of one or two block
+// columns, or the left-looking kernel for every member) on copies of the caller's
 // matrices, so the inputs are left unchanged, and keeps the fastest in the
 // handle for later calls.
 template <class T>
@@ -123,7 +124,7 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
     cholesky_impl::pposv_vbatched_check(h, uplo, batch, nrhs, max_n, d_n, d_A, d_lda, d_B, d_ldb, d_n, d_n);
     auto& tuning = tuning_of<T>(h);
     if (batch == 0) {
-        tuning = DdlaStream::BatchedTuning{true, 1, true, kern::kBatchedMaxBlocks, false};
+        tuning = DdlaStream::BatchedTuning{true, 1, true, kern::kBatchedMaxBlocks, false, 1};
         if (occupancy) *occupancy = 1;
         if (ordered) *ordered = true;
         return;
@@ -190,15 +191,17 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
         bool ordered;
         int split_blocks;
         bool left_looking;
+        int step_blocks;
         float ms;
     };
     constexpr int kNone = kern::kBatchedMaxBlocks;
-    std::vector<Candidate> cands{{1, true, kNone, false, 0}, {2, true, kNone, false, 0},
-                                {1, false, kNone, false, 0}, {2, false, kNone, false, 0}};
+    std::vector<Candidate> cands{{1, true, kNone, false, 1, 0}, {2, true, kNone, false, 1, 0},
+                                {1, false, kNone, false, 1, 0}, {2, false, kNone, false, 1, 0}};
     for (int sb : {2, 3, 4, 6})
         if (sb < max_blocks_for(max_n))
-            for (int occ : {1, 2}) cands.push_back({occ, true, sb, false, 0});
-    for (int occ : {1, 2}) cands.push_back({occ, true, kNone, true, 0});
+            for (int occ : {1, 2})
+                for (int step : {1, 2}) cands.push_back({occ, true, sb, false, step, 0});
+    for (int occ : {1, 2}) cands.push_back({occ, true, kNone, true, 1, 0});
     try {
         for (auto& c : cands) {
             float best = 1e30f;
@@ -207,7 +210,7 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
                 if (nrhs > 0) kern::batched_copy<T>(s, batch, dn, nrhs, const_cast<T* const*>(d_B), d_ldb, pB, dld, mx);
                 DDLA_HIP(hipEventRecord(e0, s));
                 launch_run<T>(h, s, uplo, dn, pA, dld, nrhs, pB, dld, info, logdet, batch, max_n, c.occupancy,
-                              c.ordered, c.split_blocks, c.left_looking, work);
+                              c.ordered, c.split_blocks, c.left_looking, c.step_blocks, work);
                 DDLA_HIP(hipEventRecord(e1, s));
                 DDLA_HIP(hipEventSynchronize(e1));
                 float ms = 0;
@@ -228,7 +231,8 @@ void pposvVbatchedTune(const DdlaHandle_t& h, char uplo, const int* d_n, T* cons
     DDLA_HIP(hipFree(base));
     const auto best = std::min_element(cands.begin(), cands.end(),
                                        [](const Candidate& a, const Candidate& b) { return a.ms < b.ms; });
-    tuning = DdlaStream::BatchedTuning{true, best->occupancy, best->ordered, best->split_blocks, best->left_looking};
+    tuning = DdlaStream::BatchedTuning{true,          best->occupancy,    best->ordered,
+                                       best->split_blocks, best->left_looking, best->step_blocks};
     if (occupancy) *occupancy = best->occupancy;
     if (ordered) *ordered = best->ordered;
 }
@@ -257,7 +261,7 @@ void* pposvVbatchedGraph(const DdlaHandle_t& h, char uplo, const int* d_n, T* co
         DDLA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         DDLA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         launch_run<T>(h, cs, uplo, d_n, d_A, d_lda, nrhs, d_B, d_ldb, d_info, d_logdet, batch, max_n, t.occupancy,
-                      t.ordered, t.split_blocks, t.left_looking, g->work);
+                      t.ordered, t.split_blocks, t.left_looking, t.step_blocks, g->work);
         DDLA_HIP(hipStreamEndCapture(cs, &graph));
         DDLA_HIP(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
         DDLA_HIP(hipGraphDestroy(graph));

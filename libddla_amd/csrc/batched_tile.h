@@ -446,11 +446,15 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_pposv_kernel(BatchArgs<
 // ---------------------------------------------------------------------------
 // Blocked path for the larger members: the members with more than tb blocks
 // (the leading count entries of order[], which is sorted by block count) are
-// factored one 64-column block column per step: blocked_panel_kernel factors
-// every such member's diagonal block (and keeps D_j); one gemmVbatched launch
-// forms all their panels L(r, j) = A(r, j) D_j^T in place, and one more
-// applies all their trailing updates (arrays from blocked_prep_kernel); then
-// blocked_solve_kernel solves them.  Each member's D_j is kept at
+// factored in steps of one or two 64-column block columns: per block column
+// j, blocked_panel_kernel factors every such member's diagonal block (and
+// keeps D_j) and one gemmVbatched launch forms all their panels L(r, j) =
+// A(r, j) D_j^T in place; inside a two-column step the first block column
+// then updates only the second, and the step's last block column applies the
+// whole step's panel to the trailing matrices in one more launch (k = 64 or
+// 128; arrays from blocked_prep_kernel), so a two-column step reads and
+// writes the trailing matrices half as often; then blocked_solve_kernel
+// solves them.  Each member's D_j is kept at
 // dinv + (b * max_blocks + j) * 64 * 64 (b = its position in order[]) and its
 // per-block log sums at logparts[item * kMaxBlocks + j].
 // ---------------------------------------------------------------------------
@@ -522,21 +526,25 @@ __global__ __launch_bounds__(kThreads) void blocked_panel_kernel(BlockedArgs<T> 
     }
 }
 
-// Arrays of step j's two gemmVbatched launches: for b < *count with a
-// successful member of order > 64 (j + 1): m[b] = n - 64 (j + 1) (rows below
-// the diagonal block), k[b] = 64, ld[b] = lda, ld64[b] = 64, P[b] = the stored
-// address of L-view element (64 (j + 1), 64 j) (the panel), C[b] that of
-// (64 (j + 1), 64 (j + 1)) (the trailing matrix), D[b] = D_j; the panel launch
-// is m x 64 (lower: L = A D_j^T) or 64 x m (upper: the stored panel U = D_j U)
-// -- pm[b], pn[b].  m[b] = pm[b] = pn[b] = 0 (no work) otherwise.
+// Arrays of block column j's gemmVbatched launches, in a step of block
+// columns J .. (J <= j): for b < *count with a successful member of order >
+// 64 (j + 1): m[b] = n - 64 (j + 1) (rows below the diagonal block), k[b] =
+// 64, ld[b] = lda, ld64[b] = 64, P[b] = the stored address of L-view element
+// (64 (j + 1), 64 j) (block column j's panel), Q[b] that of (64 (j + 1),
+// 64 J) (the step's panel, kq[b] = 64 (j + 1 - J) columns), C[b] that of
+// (64 (j + 1), 64 (j + 1)) (the trailing matrix), D[b] = D_j.  The panel
+// launch is m x 64 (lower: L = A D_j^T) or 64 x m (upper: the stored panel
+// U = D_j U) -- pm[b], pn[b]; the update of block column j + 1 alone is
+// m x min(64, m) (lower) or its mirror (upper) -- cm[b], cn[b].  Every size
+// is 0 (no work) otherwise.
 template <class T>
 struct StepArrays {
-    int *m, *k, *ld, *ld64, *pm, *pn;
-    T **P, **C, **D;
+    int *m, *k, *kq, *ld, *ld64, *pm, *pn, *cm, *cn;
+    T **P, **Q, **C, **D;
 };
 
 template <class T>
-__global__ void blocked_prep_kernel(BlockedArgs<T> a, int j, StepArrays<T> s)
+__global__ void blocked_prep_kernel(BlockedArgs<T> a, int j, int J, StepArrays<T> s)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= a.batch) return;
@@ -544,21 +552,26 @@ __global__ void blocked_prep_kernel(BlockedArgs<T> a, int j, StepArrays<T> s)
     if (b < *a.count) {
         const int item = a.order[b];
         const int n = a.n[item];
-        const long long r0 = (long long)kB * (j + 1), j0 = (long long)kB * j;
+        const long long r0 = (long long)kB * (j + 1), j0 = (long long)kB * j, q0 = (long long)kB * J;
         if (n > r0 && a.info[item] == 0) {
             const long long lda = a.lda[item];
             T* A = a.A[item];
             mm = n - (int)r0;
             s.ld[b] = (int)lda;
             s.P[b] = a.upper ? A + j0 + r0 * lda : A + r0 + j0 * lda;
+            s.Q[b] = a.upper ? A + q0 + r0 * lda : A + r0 + q0 * lda;
             s.C[b] = A + r0 + r0 * lda;
             s.D[b] = a.dinv + ((long long)b * a.max_blocks + j) * kB * kB;
         }
     }
+    const int c64 = mm < kB ? mm : kB;
     s.m[b] = mm;
     s.pm[b] = a.upper ? (mm > 0 ? kB : 0) : mm;
     s.pn[b] = a.upper ? mm : (mm > 0 ? kB : 0);
+    s.cm[b] = a.upper ? c64 : mm;
+    s.cn[b] = a.upper ? mm : c64;
     s.k[b] = kB;
+    s.kq[b] = kB * (j + 1 - J);
     s.ld64[b] = kB;
 }
 

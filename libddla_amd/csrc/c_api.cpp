@@ -698,28 +698,29 @@ int ddla_c_vbatched_set_schedule(void* handle, int element_bytes, int occupancy,
             split_blocks > ddla::kern::kBatchedMaxBlocks)
             return DDLA_C_STATUS_INVALID_VALUE;
         h->batched_tuning[element_bytes == 8 ? 1 : 0] =
-            ddla::DdlaStream::BatchedTuning{true, occupancy, ordered != 0, split_blocks, false};
+            ddla::DdlaStream::BatchedTuning{true, occupancy, ordered != 0, split_blocks, false, 1};
         return DDLA_C_STATUS_SUCCESS;
     });
 }
 
-int ddla_c_vbatched_kernel(void* handle, int element_bytes, int* left_looking)
+int ddla_c_vbatched_kernel(void* handle, int element_bytes, int* kernel)
 {
     return run(handle, [&](ddla::DdlaHandle_t h) -> int {
         if (element_bytes != 4 && element_bytes != 8) return DDLA_C_STATUS_INVALID_VALUE;
-        if (left_looking) *left_looking = h->batched_tuning[element_bytes == 8 ? 1 : 0].left_looking ? 1 : 0;
+        const auto& t = h->batched_tuning[element_bytes == 8 ? 1 : 0];
+        if (kernel) *kernel = t.left_looking ? 1 : (t.step_blocks == 2 ? 2 : 0);
         return DDLA_C_STATUS_SUCCESS;
     });
 }
 
-int ddla_c_vbatched_set_kernel(void* handle, int element_bytes, int left_looking)
+int ddla_c_vbatched_set_kernel(void* handle, int element_bytes, int kernel)
 {
     return run(handle, [&](ddla::DdlaHandle_t h) -> int {
-        if ((element_bytes != 4 && element_bytes != 8) || (left_looking != 0 && left_looking != 1))
-            return DDLA_C_STATUS_INVALID_VALUE;
+        if ((element_bytes != 4 && element_bytes != 8) || kernel < 0 || kernel > 2) return DDLA_C_STATUS_INVALID_VALUE;
         auto& t = h->batched_tuning[element_bytes == 8 ? 1 : 0];
-        if (!t.tuned) t = ddla::DdlaStream::BatchedTuning{true, 1, true, ddla::kern::kBatchedMaxBlocks, false};
-        t.left_looking = left_looking != 0;
+        if (!t.tuned) t = ddla::DdlaStream::BatchedTuning{true, 1, true, ddla::kern::kBatchedMaxBlocks, false, 1};
+        t.left_looking = kernel == 1;
+        t.step_blocks = kernel == 2 ? 2 : 1;
         return DDLA_C_STATUS_SUCCESS;
     });
 }

@@ -58,6 +58,7 @@ public:
         bool ordered = true;
         int split_blocks = 16;  // members of more blocks take the blocked path (16: none)
         bool left_looking = false;  // every member on the left-looking persistent kernel
+        int step_blocks = 1;        // 64-column block columns per step of the blocked path (1 or 2)
     };
     BatchedTuning batched_tuning[2];  // [0] float, [1] double
 

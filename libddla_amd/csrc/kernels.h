@@ -159,7 +159,9 @@ void stage_sweeps(hipStream_t s, bool upper, int n, int nrhs, const T* L, long l
 // of them in one gemm_vbatched launch per step, then their solves -- and the
 // persistent launch takes the rest.  work holds batched_work_bytes(sizeof(T),
 // batch, grid, max_blocks, blocked) bytes, blocked = whether split_blocks may be
-// used.  left_looking: every member on the left-looking persistent kernel
+// used; step_blocks (1 or 2): 64-column block columns per step of the blocked
+// path (with 2 its trailing updates take k = 128, half as many of them).
+// left_looking: every member on the left-looking persistent kernel
 // instead (batched_ll_kernel: block columns formed once by one MFMA product
 // each; split_blocks is then ignored).
 constexpr int kBatchedMaxBlocks = 16;
@@ -167,7 +169,7 @@ size_t batched_work_bytes(size_t elem, int batch, int grid, int max_blocks, bool
 template <class T>
 void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
                    T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
-                   int occupancy, int split_blocks, void* work, bool left_looking = false);
+                   int occupancy, int split_blocks, void* work, bool left_looking = false, int step_blocks = 1);
 // Batched copy of matrix i (rows n[i], columns cols or n[i] when cols < 0)
 // from src[i] (leading dimension lds[i]) to dst[i] (leading dimension ldd[i]).
 template <class T>

@@ -718,13 +718,13 @@ namespace {
 size_t pad256(size_t b) { return (b + 255) / 256 * 256; }
 
 // [count | D_j of every member (batch x max_blocks x 64 x 64) | log sums
-// (batch x 16 doubles) | m | k | ld | ld64 | pm | pn | P | C | D] of the
-// blocked path
+// (batch x 16 doubles) | m | k | kq | ld | ld64 | pm | pn | cm | cn | P | Q | C
+// | D] of the blocked path
 size_t blocked_bytes(size_t elem, int batch, int max_blocks)
 {
     return 256 + pad256((size_t)batch * max_blocks * 64 * 64 * elem) +
-           pad256((size_t)batch * batched_detail::kMaxBlocks * sizeof(double)) + 6 * pad256((size_t)batch * sizeof(int)) +
-           3 * pad256((size_t)batch * sizeof(void*));
+           pad256((size_t)batch * batched_detail::kMaxBlocks * sizeof(double)) + 9 * pad256((size_t)batch * sizeof(int)) +
+           4 * pad256((size_t)batch * sizeof(void*));
 }
 
 size_t fused_bytes(size_t elem, int batch, int grid, int max_blocks)
@@ -765,7 +765,7 @@ __global__ __launch_bounds__(256) void batched_copy_kernel(int batch, const int*
 template <class T>
 void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n, T* const* A, const int* lda,
                    T* const* B, const int* ldb, int* info, double* logdet, int grid, int max_blocks, bool ordered,
-                   int occupancy, int split_blocks, void* work, bool left_looking)
+                   int occupancy, int split_blocks, void* work, bool left_looking, int step_blocks)
 {
     using namespace batched_detail;
     if (batch <= 0) return;
@@ -803,12 +803,12 @@ void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n,
         double* logparts = reinterpret_cast<double*>(bw);
         bw += pad256((size_t)batch * kMaxBlocks * sizeof(double));
         StepArrays<T> sa;
-        int** ints[] = {&sa.m, &sa.k, &sa.ld, &sa.ld64, &sa.pm, &sa.pn};
+        int** ints[] = {&sa.m, &sa.k, &sa.kq, &sa.ld, &sa.ld64, &sa.pm, &sa.pn, &sa.cm, &sa.cn};
         for (int** q : ints) {
             *q = reinterpret_cast<int*>(bw);
             bw += pad256((size_t)batch * sizeof(int));
         }
-        T*** ptrs[] = {&sa.P, &sa.C, &sa.D};
+        T*** ptrs[] = {&sa.P, &sa.Q, &sa.C, &sa.D};
         for (T*** q : ptrs) {
             *q = reinterpret_cast<T**>(bw);
             bw += pad256((size_t)batch * sizeof(void*));
@@ -816,11 +816,13 @@ void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n,
         count_large_kernel<<<1, 256, 0, s>>>(batch, max_blocks, n, split_blocks, count);
         check_launch("batched_pposv");
         BlockedArgs<T> ba{batch, upper ? 1 : 0, max_blocks, nrhs, n, A, lda, ldb, info, order, count, dinv, logparts};
+        const int S = step_blocks == 2 ? 2 : 1;
         for (int j = 0; j < max_blocks; ++j) {
             blocked_panel_kernel<T><<<batch, kThreads, 0, s>>>(ba, j);
             check_launch("batched_pposv");
             if (j + 1 < max_blocks) {
-                blocked_prep_kernel<T><<<(batch + 255) / 256, 256, 0, s>>>(ba, j, sa);
+                const int J = j - j % S;  // the step's first block column
+                blocked_prep_kernel<T><<<(batch + 255) / 256, 256, 0, s>>>(ba, j, J, sa);
                 check_launch("batched_pposv");
                 const int mx = (max_blocks - 1 - j) * 64;
                 // panel: L(r, j) = A(r, j) D_j^T in place (lower), U = D_j U in place (upper)
@@ -830,9 +832,16 @@ void batched_pposv(hipStream_t s, bool upper, int batch, int nrhs, const int* n,
                 else
                     gemm_vbatched<T>(s, 'N', 'T', sa.pm, sa.pn, sa.k, T(1), sa.P, sa.ld, sa.D, sa.ld64, T(0), sa.P,
                                      sa.ld, batch, mx, 64);
-                // trailing update
-                gemm_vbatched<T>(s, upper ? 'T' : 'N', upper ? 'N' : 'T', sa.m, sa.m, sa.k, T(-1), sa.P, sa.ld, sa.P,
-                                 sa.ld, T(1), sa.C, sa.ld, batch, mx, mx, upper ? kTriUpper : kTriLower);
+                if ((j + 1) % S != 0) {
+                    // the step's next block column alone, by this block column
+                    gemm_vbatched<T>(s, upper ? 'T' : 'N', upper ? 'N' : 'T', sa.cm, sa.cn, sa.k, T(-1), sa.P, sa.ld,
+                                     sa.P, sa.ld, T(1), sa.C, sa.ld, batch, upper ? 64 : mx, upper ? mx : 64,
+                                     upper ? kTriUpper : kTriLower);
+                } else {
+                    // trailing update by the whole step's panel (k = 64 (j + 1 - J))
+                    gemm_vbatched<T>(s, upper ? 'T' : 'N', upper ? 'N' : 'T', sa.m, sa.m, sa.kq, T(-1), sa.Q, sa.ld,
+                                     sa.Q, sa.ld, T(1), sa.C, sa.ld, batch, mx, mx, upper ? kTriUpper : kTriLower);
+                }
             }
         }
         blocked_solve_kernel<T><<<batch, kThreads, 0, s>>>(ba, B, logdet);
@@ -1348,7 +1357,7 @@ void convert(hipStream_t s, int m, int n, const S* in, long long ldi, D* out, lo
     template int* panel_error_word<T>(int, void*);                                                            \
     template void panel_factor<T>(hipStream_t, bool, int, T*, long long, int, int, int*, void*, const Batch&); \
     template void batched_pposv<T>(hipStream_t, bool, int, int, const int*, T* const*, const int*, T* const*,  \
-                                   const int*, int*, double*, int, int, bool, int, int, void*, bool);          \
+                                   const int*, int*, double*, int, int, bool, int, int, void*, bool, int);     \
     template void batched_copy<T>(hipStream_t, int, const int*, int, T* const*, const int*, T* const*,         \
                                   const int*, int);                                                            \
     template void trtri_diag_blocks<T>(hipStream_t, bool, int, const T*, long long, T*, bool, const Batch&);  \

@@ -69,7 +69,8 @@ def check_member(vb, i, a, b, raw, dtype, uplo):
 
 # (workgroups per CU, largest first, split_blocks[, left-looking]): fused only; blocked above 1 and 3
 # blocks; the left-looking kernel for every member
-SCHEDULES = [(1, True, 16), (2, False, 16), (1, True, 1), (2, True, 3), (1, True, 16, True), (2, False, 16, True)]
+SCHEDULES = [(1, True, 16), (2, False, 16), (1, True, 1), (2, True, 3), (1, True, 16, True), (2, False, 16, True),
+             (1, True, 1, False, 128), (2, True, 2, False, 128)]
 
 
 @pytest.mark.parametrize("schedule", SCHEDULES)
@@ -87,7 +88,8 @@ def test_batched_members_match_oracle(handle, dtype, uplo, nrhs, schedule):
         check_member(vb, i, mats[i], rhs[i], raws[i], dtype, uplo)
 
 
-@pytest.mark.parametrize("schedule", [(1, True, 16), (1, True, 1), (1, True, 16, True), (2, True, 16, True)])
+@pytest.mark.parametrize("schedule", [(1, True, 16), (1, True, 1), (1, True, 16, True), (2, True, 16, True),
+                                      (1, True, 1, False, 128)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batched_failures_are_per_member(handle, dtype, schedule):
     """A non-PD member reports its pivot (B untouched, logdet NaN); an order
@@ -180,11 +182,16 @@ def test_batched_invalid_arguments(handle):
         with pytest.raises(dd.DdlaError):
             dd.set_vbatched_schedule(handle, np.float64, *bad)
     with pytest.raises(dd.DdlaError):
-        dd.check_status(dd.lib.ddla_c_vbatched_set_kernel(handle.ptr, 8, 2), "ddla_c_vbatched_set_kernel")
+        dd.check_status(dd.lib.ddla_c_vbatched_set_kernel(handle.ptr, 8, 3), "ddla_c_vbatched_set_kernel")
+    with pytest.raises(ValueError):
+        dd.set_vbatched_schedule(handle, np.float64, 1, True, 4, False, 96)
     dd.set_vbatched_schedule(handle, np.float64, 2, False, 5, True)
     sched = dd.vbatched_schedule(handle, np.float64)
     assert sched == {"tuned": True, "workgroups_per_cu": 2, "largest_first": False, "split_blocks": 5,
-                     "left_looking": True}
+                     "left_looking": True, "step_columns": 64}
+    dd.set_vbatched_schedule(handle, np.float64, 1, True, 3, False, 128)
+    sched = dd.vbatched_schedule(handle, np.float64)
+    assert sched["step_columns"] == 128 and not sched["left_looking"] and sched["split_blocks"] == 3
     dd.set_vbatched_schedule(handle, np.float64, 1, True, 16)  # back to the right-looking kernels
     assert not dd.vbatched_schedule(handle, np.float64)["left_looking"]
     vb = dd.VBatch(handle, [4, 4], nrhs=1)
@@ -196,16 +203,17 @@ def test_batched_invalid_arguments(handle):
     dd.pposv_vbatched(empty, "L")  # no-op
 
 
-@pytest.mark.parametrize("left_looking", [False, True])
+@pytest.mark.parametrize("kernel", ["blocked", "left_looking", "blocked_128"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_full_batch_4096_members(handle, dtype, left_looking):
+def test_full_batch_4096_members(handle, dtype, kernel):
     """4096 members, orders uniform in [32, 512] (seed 3), A_i = G G^T / n + 0.5 I,
     one right-hand side each: every member succeeds; a sample of 64 members has
     backward error ||b - A x||_inf / (||A||_inf ||x||_inf) <= 64 n eps and a
     log-determinant within 1e-9 n (FP64) / 1e-3 n (FP32) of numpy's slogdet."""
-    print(f"case batched full {np.dtype(dtype).name} left_looking={left_looking}", flush=True)
-    # blocked path above order 256, or the left-looking kernel for every member
-    dd.set_vbatched_schedule(handle, dtype, 1, True, 4, left_looking)
+    print(f"case batched full {np.dtype(dtype).name} kernel={kernel}", flush=True)
+    # blocked path above order 256 (64- or 128-column steps), or the left-looking kernel for every member
+    dd.set_vbatched_schedule(handle, dtype, 1, True, 4, kernel == "left_looking",
+                             128 if kernel == "blocked_128" else 64)
     rng = np.random.RandomState(3)
     sizes = rng.randint(32, 513, size=4096)
     vb = dd.VBatch(handle, sizes, nrhs=1, dtype=dtype)
