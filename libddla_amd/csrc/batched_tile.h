@@ -667,7 +667,7 @@ template <class T, int NC, int OCC>
 __device__ __attribute__((noinline)) void solve_member_ll(const T* A, View v, int n, int nb, T* Bm, long long ldb,
                                                           int nrhs, const T* dinv, T* y)
 {
-    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    const int tid = threadIdx.x;
     A = uniform_ptr(A);
     Bm = uniform_ptr(Bm);
     dinv = uniform_ptr(dinv);
@@ -685,62 +685,69 @@ __device__ __attribute__((noinline)) void solve_member_ll(const T* A, View v, in
             y[e] = c < nc ? Bm[i + (long long)(c0 + c) * ldb] : T(0);
         }
         __syncthreads();
+        // Lane layout of a step: wavefront w, lane l handles row 16 w + l / 4 of a
+        // 64-row pass and the quarter k in [16 (l % 4), 16 (l % 4) + 16) of the
+        // 64-long dot products, so each lane has 16 independent loads in flight;
+        // the four quarters meet by two fixed-order cross-lane sums.
+        const int lrow = 16 * (tid >> 6) + ((tid & 63) >> 2), kq = 16 * (tid & 3);
+        // sum over the four lanes of a row, in a fixed order
+        auto quarter_sum = [&](T x) {
+            x += __shfl_xor(x, 1);
+            return x + __shfl_xor(x, 2);
+        };
+        // z(r) = sum_k D(r, k) y_j(k) (or D(k, r)) for the block's rows r < wj, written over y_j
+        auto diag_product = [&](const T* D, int j0, int wj, bool transposed) {
+            T dv[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) dv[t] = transposed ? D[kq + t + kB * lrow] : D[lrow + kB * (kq + t)];
+            T z[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                T acc = T(0);
+#pragma unroll
+                for (int t = 0; t < 16; ++t) acc = fma(dv[t], kq + t < wj ? y[(j0 + kq + t) * NC + c] : T(0), acc);
+                z[c] = quarter_sum(acc);
+            }
+            __syncthreads();
+            if (lrow < wj && (tid & 3) == 0) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    if (c < nc) y[(j0 + lrow) * NC + c] = z[c];
+            }
+            __syncthreads();
+        };
+        // y_i -= sum_k L(row i, k) y_j(k) for rows i0 <= i < i1 (li(i): the row's
+        // first element, ks: the stride along it)
+        auto rows_update = [&](int i0, int i1, int j0, int wj, bool below) {
+            for (int base = i0; base < i1; base += 64) {
+                const int i = base + lrow;
+                const bool row = i < i1;
+                const T* li = below ? (v.upper ? A + j0 + (long long)i * lda : A + i + (long long)j0 * lda)
+                                    : (v.upper ? A + i + (long long)j0 * lda : A + j0 + (long long)i * lda);
+                const long long ks = below ? (v.upper ? 1 : lda) : (v.upper ? lda : 1);
+                T lv[16];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) lv[t] = row && kq + t < wj ? li[(kq + t) * ks] : T(0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    T acc = T(0);
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) acc = fma(lv[t], kq + t < wj ? y[(j0 + kq + t) * NC + c] : T(0), acc);
+                    acc = quarter_sum(acc);
+                    if (row && (tid & 3) == 0 && c < nc) y[i * NC + c] -= acc;
+                }
+            }
+            __syncthreads();
+        };
         for (int j = 0; j < nb; ++j) {
             const int j0 = kB * j, wj = min(kB, n - j0);
-            const T* D = dinv + (long long)j * kB * kB;
-            T z = T(0);
-            if (r < wj && q < nc) {
-#pragma unroll 8
-                for (int k = 0; k < wj; ++k) z = fma(D[r + kB * k], y[(j0 + k) * NC + q], z);
-            }
-            __syncthreads();
-            if (r < wj && q < nc) y[(j0 + r) * NC + q] = z;
-            __syncthreads();
-            for (int i = j0 + wj + tid; i < n; i += kThreads) {
-                T acc[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) acc[c] = y[i * NC + c];
-                const T* li = v.upper ? A + j0 + (long long)i * lda : A + i + (long long)j0 * lda;
-                const long long ks = v.upper ? 1 : lda;
-#pragma unroll 16
-                for (int k = 0; k < wj; ++k) {
-                    const T l = li[k * ks];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) acc[c] = fma(-l, y[(j0 + k) * NC + c], acc[c]);
-                }
-#pragma unroll
-                for (int c = 0; c < NC; ++c) y[i * NC + c] = acc[c];
-            }
-            __syncthreads();
+            diag_product(dinv + (long long)j * kB * kB, j0, wj, false);
+            rows_update(j0 + wj, n, j0, wj, true);
         }
         for (int j = nb - 1; j >= 0; --j) {
             const int j0 = kB * j, wj = min(kB, n - j0);
-            const T* D = dinv + (long long)j * kB * kB;
-            T z = T(0);
-            if (r < wj && q < nc) {
-#pragma unroll 8
-                for (int k = 0; k < wj; ++k) z = fma(D[k + kB * r], y[(j0 + k) * NC + q], z);
-            }
-            __syncthreads();
-            if (r < wj && q < nc) y[(j0 + r) * NC + q] = z;
-            __syncthreads();
-            for (int i = tid; i < j0; i += kThreads) {
-                T acc[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) acc[c] = y[i * NC + c];
-                // L(j0 + k, i)
-                const T* li = v.upper ? A + i + (long long)j0 * lda : A + j0 + (long long)i * lda;
-                const long long ks = v.upper ? lda : 1;
-#pragma unroll 16
-                for (int k = 0; k < wj; ++k) {
-                    const T l = li[k * ks];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) acc[c] = fma(-l, y[(j0 + k) * NC + c], acc[c]);
-                }
-#pragma unroll
-                for (int c = 0; c < NC; ++c) y[i * NC + c] = acc[c];
-            }
-            __syncthreads();
+            diag_product(dinv + (long long)j * kB * kB, j0, wj, true);
+            rows_update(0, j0, j0, wj, false);  // L(j0 + k, i) for the rows i above the block
         }
         for (int e = tid; e < n * NC; e += kThreads) {
             const int i = e / NC, c = e % NC;
@@ -897,7 +904,9 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_ll_kernel(BatchArgs<T> 
 
 // Solves and log-determinants of the members factored on the blocked path.
 template <class T>
-__global__ __launch_bounds__(kThreads) void blocked_solve_kernel(BlockedArgs<T> a, T* const* B, double* logdet)
+// (four workgroups per CU: the solve is latency-bound, and its out-of-line
+// instance for this kernel is compiled to that register budget)
+__global__ __launch_bounds__(kThreads, 4) void blocked_solve_kernel(BlockedArgs<T> a, T* const* B, double* logdet)
 {
     __shared__ T part[kSlab * 4 * kB];
     static_assert(kSlab * 4 * kB >= kMaxBlocks * kB * kLLCols, "the solve's right-hand sides must fit");
@@ -912,8 +921,8 @@ __global__ __launch_bounds__(kThreads) void blocked_solve_kernel(BlockedArgs<T> 
     }
     const View v{a.lda[item], a.upper};
     const T* dinv = a.dinv + (long long)b * a.max_blocks * kB * kB;
-    if (a.nrhs == 1) solve_member_ll<T, 1, 1>(a.A[item], v, n, nb, B[item], a.ldb[item], 1, dinv, part);
-    else if (a.nrhs > 1) solve_member_ll<T, kLLCols, 1>(a.A[item], v, n, nb, B[item], a.ldb[item], a.nrhs, dinv, part);
+    if (a.nrhs == 1) solve_member_ll<T, 1, 4>(a.A[item], v, n, nb, B[item], a.ldb[item], 1, dinv, part);
+    else if (a.nrhs > 1) solve_member_ll<T, kLLCols, 4>(a.A[item], v, n, nb, B[item], a.ldb[item], a.nrhs, dinv, part);
     if (threadIdx.x == 0) {
         double sum = 0.0;
         for (int jj = 0; jj < nb; ++jj) sum += a.logparts[(long long)item * kMaxBlocks + jj];
