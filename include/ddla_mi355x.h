@@ -452,7 +452,7 @@ int ddla_d_potri(void* handle, char uplo, int n, const void* L, int ldl, void* A
  * synchronized; backward writes Kbar_b = g[b] (K_b^-1 - a a^T) / 2 (full
  * symmetric, at Kbar + b * kbar_stride) and, unless ybar is null, ybar_b =
  * g[b] a (a = K_b^-1 y_b) for the device upstream gradients g, asynchronously
- * on the handle's stream.  batch in [0, 65535], n >= 1. */
+ * on the handle's stream.  n >= 1, batch >= 0 and batch * (n / 128) <= 65535. */
 int ddla_d_gp_batched_workspace_size(int n, long long* bytes_per_problem);
 int ddla_d_gp_batched_forward(void* handle, int batch, int n, const void* K, long long k_stride, const void* y,
                               long long y_stride, void* work, void* loss, int* info);

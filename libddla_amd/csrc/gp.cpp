@@ -17,6 +17,7 @@
 #include "internal.h"
 #include "kernels.h"
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,9 @@ void check_args(const DdlaHandle_t& h, const char* who, int batch, int n, const 
     require_gpu(h, who);
     if (batch < 0 || batch > 65535) throw std::invalid_argument(std::string(who) + ": batch must be in [0, 65535]");
     if (n < 1) throw std::invalid_argument(std::string(who) + ": the order must be positive");
+    // the merges of L^-1 launch every problem's pairs of a level at once (grid z <= 65535)
+    if ((long long)batch * std::max(1, n / (2 * kIB)) > 65535)
+        throw std::invalid_argument(std::string(who) + ": batch * (n / 128) must not exceed 65535");
     if (batch > 0 && (work == nullptr || (uintptr_t)work % 256 != 0))
         throw std::invalid_argument(std::string(who) + ": work must be a 256-byte aligned device buffer");
 }
@@ -145,6 +149,7 @@ void gpBatchedForward(const DdlaHandle_t& h, int batch, int n, const double* K, 
     if (!K || !y || !loss || !info) throw std::invalid_argument("gpBatchedForward: null device array");
     if ((batch > 1 && k_stride < (long long)n * n) || (batch > 1 && y_stride < n))
         throw std::invalid_argument("gpBatchedForward: problems overlap (stride below the problem size)");
+
     const GpLayout l = gp_layout(n);
     const kern::Batch b{batch, (long long)l.bytes};
     char* base = static_cast<char*>(work);

@@ -9,6 +9,8 @@
     eq. 5.9), relative error <= 1e-10; and central finite differences with
     step 1e-6 on l and on 64 random symmetric entry pairs of K.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -313,5 +315,11 @@ def test_gp_batched_rejects_bad_input_and_non_pd(handle):
     Kb[1, 2, 2] = -1.0  # problem 1 fails at its third pivot; the others are fine
     with pytest.raises(RuntimeError, match="problem 1 .*order 3"):
         ag.gp_negative_log_likelihood_batched(Kb, Y, handle)
+    per = ctypes.c_longlong()
+    assert dd.lib.ddla_d_gp_batched_workspace_size(4096, ctypes.byref(per)) == 0
+    # 2100 problems x 32 pairs of the first merge level exceed one launch: refused before any argument is used
+    with pytest.raises(dd.DdlaError, match="65535"):
+        dd.check_status(dd.lib.ddla_d_gp_batched_forward(handle.ptr, 2100, 4096, None, 4096 * 4096, None, 4096,
+                                                         None, None, None), "ddla_d_gp_batched_forward")
     losses = ag.gp_negative_log_likelihood_batched(K, Y, handle)  # the handle still works
     assert torch.allclose(losses, torch.full((3,), 2.0, dtype=torch.float64, device=DEV))
