@@ -69,6 +69,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TFLOP/s, % peak: FP64 vs mixed-prec pposv n=32768 at 1e-14 bwd err, 1 GPU"
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+FP32_PEAK = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 FACTOR_NAMES = {0: "fp32", 1: "bf16x3", 2: "bf16x6", 3: "fp16x3"}  # pposv_mixed factor argument
 
 
@@ -367,6 +368,13 @@ def batched_leg(h, dd, args):
             "ms": round(t * 1e3, 3),
             "tflops": round(flops / t / 1e12, 3),
             "hbm_gb_s_a_read_write": round(2.0 * a_bytes / t / 1e9, 1),
+            # fractions of both bounds: the matrices read and written once, against
+            # HBM, and the flops against the matrix cores' dense peak of the type
+            "roofline": {"mfma_peak_tflops": FP64_PEAK if dtype == np.float64 else FP32_PEAK,
+                         "mfma_frac": round(flops / t / 1e12 / (FP64_PEAK if dtype == np.float64 else FP32_PEAK), 4),
+                         "hbm_peak_gb_s": 8000.0, "hbm_frac": round(2.0 * a_bytes / t / 1e9 / 8000.0, 4),
+                         "bound": "neither: each member's chain of dependent diagonal-block steps runs on one "
+                                  "workgroup (latency)"},
             "loop_of_plain_calls_ms": round(t_loop * 1e3, 1),
             "speedup_vs_loop": round(t_loop / t, 1),
             "all_info_zero": bool((info == 0).all()),
@@ -511,6 +519,7 @@ def gp_leg(h, dd, args):
     dl_ref = float((kbar_ref * (k - 1e-2 * np.eye(n)) * d2 / 0.2 ** 3).sum())
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     return {"problems": problems, "n": n, "ms": round(t_b * 1e3, 2), "tflops": round(flops_b / t_b / 1e12, 3),
+            "fp64_peak_frac": round(flops_b / t_b / 1e12 / FP64_PEAK, 4),
             "flops_convention": "per problem: forward n^3/3 (potrf) + L^-1 by merging inverted 64 x 64 diagonal "
                                 "blocks in triangle-aware slices (0.42 n^3 at n = 4096) + 2n^2; backward "
                                 "tril(L^-T L^-1) in four k blocks (about n^2/2 * n) + 3n^2",

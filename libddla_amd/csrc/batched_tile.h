@@ -186,10 +186,13 @@ __device__ inline void block_apply(const T* m, long long rs, long long ks, int r
 // blocks below as L(r, j) = A(r, j) D_j^T on the matrix cores, all in place.  Returns 0, or
 // the 1-based global index of a failed pivot (then only L(j, j) is written).
 // logpart (thread 0) receives sum log L(k, k) over the block (fixed-order
-// butterfly).  S, X: kB * kL elements of LDS each; rdiag: kB; s_flag: one int.
+// butterfly) when with_log.  S, X: kB * kL elements of LDS each; rdiag: kB;
+// s_flag: one int.  Without the blocks below, X keeps its scratch above the
+// diagonal (dj still gets the zeros) and no barrier follows the stores: the
+// caller synchronizes before it reads dj or reuses S and X.
 template <class T, bool OUT_OF_LINE = false>
 __device__ inline int factor_block_column(T* A, View v, int n, int nb, int j, T* dj, T* S, T* X, T* rdiag,
-                                          int* s_flag, double& logpart, bool below = true)
+                                          int* s_flag, double& logpart, bool below = true, bool with_log = true)
 {
     const int tid = threadIdx.x;
     const long long j0 = (long long)kB * j, lda = v.lda;
@@ -202,11 +205,16 @@ __device__ inline int factor_block_column(T* A, View v, int n, int nb, int j, T*
     potf2_detail::store_factor<T>(v.upper != 0, wj, S, Ajj, lda);
     if (f != 0) return (int)j0 + f;
     // log of the diagonal: wave 0, fixed-order butterfly
-    if (tid < 64) {
+    if (with_log && tid < 64) {
         double l = tid < wj ? log((double)S[tid * kL + tid]) : 0.0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
         if (tid == 0) logpart = l;
+    }
+    if (!below) {
+        // X is complete (invert_lower ends with a barrier); dj gets the zeros from store_inverse
+        potf2_detail::store_inverse<T>(X, dj);
+        return 0;
     }
     // D_j with exact zeros above the diagonal: kept in X for the panel, saved for the solves
 #pragma unroll
@@ -217,7 +225,7 @@ __device__ inline int factor_block_column(T* A, View v, int n, int nb, int j, T*
     __syncthreads();
     potf2_detail::store_inverse<T>(X, dj);
     // blocks below: L(r, j) = A(r, j) D_j^T
-    for (int r = j + 1; below && r < nb; ++r) {
+    for (int r = j + 1; r < nb; ++r) {
         const long long r0 = (long long)kB * r;
         const int wr = min(kB, n - (int)r0);
         load_view_block<T>(A, v, r0, j0, wr, wj, S);
@@ -586,8 +594,8 @@ __global__ void blocked_prep_kernel(BlockedArgs<T> a, int j, int J, StepArrays<T
 // So each block column is written twice in all, where the right-looking
 // kernels read and write the whole trailing matrix at every step.  The
 // solve then runs as column sweeps (solve_member_ll) and the log-determinant
-// is the fixed-order sum of the blocks' butterflies.  Uplo 'U' runs the same
-// steps on the stored upper triangle (transposed products).
+// is one fixed-order sum over the factor's diagonal per member.  Uplo 'U'
+// runs the same steps on the stored upper triangle (transposed products).
 // ---------------------------------------------------------------------------
 
 // LDS of the left-looking kernel, in elements of T: the larger of the
@@ -759,12 +767,11 @@ __device__ __attribute__((noinline)) void solve_member_ll(const T* A, View v, in
 
 // Block column j of the member (L view v, order n): the left-looking product
 // C = A(j0:n, j) - L(j0:n, 0:j0) L(j, 0:j0)^T in place (j > 0), the diagonal
-// block (factor_block_column: L(j, j), D_j to dj, log sum to logpart), then
-// the panel L(j0 + 64:n, j) = C D_j^T in place.  Returns 0 or the failed
-// pivot (1-based).  smem: ll_lds_elems<T>() elements.
+// block (factor_block_column: L(j, j), D_j to dj), then the panel
+// L(j0 + 64:n, j) = C D_j^T in place.  Returns 0 or the failed pivot
+// (1-based).  smem: ll_lds_elems<T>() elements.
 template <class T, int OCC>
-__device__ inline int ll_block_column(T* A, View v, int n, int nb, int j, T* dj, T* smem, int* s_flag,
-                                      double& logpart)
+__device__ inline int ll_block_column(T* A, View v, int n, int nb, int j, T* dj, T* smem, int* s_flag)
 {
     using gemm_detail::GemmProblem;
     const long long lda = v.lda;
@@ -803,7 +810,8 @@ __device__ inline int ll_block_column(T* A, View v, int n, int nb, int j, T* dj,
     T* S = smem;
     T* X = smem + kB * kL;
     T* rdiag = smem + 2 * kB * kL;
-    const int f = factor_block_column<T, (OCC > 1)>(A, v, n, nb, j, dj, S, X, rdiag, s_flag, logpart, false);
+    double unused = 0.0;
+    const int f = factor_block_column<T, (OCC > 1)>(A, v, n, nb, j, dj, S, X, rdiag, s_flag, unused, false, false);
     if (f != 0) return f;
     __syncthreads();
     if (m > kB) {
@@ -848,6 +856,7 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_ll_kernel(BatchArgs<T> 
 {
     __shared__ T smem[ll_lds_elems<T>()];
     __shared__ int s_item, s_flag;
+    __shared__ double s_log[kThreads / 64];
     const int tid = threadIdx.x;
     T* const slot = a.xinv + (long long)blockIdx.x * a.max_blocks * kB * kB;
 
@@ -877,12 +886,9 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_ll_kernel(BatchArgs<T> 
         T* A = uniform_ptr(a.A[item]);
         const View v{lda, a.upper};
         int fail = 0;
-        double logsum = 0.0;  // meaningful in thread 0
         for (int j = 0; j < nb; ++j) {
-            double l = 0.0;
-            fail = ll_block_column<T, OCC>(A, v, n, nb, j, slot + (long long)j * kB * kB, smem, &s_flag, l);
+            fail = ll_block_column<T, OCC>(A, v, n, nb, j, slot + (long long)j * kB * kB, smem, &s_flag);
             if (fail != 0) break;
-            logsum += l;
         }
         __syncthreads();
         if (fail != 0) {
@@ -892,6 +898,19 @@ __global__ __launch_bounds__(kThreads, OCC) void batched_ll_kernel(BatchArgs<T> 
             }
             continue;
         }
+        // sum log L(k, k) once per member, in a fixed order: thread t takes
+        // k = t, t + 256, ..., then a butterfly per wavefront and the four
+        // wavefronts' sums in order (the factor's stores are visible to the
+        // workgroup after the barrier above)
+        {
+            double l = 0.0;
+            for (int k = tid; k < n; k += kThreads) l += log((double)A[k + (long long)k * lda]);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+            if ((tid & 63) == 0) s_log[tid >> 6] = l;
+        }
+        __syncthreads();
+        const double logsum = (s_log[0] + s_log[1]) + (s_log[2] + s_log[3]);  // used by thread 0
         if (a.nrhs == 1) solve_member_ll<T, 1, OCC>(A, v, n, nb, uniform_ptr(a.B[item]), ldb, 1, slot, smem);
         else if (a.nrhs > 1)
             solve_member_ll<T, kLLCols, OCC>(A, v, n, nb, uniform_ptr(a.B[item]), ldb, a.nrhs, slot, smem);
