@@ -394,22 +394,61 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
         }
         return;
     }
+    // Edge and diagonal tiles: each element masked by the bounds and the
+    // triangle.  FP64: the same rows of blocks as above, all reads of a row
+    // issued before its writes (element by element, each read would also
+    // wait for the previous element's write, one counter covering both on
+    // gfx950): 6% off the batched left-looking factorization, whose every
+    // product starts with a diagonal tile.  FP32 keeps the element loop,
+    // which measured 4% faster there.
+    const int i0 = m0 + ci, j0 = n0 + wn * (BN / 2);
+    auto live = [&](int i, int j) {
+        if (!full && (i >= p.m || j >= p.n)) return false;
+        if (p.tri == kTriLowerD && (long long)i + p.diag < j) return false;
+        if (p.tri == kTriUpperD && (long long)i + p.diag > j) return false;
+        return true;
+    };
+    if constexpr (sizeof(T) == 8) {
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
+        for (int x = 0; x < TM; ++x) {
+            const int i = i0 + x * 16;
+            T cv[TN][4];
+            if (p.beta != T(0)) {
 #pragma unroll
-        for (int y = 0; y < TN; ++y)
+                for (int y = 0; y < TN; ++y)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = m0 + ci + x * 16;
-                const int j = n0 + wn * (BN / 2) + y * 16 + Op::row(lane, r);
-                if (!full && (i >= p.m || j >= p.n)) continue;
-                if (p.tri == kTriLowerD && (long long)i + p.diag < j) continue;
-                if (p.tri == kTriUpperD && (long long)i + p.diag > j) continue;
-                T* c = p.C + i + (long long)j * p.ldc;
-                T v = p.alpha * acc[x][y][r];
-                if (p.beta != T(0)) v += p.beta * *c;
-                *c = v;
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = j0 + y * 16 + Op::row(lane, r);
+                        cv[y][r] = live(i, j) ? p.C[i + (long long)j * p.ldc] : T(0);
+                    }
             }
+#pragma unroll
+            for (int y = 0; y < TN; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = j0 + y * 16 + Op::row(lane, r);
+                    if (!live(i, j)) continue;
+                    T v = p.alpha * acc[x][y][r];
+                    if (p.beta != T(0)) v += p.beta * cv[y][r];
+                    p.C[i + (long long)j * p.ldc] = v;
+                }
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int y = 0; y < TN; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + x * 16;
+                    const int j = j0 + y * 16 + Op::row(lane, r);
+                    if (!live(i, j)) continue;
+                    T* c = p.C + i + (long long)j * p.ldc;
+                    T v = p.alpha * acc[x][y][r];
+                    if (p.beta != T(0)) v += p.beta * *c;
+                    *c = v;
+                }
+    }
 }
 
 template <class T, int BM, int BN, int BK, bool TA, bool TB, int NBUF = 2>
