@@ -148,6 +148,9 @@ def counter_table(pmc_dirs, trace_dir, focus, cus=256, xcds=8):
             if "SQ_VALU_MFMA_BUSY_CYCLES" in mean:
                 derived.append(f"matrix cores busy {mean['SQ_VALU_MFMA_BUSY_CYCLES'] / (4 * cus * per_xcd):.3f} "
                                f"of {4 * cus} SIMDs")
+                if "SQ_VALU_MFMA_COEXEC_CYCLES" in mean:
+                    derived.append(f"cycles with VALU and MFMA executing together "
+                                   f"{mean['SQ_VALU_MFMA_COEXEC_CYCLES']:.0f}")
         if mean.get("SQ_INSTS_MFMA"):
             if "SQ_INSTS_LDS" in mean:
                 derived.append(f"LDS instructions per MFMA {mean['SQ_INSTS_LDS'] / mean['SQ_INSTS_MFMA']:.3f}")
