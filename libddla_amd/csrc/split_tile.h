@@ -375,19 +375,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
         return;
     }
+    // The same in rounds of 16 elements per lane, each masked: all reads of a
+    // round before its writes (element by element, each read would also wait
+    // for the previous element's write, one counter covering both).
+    auto live = [&](int i, int j) { return i < p.m && j < p.n && (!p.tri || (long long)i + p.diag >= j); };
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 8; ++y)
+        for (int h = 0; h < 2; ++h) {
+            const int i = mw + 16 * x + cc;
+            float cv[4][4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = mw + 16 * x + cc;
-                const int j = n0 + 16 * y + rr0 + r;
-                if (i >= p.m || j >= p.n) continue;
-                if (p.tri && (long long)i + p.diag < j) continue;
-                float* c = p.C + i + (long long)j * p.ldc;
-                *c -= update(x, y, r);
-            }
+            for (int y = 0; y < 4; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = n0 + 16 * (4 * h + y) + rr0 + r;
+                    cv[y][r] = live(i, j) ? p.C[i + (long long)j * p.ldc] : 0.0f;
+                }
+#pragma unroll
+            for (int y = 0; y < 4; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = n0 + 16 * (4 * h + y) + rr0 + r;
+                    if (live(i, j)) p.C[i + (long long)j * p.ldc] = cv[y][r] - update(x, 4 * h + y, r);
+                }
+        }
 }
 
 } // namespace split_detail
