@@ -656,9 +656,9 @@ __device__ __attribute__((noinline)) void ll_product(gemm_detail::GemmProblem<T>
     p.vec_ab = __builtin_amdgcn_readfirstlane(q.vec_ab);
     static_assert(BM == 64 || BN == 64, "one tile dimension spans the block column");
     if (BN == 64) {
-        for (int m0 = 0; m0 < p.m; m0 += BM) gemm_detail::gemm_tile<T, BM, BN, 16, TA, TB>(p, m0, 0, smem);
+        for (int m0 = 0; m0 < p.m; m0 += BM) gemm_detail::gemm_tile<T, BM, BN, 16, TA, TB, 2, false>(p, m0, 0, smem);
     } else {
-        for (int n0 = 0; n0 < p.n; n0 += BN) gemm_detail::gemm_tile<T, BM, BN, 16, TA, TB>(p, 0, n0, smem);
+        for (int n0 = 0; n0 < p.n; n0 += BN) gemm_detail::gemm_tile<T, BM, BN, 16, TA, TB, 2, false>(p, 0, n0, smem);
     }
 }
 

@@ -174,7 +174,7 @@ constexpr int lds_elems()
 // (C := alpha * acc + beta * C): starting them at beta * C instead rounds
 // every partial sum at the scale of C, which in FP32 visibly slows the
 // iterative refinement of nearly cancelling Schur complements.
-template <class T, int BM, int BN, int BK, bool TA, bool TB, bool FAST, int NBUF = 2, bool PF2 = false>
+template <class T, int BM, int BN, int BK, bool TA, bool TB, bool FAST, int NBUF = 2, bool PF2 = false, bool BUF = true>
 __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, int n0, T* smem)
 {
     using Op = Mfma<T>;
@@ -220,14 +220,64 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
         if (TB) { j = (v % (BN / VW)) * VW; kk = v / (BN / VW); }
         else { kk = (v % (BK / VW)) * VW; j = v / (BK / VW); }
     };
+    // FAST loads (BUF): the slab's origin in each operand is uniform over the
+    // workgroup (scalar registers) and each thread's place in it a 32-bit
+    // byte offset fixed for the whole tile, so a load needs no per-slab
+    // vector address arithmetic (the caller keeps the leading dimensions at
+    // most 2^20, which bounds the offsets by 2^31).  This took the FP64
+    // trailing update from 51.4 to 53.0 TF/s per launch; the batched
+    // left-looking kernel's small products keep 64-bit addresses (BUF =
+    // false), with which its FP64 run measured 5.31 against 6.0 ms.
+    unsigned a_off[AVT], b_off[BVT];
+    if (FAST && BUF) {
+#pragma unroll
+        for (int e = 0; e < AVT; ++e) {
+            int i, kk;
+            a_coord(e, i, kk);
+            a_off[e] = (unsigned)(TA ? kk + i * (int)p.lda : i + kk * (int)p.lda) * (unsigned)sizeof(T);
+        }
+#pragma unroll
+        for (int e = 0; e < BVT; ++e) {
+            int j, kk;
+            b_coord(e, j, kk);
+            b_off[e] = (unsigned)(TB ? j + kk * (int)p.ldb : kk + j * (int)p.ldb) * (unsigned)sizeof(T);
+        }
+    }
     auto load = [&](int k0, int set) {
+        if (FAST && BUF) {
+            // the tile's coordinates are uniform but may come out of vector
+            // arithmetic (the tile order's square root): made scalar here
+            const int ms = __builtin_amdgcn_readfirstlane(m0), ns = __builtin_amdgcn_readfirstlane(n0);
+            const int ks = __builtin_amdgcn_readfirstlane(k0);
+            const T* a0 = TA ? p.A + ks + (long long)ms * p.lda : p.A + ms + (long long)ks * p.lda;
+            const T* b0 = TB ? p.B + ns + (long long)ks * p.ldb : p.B + ks + (long long)ns * p.ldb;
+            // buffer loads from the slab's origin: the 32-bit per-thread offset
+            // goes straight into the instruction (raw buffer, gfx9 resource word 3)
+            const __amdgpu_buffer_rsrc_t ar = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(a0), 0, 0x7fffffff,
+                                                                                0x00020000);
+            const __amdgpu_buffer_rsrc_t br = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(b0), 0, 0x7fffffff,
+                                                                                0x00020000);
+#pragma unroll
+            for (int e = 0; e < AVT; ++e) {
+                const vec_t v = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b128(ar, a_off[e], 0, 0));
+#pragma unroll
+                for (int q = 0; q < VW; ++q) ra[set][e][q] = v[q];
+            }
+#pragma unroll
+            for (int e = 0; e < BVT; ++e) {
+                const vec_t v = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b128(br, b_off[e], 0, 0));
+#pragma unroll
+                for (int q = 0; q < VW; ++q) rb[set][e][q] = v[q];
+            }
+            return;
+        }
 #pragma unroll
         for (int e = 0; e < AVT; ++e) {
             int i, kk;
             a_coord(e, i, kk);
             const long long gi = m0 + i, gk = k0 + kk;
-            const T* src = TA ? p.A + gk + gi * p.lda : p.A + gi + gk * p.lda;
             if (FAST) {
+                const T* src = TA ? p.A + gk + gi * p.lda : p.A + gi + gk * p.lda;
                 const vec_t v = *reinterpret_cast<const vec_t*>(src);
 #pragma unroll
                 for (int q = 0; q < VW; ++q) ra[set][e][q] = v[q];
@@ -244,8 +294,8 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
             int j, kk;
             b_coord(e, j, kk);
             const long long gj = n0 + j, gk = k0 + kk;
-            const T* src = TB ? p.B + gj + gk * p.ldb : p.B + gk + gj * p.ldb;
             if (FAST) {
+                const T* src = TB ? p.B + gj + gk * p.ldb : p.B + gk + gj * p.ldb;
                 const vec_t v = *reinterpret_cast<const vec_t*>(src);
 #pragma unroll
                 for (int q = 0; q < VW; ++q) rb[set][e][q] = v[q];
@@ -451,13 +501,14 @@ __device__ __forceinline__ void gemm_tile_body(const GemmProblem<T>& p, int m0, 
     }
 }
 
-template <class T, int BM, int BN, int BK, bool TA, bool TB, int NBUF = 2>
+template <class T, int BM, int BN, int BK, bool TA, bool TB, int NBUF = 2, bool BUF = true>
 __device__ __forceinline__ void gemm_tile(const GemmProblem<T>& p, int m0, int n0, T* smem)
 {
     if (p.tri == kTriLowerD && (long long)(m0 + BM - 1) + p.diag < n0) return;
     if (p.tri == kTriUpperD && (long long)m0 + p.diag > n0 + BN - 1) return;
-    const bool fast = p.vec_ab && m0 + BM <= p.m && n0 + BN <= p.n && p.k % BK == 0;
-    if (fast) gemm_tile_body<T, BM, BN, BK, TA, TB, true, NBUF>(p, m0, n0, smem);
+    const bool fast = p.vec_ab && m0 + BM <= p.m && n0 + BN <= p.n && p.k % BK == 0 &&
+                      (!BUF || (p.lda <= (1 << 20) && p.ldb <= (1 << 20)));
+    if (fast) gemm_tile_body<T, BM, BN, BK, TA, TB, true, NBUF, false, BUF>(p, m0, n0, smem);
     else gemm_tile_body<T, BM, BN, BK, TA, TB, false, NBUF>(p, m0, n0, smem);
 }
 
