@@ -252,21 +252,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __shared__ u4_t lds[2 * BF * kFragU4];
     const int nks = p.nks;
     const int rbs = nks * NP * kFragU4;  // 16-byte units per 16-row block
-    const u4_t* gA = p.A + (long long)(mw / 16) * rbs + lane;
-    const u4_t* gB = p.B + (long long)(n0 / 16) * rbs + lane;
+    // Fragment loads through buffer instructions: the slab's origin (per
+    // wavefront for A) goes into the resource, the fragment's place into the
+    // scalar offset and the lane's 16 bytes into the one vector offset that
+    // every load shares, so no load needs vector address arithmetic.
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const char* aw = reinterpret_cast<const char*>(p.A + (long long)((m0 + 64 * wv) / 16) * rbs);
+    const char* bw = reinterpret_cast<const char*>(p.B + (long long)(n0 / 16) * rbs);
+    const unsigned loff = (unsigned)lane * 16u;
+    const int rbb = rbs * 16;                   // bytes per 16-row block
+    constexpr int kFragBytes = kFragU4 * 16;    // one fragment
     u4_t ra[4][NP];
     u4_t rb[BPW];
     auto load_a = [&](int s) {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(aw + (long long)s * NP * kFragBytes), 0, 0x7fffffff, 0x00020000);
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int q = 0; q < NP; ++q) ra[x][q] = gA[x * rbs + (s * NP + q) * kFragU4];
+            for (int q = 0; q < NP; ++q)
+                ra[x][q] = __builtin_bit_cast(u4_t, __builtin_amdgcn_raw_buffer_load_b128(r, loff, x * rbb + q * kFragBytes, 0));
     };
     auto load_b = [&](int s) {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(bw + (long long)s * NP * kFragBytes), 0, 0x7fffffff, 0x00020000);
 #pragma unroll
         for (int e = 0; e < BPW; ++e) {
-            const int f = wave + 4 * e, rbk = f / NP, q = f % NP;
-            rb[e] = gB[rbk * rbs + (s * NP + q) * kFragU4];
+            const int f = wv + 4 * e, rbk = f / NP, q = f % NP;
+            rb[e] = __builtin_bit_cast(u4_t, __builtin_amdgcn_raw_buffer_load_b128(r, loff, rbk * rbb + q * kFragBytes, 0));
         }
     };
     auto store_b = [&](int buf) {
