@@ -581,7 +581,7 @@ void split_panel(hipStream_t s, int mode, bool upper, int m, int kb, const float
     int* exps = reinterpret_cast<int*>(static_cast<char*>(out) + split_fragment_bytes(rows, nks, planes));
     const dim3 grid((unsigned)((rows + 255) / 256), (unsigned)(4 * nks)), block(256);
     if (mode == split_detail::kSplitFp16x3) {
-        split_detail::split_rowexp_kernel<<<(rows + 255) / 256, block, 0, s>>>(m, kb, rows, X, ldx, upper, exps);
+        split_detail::split_rowexp_kernel<<<(rows + 63) / 64, block, 0, s>>>(m, kb, rows, X, ldx, upper, exps);
         check_launch("split_panel");
         split_detail::split_panel_kernel<2, true><<<grid, block, 0, s>>>(m, kb, rows, nks, X, ldx, upper, exps, o);
     } else if (mode == split_detail::kSplitBf16x6) {

@@ -82,16 +82,30 @@ __host__ __device__ constexpr int split_planes(int mode) { return mode == kSplit
 
 // Per-row exponents of the fp16x3 split: exps[r] = e with the largest
 // |X(r, c)|, c < kb, times 2^e in [2^14, 2^15); 0 for a zero or nonfinite
-// largest magnitude and for the padding rows m <= r < rows.  One thread per
-// row (consecutive rows, so every column's loads coalesce).
+// largest magnitude and for the padding rows m <= r < rows.  A workgroup
+// takes 64 rows; thread (r, g) scans column quarter g of row r (consecutive
+// rows, so every column's loads coalesce), and the four partial maxima meet
+// in LDS.  The maximum does not depend on the grouping (fmaxf drops NaNs
+// either way), so the exponents are those of a plain row scan; one thread
+// per row walking all columns took ~270 us per 512-column panel at order
+// 32768, four times the panel's own factorization.
 __global__ __launch_bounds__(256) void split_rowexp_kernel(int m, int kb, int rows, const float* X, long long ldx,
                                                            int upper, int* exps)
 {
-    const int row = blockIdx.x * 256 + threadIdx.x;
-    if (row >= rows) return;
+    __shared__ float part[4][64];
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int row = blockIdx.x * 64 + r;
     float big = 0.0f;
-    if (row < m)
-        for (int c = 0; c < kb; ++c) big = fmaxf(big, fabsf(upper ? X[c + (long long)row * ldx] : X[row + (long long)c * ldx]));
+    if (row < m) {
+        const int q = (kb + 3) / 4, c0 = g * q, c1 = min(kb, c0 + q);
+#pragma unroll 8
+        for (int c = c0; c < c1; ++c)
+            big = fmaxf(big, fabsf(upper ? X[c + (long long)row * ldx] : X[row + (long long)c * ldx]));
+    }
+    part[g][r] = big;
+    __syncthreads();
+    if (g != 0 || row >= rows) return;
+    big = fmaxf(fmaxf(part[0][r], part[1][r]), fmaxf(part[2][r], part[3][r]));
     int e = 0;
     if (big > 0.0f && big <= 3.402823466e38f) {
         int q;
