@@ -168,15 +168,22 @@ __global__ __launch_bounds__(256) void convert_sym_norm_kernel(int n, int lower,
     const long long i0 = (long long)ib * 64, j0 = (long long)jb * 64;
     const int rows = (int)min(64LL, n - i0), cols = (int)min(64LL, n - j0);
     const bool diag = ib == jb;
-    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-        const int r = idx & 63, c = idx >> 6;
-        double v = 0;
-        if (r < rows && c < cols && (!diag || (lower ? r >= c : r <= c))) {
-            const S x = in[(i0 + r) + (j0 + c) * ldi];
-            out[(i0 + r) + (j0 + c) * ldo] = (D)x;
-            v = fabs((double)x);
-        }
-        Ts[r][c] = v;
+    // every read of the thread's 16 elements before its writes (element by
+    // element, each read would also wait for the previous element's write,
+    // one counter covering both)
+    S x[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = threadIdx.x + 256 * e, r = idx & 63, c = idx >> 6;
+        const bool live = r < rows && c < cols && (!diag || (lower ? r >= c : r <= c));
+        x[e] = live ? in[(i0 + r) + (j0 + c) * ldi] : S(0);
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = threadIdx.x + 256 * e, r = idx & 63, c = idx >> 6;
+        const bool live = r < rows && c < cols && (!diag || (lower ? r >= c : r <= c));
+        if (live) out[(i0 + r) + (j0 + c) * ldo] = (D)x[e];
+        Ts[r][c] = live ? fabs((double)x[e]) : 0.0;
     }
     __syncthreads();
     const int t = threadIdx.x;
@@ -204,11 +211,16 @@ __global__ void sym_rowsum_kernel(int n, int nblk, int lower, const double* prow
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int bi = i / 64;
         double s = 0;
+        // unrolled so that several loads are in flight; the sum keeps its order
         if (lower) {
+#pragma unroll 8
             for (int jb = 0; jb <= bi; ++jb) s += prow[(long long)jb * n + i];
+#pragma unroll 8
             for (int ib = bi; ib < nblk; ++ib) s += pcol[(long long)ib * n + i];
         } else {
+#pragma unroll 8
             for (int jb = bi; jb < nblk; ++jb) s += prow[(long long)jb * n + i];
+#pragma unroll 8
             for (int ib = 0; ib <= bi; ++ib) s += pcol[(long long)ib * n + i];
         }
         rowsum[i] = s;
