@@ -234,6 +234,7 @@ __global__ __launch_bounds__(256) void colmax_kernel(int m, const T* M, long lon
     __shared__ double red[256];
     const int j = blockIdx.x;
     double v = 0;
+#pragma unroll 8
     for (int i = threadIdx.x; i < m; i += 256) {
         const double x = fabs((double)M[i + (long long)j * ld]);
         v = (x > v || x != x) ? x : v;
