@@ -421,3 +421,36 @@ def test_local_potrf_argument_rules(handle):
     assert dd.lib.ddla_d_potrf(handle.ptr, b"L", 4, pt, 3, pi) == 6
     assert dd.lib.ddla_d_potrf(handle.ptr, b"L", 4, pt, 4, None) == 6
     assert dd.lib.ddla_d_potrf(handle.ptr, b"X", 4, pt, 4, pi) == 6
+
+
+def test_leading_dimension_beyond_buffer_offsets(handle):
+    """Leading dimensions above 2^20 leave the GEMM tile's buffer-load path
+    (its 32-bit per-thread offsets) for the general one, which computes in the
+    same order: the local FP64 potrf at order 1536 (three 512-column panels,
+    128 x 128 trailing tiles) with lda = 2^20 + 64 must match lda = n bit for
+    bit, and leave the padding rows alone (about 13 GB of device memory)."""
+    import torch
+
+    n, big = 1536, (1 << 20) + 64
+    print(f"case potrf n={n} lda={big}", flush=True)
+    a = orc.spd_matrix(n, seed=97, delta=0.3)
+    info = torch.full((1,), -7, dtype=torch.int32, device="cuda")
+    fn = dd.lib.ddla_d_potrf
+    small = torch.tensor(np.ascontiguousarray(a.T), device="cuda")  # column-major n x n, lda = n
+    torch.cuda.synchronize()
+    assert fn(handle.ptr, b"L", n, ctypes.c_void_p(small.data_ptr()), n, ctypes.c_void_p(info.data_ptr())) == 0
+    handle.synchronize()
+    assert int(info.item()) == 0
+    want = small.cpu().numpy().T
+    wide = torch.full((n, big), -5.0, dtype=torch.float64, device="cuda")  # column-major big x n
+    wide[:, :n] = torch.tensor(np.ascontiguousarray(a.T), device="cuda")
+    torch.cuda.synchronize()
+    assert fn(handle.ptr, b"L", n, ctypes.c_void_p(wide.data_ptr()), big, ctypes.c_void_p(info.data_ptr())) == 0
+    handle.synchronize()
+    assert int(info.item()) == 0
+    got = wide[:, :n].cpu().numpy().T
+    low = np.tril(np.ones((n, n), bool))
+    assert (got[low] == want[low]).all()
+    assert bool((wide[:, n:n + 4096] == -5.0).all()) and bool((wide[:, big - 64:] == -5.0).all())
+    del wide
+    torch.cuda.empty_cache()
